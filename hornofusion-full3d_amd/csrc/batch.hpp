@@ -1,0 +1,94 @@
+// batch.hpp -- host-side container of one or more p' systems resident in HBM.
+//
+// A Batch owns, for every system, the FP64 operand set of cg_3d_mpi
+// (aW, aE, aS, aN, aB, aT, aP and Su, src/mod_solver_3d.f90:485-496) in the
+// compact z-fastest layout, the initial phi, and a padded FP64 copy of phi
+// used to hand the result back in the caller's (-1:n+2) Fortran layout
+// without touching the caller's r/z halos.  Solvers (cg64.hip, mixed.hip)
+// attach their work vectors to the Batch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "eafp_common.hpp"
+#include "../../include/eafp.h"
+
+namespace eafp {
+
+#define EAFP_TRY(expr)                                  \
+    do {                                                \
+        hipError_t e_ = (expr);                         \
+        if (e_ != hipSuccess) return (int)e_;           \
+    } while (0)
+
+#define EAFP_TRY_RC(expr)                               \
+    do {                                                \
+        int rc_ = (expr);                               \
+        if (rc_ != EAFP_OK) return rc_;                 \
+    } while (0)
+
+// Device buffer with RAII release.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    int alloc(size_t count) {
+        if (count == n && p) return EAFP_OK;
+        release();
+        if (count == 0) return EAFP_OK;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) { p = nullptr; return (int)e; }
+        n = count;
+        return EAFP_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct Cg64Work;     // cg64.hip
+struct MixedWork;    // mixed.hip
+
+enum { C_AW = 0, C_AE, C_AS, C_AN, C_AB, C_AT, C_AP, NCOEF };
+
+struct Batch {
+    int nsys = 0;
+    std::vector<SysGeom> geom;           // host copy
+    DevBuf<SysGeom> geom_d;
+    long ncells = 0;
+    int ntiles = 0;
+    std::vector<int> tile_sys_h, tile_start_h;
+    DevBuf<int> tile_sys, tile_start;    // per tile: system id, first local cell
+    DevBuf<double> coef[NCOEF];          // FP64 operand arenas, compact
+    DevBuf<double> b;                    // Su plus the folded wall-halo terms of phi0
+    DevBuf<double> x0;                   // initial phi interior
+    std::vector<DevBuf<double>*> phi_pad;  // padded phi per system (device)
+    std::vector<char> uploaded;
+    DevBuf<double> staging;              // padded staging for uploads
+    hipStream_t stream = nullptr;
+    int grid = 0;                        // blocks per tile-walking launch
+    Cg64Work* cg64 = nullptr;
+    MixedWork* mixed = nullptr;
+
+    ~Batch();
+    int init(int n, const eafp_desc* descs);
+    int upload(int s, const double* const host[8], const double* phi0, int arrays_on_device);
+    int download(int s, double* phi_host, int to_device);
+    long padded_len(int s) const {
+        const SysGeom& g = geom[s];
+        return (long)(g.nr + 4) * (g.nth + 4) * (g.nz + 4);
+    }
+};
+
+int validate_desc(const eafp_desc* d);
+int choose_grid(int ntiles);
+
+}  // namespace eafp

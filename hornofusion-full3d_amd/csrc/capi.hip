@@ -1,0 +1,129 @@
+// capi.hip -- extern "C" entry points of libeafp.so (declared in
+// include/eafp.h).  Argument checks happen before any HIP call, so invalid
+// descriptors are reported the same way with or without a GPU.
+#include <memory>
+#include <mutex>
+
+#include "batch.hpp"
+
+namespace eafp {
+int solve_fp64(Batch& B, int max_iter, double tol, eafp_stats* stats, int chunk);
+int solve_mixed(Batch& B, const eafp_mixed_opts& o, eafp_stats* stats);
+}  // namespace eafp
+
+struct eafp_batch {
+    eafp::Batch B;
+};
+
+namespace {
+
+int device_check() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return EAFP_ERR_NO_DEVICE;
+    return EAFP_OK;
+}
+
+bool same_shape(const eafp_desc& a, const eafp_desc& b) {
+    return a.nr == b.nr && a.ntheta == b.ntheta && a.nz == b.nz;
+}
+
+// Cached single-system batch of the drop-in, keyed by mesh shape.
+std::mutex g_mu;
+std::unique_ptr<eafp_batch> g_cache;
+eafp_desc g_cache_desc{};
+
+}  // namespace
+
+extern "C" {
+
+const char* eafp_version(void) { return "eafp 0.1 (gfx950)"; }
+
+int eafp_cg_solve(const eafp_desc* desc, const double* aW, const double* aE, const double* aS,
+                  const double* aN, const double* aB, const double* aT, const double* aP,
+                  const double* Su, double* phi, int max_iter, double tol, double* residual,
+                  int* n_iter) {
+    int rc = eafp::validate_desc(desc);
+    if (rc != EAFP_OK) return rc;
+    if (!aW || !aE || !aS || !aN || !aB || !aT || !aP || !Su || !phi || !residual || !n_iter)
+        return EAFP_ERR_ARG;
+    if ((rc = device_check()) != EAFP_OK) return rc;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_cache || !same_shape(g_cache_desc, *desc)) {
+        g_cache.reset();
+        auto nb = std::make_unique<eafp_batch>();
+        if ((rc = nb->B.init(1, desc)) != EAFP_OK) return rc;
+        g_cache = std::move(nb);
+        g_cache_desc = *desc;
+    }
+    eafp::Batch& B = g_cache->B;
+    const double* arr[8] = {aW, aE, aS, aN, aB, aT, aP, Su};
+    if ((rc = B.upload(0, arr, phi, 0)) != EAFP_OK) return rc;
+    eafp_stats st{};
+    if ((rc = eafp::solve_fp64(B, max_iter, tol, &st, 0)) != EAFP_OK) return rc;
+    if ((rc = B.download(0, phi, 0)) != EAFP_OK) return rc;
+    *residual = st.residual;
+    *n_iter = st.n_iter;
+    return EAFP_OK;
+}
+
+int eafp_release(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_cache.reset();
+    return EAFP_OK;
+}
+
+int eafp_batch_create(int nsys, const eafp_desc* descs, eafp_batch** out) {
+    if (!out || nsys < 1 || !descs) return EAFP_ERR_ARG;
+    *out = nullptr;
+    for (int s = 0; s < nsys; ++s) {
+        int rc = eafp::validate_desc(&descs[s]);
+        if (rc != EAFP_OK) return rc;
+    }
+    int rc = device_check();
+    if (rc != EAFP_OK) return rc;
+    auto nb = std::make_unique<eafp_batch>();
+    if ((rc = nb->B.init(nsys, descs)) != EAFP_OK) return rc;
+    *out = nb.release();
+    return EAFP_OK;
+}
+
+int eafp_batch_upload(eafp_batch* b, int s, const double* aW, const double* aE, const double* aS,
+                      const double* aN, const double* aB, const double* aT, const double* aP,
+                      const double* Su, const double* phi0, int arrays_on_device) {
+    if (!b) return EAFP_ERR_ARG;
+    const double* arr[8] = {aW, aE, aS, aN, aB, aT, aP, Su};
+    return b->B.upload(s, arr, phi0, arrays_on_device);
+}
+
+int eafp_batch_solve_fp64(eafp_batch* b, int max_iter, double tol, eafp_stats* stats) {
+    if (!b) return EAFP_ERR_ARG;
+    return eafp::solve_fp64(b->B, max_iter, tol, stats, 0);
+}
+
+int eafp_batch_solve_mixed(eafp_batch* b, const eafp_mixed_opts* opts, eafp_stats* stats) {
+    if (!b || !opts) return EAFP_ERR_ARG;
+    if (!(opts->tol > 0.0) || !(opts->inner_tol > 0.0) || opts->max_inner < 1 || opts->max_refine < 1)
+        return EAFP_ERR_ARG;
+    if (opts->coef_format < EAFP_COEF_FP32 || opts->coef_format > EAFP_COEF_FP16) return EAFP_ERR_ARG;
+    return eafp::solve_mixed(b->B, *opts, stats);
+}
+
+int eafp_batch_download(eafp_batch* b, int s, double* phi, int to_device) {
+    if (!b) return EAFP_ERR_ARG;
+    return b->B.download(s, phi, to_device);
+}
+
+int eafp_batch_info(const eafp_batch* b, int* grid, int* tile_cells, long* ncells) {
+    if (!b) return EAFP_ERR_ARG;
+    if (grid) *grid = b->B.grid;
+    if (tile_cells) *tile_cells = eafp::TILE_CELLS;
+    if (ncells) *ncells = b->B.ncells;
+    return EAFP_OK;
+}
+
+int eafp_batch_destroy(eafp_batch* b) {
+    delete b;
+    return EAFP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,12 @@
+"""eafp -- MI355X-native p' (pressure-correction) solver for EAF3D.
+
+The compute path is libeafp.so (hand-written HIP kernels for gfx950 behind
+the C-ABI of include/eafp.h).  This package holds the host-side mirror of
+the reference interface (``solver.cg_3d_mpi``), device-resident batches
+(``solver.Batch``), the seeded synthetic system generator (``systems``) and
+the build step (``build``).
+"""
+from . import systems  # noqa: F401
+from .solver import Batch, Mesh, cg_3d_mpi, release  # noqa: F401
+
+__all__ = ["Batch", "Mesh", "cg_3d_mpi", "release", "systems"]

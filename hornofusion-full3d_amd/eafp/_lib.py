@@ -1,0 +1,106 @@
+"""ctypes binding of libeafp.so (the C-ABI declared in include/eafp.h).
+
+The library is the only compute path: if it is missing this module raises,
+it never falls back to a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libeafp.so")
+CSRC = os.path.join(_PKG, "csrc")
+
+EAFP_OK = 0
+EAFP_ERR_ARG = -1
+EAFP_ERR_UNSUPPORTED = -2
+EAFP_ERR_COMPONENTS = -3
+EAFP_ERR_NO_DEVICE = -4
+EAFP_ERR_STATE = -5
+
+STOP_NAMES = {0: "running", 1: "converged", 2: "breakdown", 3: "max_iter", 4: "zero_rhs"}
+
+COEF_FP32, COEF_BF16, COEF_FP16 = 0, 1, 2
+
+# Every symbol declared in include/eafp.h.
+EXPORTS = (
+    "eafp_cg_solve", "eafp_release", "eafp_batch_create", "eafp_batch_upload",
+    "eafp_batch_solve_fp64", "eafp_batch_solve_mixed", "eafp_batch_download",
+    "eafp_batch_info", "eafp_batch_destroy", "eafp_version",
+)
+
+
+class EafpError(RuntimeError):
+    def __init__(self, rc, where):
+        names = {EAFP_ERR_ARG: "invalid argument", EAFP_ERR_UNSUPPORTED: "unsupported mesh/descriptor",
+                 EAFP_ERR_COMPONENTS: "too many singular components", EAFP_ERR_NO_DEVICE: "no HIP device",
+                 EAFP_ERR_STATE: "invalid state"}
+        msg = names.get(rc, f"HIP error {rc}" if rc > 0 else f"error {rc}")
+        super().__init__(f"{where}: {msg} (rc={rc})")
+        self.rc = rc
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [("nr", ctypes.c_int), ("ntheta", ctypes.c_int), ("nz", ctypes.c_int),
+                ("halo", ctypes.c_int), ("lbound", ctypes.c_int), ("is_parallel", ctypes.c_int),
+                ("periodic_theta", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("residual", ctypes.c_double), ("n_iter", ctypes.c_int), ("n_refine", ctypes.c_int),
+                ("status", ctypes.c_int), ("pad_", ctypes.c_int), ("true_rel_l2", ctypes.c_double),
+                ("true_rel_l1", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {"residual": self.residual, "n_iter": self.n_iter, "n_refine": self.n_refine,
+                "status": STOP_NAMES.get(self.status, str(self.status)),
+                "true_rel_l2": self.true_rel_l2, "true_rel_l1": self.true_rel_l1,
+                "seconds": self.seconds}
+
+
+class MixedOpts(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("inner_tol", ctypes.c_double), ("max_inner", ctypes.c_int),
+                ("max_refine", ctypes.c_int), ("coef_format", ctypes.c_int), ("chunk", ctypes.c_int)]
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def load():
+    """Load libeafp.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    L = ctypes.CDLL(LIB_PATH)
+    i, d, vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    L.eafp_cg_solve.argtypes = [ctypes.POINTER(Desc)] + [vp] * 9 + [i, d, _dp, ctypes.POINTER(i)]
+    L.eafp_cg_solve.restype = i
+    L.eafp_release.argtypes = []
+    L.eafp_release.restype = i
+    L.eafp_batch_create.argtypes = [i, ctypes.POINTER(Desc), ctypes.POINTER(vp)]
+    L.eafp_batch_create.restype = i
+    L.eafp_batch_upload.argtypes = [vp, i] + [vp] * 9 + [i]
+    L.eafp_batch_upload.restype = i
+    L.eafp_batch_solve_fp64.argtypes = [vp, i, d, ctypes.POINTER(Stats)]
+    L.eafp_batch_solve_fp64.restype = i
+    L.eafp_batch_solve_mixed.argtypes = [vp, ctypes.POINTER(MixedOpts), ctypes.POINTER(Stats)]
+    L.eafp_batch_solve_mixed.restype = i
+    L.eafp_batch_download.argtypes = [vp, i, vp, i]
+    L.eafp_batch_download.restype = i
+    L.eafp_batch_info.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(ctypes.c_long)]
+    L.eafp_batch_info.restype = i
+    L.eafp_batch_destroy.argtypes = [vp]
+    L.eafp_batch_destroy.restype = i
+    L.eafp_version.argtypes = []
+    L.eafp_version.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc, where):
+    if rc != EAFP_OK:
+        raise EafpError(rc, where)

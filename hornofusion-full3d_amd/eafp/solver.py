@@ -1,0 +1,154 @@
+"""Python mirror of the reference's p' solver interface, on libeafp.so.
+
+``cg_3d_mpi`` keeps the argument list and meaning of the Fortran subroutine
+(``src/mod_solver_3d.f90:485-496``): seven coefficient arrays and ``Su`` in
+the padded ``(-1:n+2)`` Fortran layout, ``phi`` updated in place, the mesh,
+``max_iter`` and ``tol`` in; ``residual`` and ``n_iter`` out.  As in the
+reference, failure is reported through a NaN residual (the library's status
+is kept in ``last_status``).
+
+``Batch`` holds one or more systems resident on the GPU for repeated solves:
+the plain FP64 CG (the reference algorithm) and the mixed-precision solver
+(FP32 inner CG with FP64 iterative refinement).
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import Desc, MixedOpts, Stats, check
+
+COEF_NAMES = ("aW", "aE", "aS", "aN", "aB", "aT", "aP")
+
+
+@dataclasses.dataclass
+class Mesh:
+    """The fields of ``mesh_t`` that ``cg_3d_mpi`` reads (src/mod_types_3d.f90:143-193)."""
+    nr: int
+    ntheta: int
+    nz: int
+    is_parallel: bool = False
+    periodic_theta: bool = True
+
+
+def desc_of(m) -> Desc:
+    return Desc(int(m.nr), int(m.ntheta), int(m.nz), 2, -1, int(bool(m.is_parallel)),
+                int(bool(getattr(m, "periodic_theta", True))))
+
+
+def _padded(a, shape, name, writable=False):
+    nr, nth, nz = shape
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64:
+        raise TypeError(f"{name} must be a float64 numpy array")
+    if a.shape != (nr + 4, nth + 4, nz + 4) or not a.flags["F_CONTIGUOUS"]:
+        raise ValueError(f"{name} must be Fortran-ordered with bounds (-1:n+2): shape {(nr + 4, nth + 4, nz + 4)}")
+    if writable and not a.flags["WRITEABLE"]:
+        raise ValueError(f"{name} must be writable")
+    return a.ctypes.data
+
+
+last_status = 0
+
+
+def cg_3d_mpi(aW, aE, aS, aN, aB, aT, aP, Su, phi, m, max_iter, tol):
+    """Drop-in for ``cg_3d_mpi``; returns ``(residual, n_iter)``, updates ``phi``.
+
+    A nonzero library status (bad mesh descriptor, parallel mesh, HIP error)
+    gives ``residual = nan`` and ``n_iter = 0``, the failure signal the
+    reference's caller consumes (src/mod_pressure_3d.f90:322).
+    """
+    global last_status
+    L = _lib.load()
+    d = desc_of(m)
+    shape = (m.nr, m.ntheta, m.nz)
+    ptrs = [_padded(a, shape, n) for a, n in zip((aW, aE, aS, aN, aB, aT, aP, Su), COEF_NAMES + ("Su",))]
+    ptrs.append(_padded(phi, shape, "phi", writable=True))
+    res = ctypes.c_double(0.0)
+    nit = ctypes.c_int(0)
+    rc = L.eafp_cg_solve(ctypes.byref(d), *ptrs, int(max_iter), float(tol), ctypes.byref(res), ctypes.byref(nit))
+    last_status = rc
+    if rc != 0:
+        return math.nan, 0
+    return res.value, nit.value
+
+
+def release():
+    _lib.load().eafp_release()
+
+
+class Batch:
+    """Systems resident in HBM: upload once, solve many times."""
+
+    def __init__(self, shapes):
+        L = _lib.load()
+        self._L = L
+        self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        n = len(self.shapes)
+        arr = (Desc * n)(*[desc_of(Mesh(*s)) for s in self.shapes])
+        h = ctypes.c_void_p()
+        check(L.eafp_batch_create(n, arr, ctypes.byref(h)), "eafp_batch_create")
+        self._h = h
+
+    @classmethod
+    def from_systems(cls, systems):
+        b = cls([ps.shape for ps in systems])
+        for s, ps in enumerate(systems):
+            b.upload(s, ps)
+        return b
+
+    def __len__(self):
+        return len(self.shapes)
+
+    def upload(self, s, ps, phi0=None):
+        shape = self.shapes[s]
+        ptrs = [_padded(ps.padded(n), shape, n) for n in COEF_NAMES + ("Su",)]
+        p0 = _padded(phi0, shape, "phi0") if phi0 is not None else None
+        check(self._L.eafp_batch_upload(self._h, s, *ptrs, p0, 0), "eafp_batch_upload")
+
+    def solve_fp64(self, max_iter=100000, tol=1e-8):
+        st = (Stats * len(self))()
+        check(self._L.eafp_batch_solve_fp64(self._h, int(max_iter), float(tol), st), "eafp_batch_solve_fp64")
+        return [s.as_dict() for s in st]
+
+    def solve_mixed(self, tol=1e-8, inner_tol=1e-2, max_inner=20000, max_refine=30,
+                    coef_format=_lib.COEF_FP32, chunk=0):
+        o = MixedOpts(float(tol), float(inner_tol), int(max_inner), int(max_refine), int(coef_format), int(chunk))
+        st = (Stats * len(self))()
+        check(self._L.eafp_batch_solve_mixed(self._h, ctypes.byref(o), st), "eafp_batch_solve_mixed")
+        return [s.as_dict() for s in st]
+
+    def download(self, s, phi=None):
+        nr, nth, nz = self.shapes[s]
+        if phi is None:
+            phi = np.zeros((nr + 4, nth + 4, nz + 4), order="F")
+        check(self._L.eafp_batch_download(self._h, s, _padded(phi, self.shapes[s], "phi", True), 0),
+              "eafp_batch_download")
+        return phi
+
+    def info(self):
+        g = ctypes.c_int()
+        t = ctypes.c_int()
+        n = ctypes.c_long()
+        check(self._L.eafp_batch_info(self._h, ctypes.byref(g), ctypes.byref(t), ctypes.byref(n)), "eafp_batch_info")
+        return {"grid": g.value, "tile_cells": t.value, "ncells": n.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.eafp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,149 @@
+/*
+ * eafp.h -- C-ABI of libeafp.so, the MI355X p' (pressure-correction) solver
+ * for EAF3D.
+ *
+ * The drop-in entry point eafp_cg_solve() replaces the body of the
+ * reference's cg_3d_mpi (src/mod_solver_3d.f90:485-620), called from
+ * solve_pressure_correction (src/mod_pressure_3d.f90:316-318).  The Fortran
+ * shim mod_eafp_solver (hornofusion-full3d_amd/fortran/mod_eafp_solver.f90)
+ * keeps cg_3d_mpi's subroutine interface and binds to it with ISO_C_BINDING.
+ *
+ * Host arrays are the reference's storage: real(8), Fortran order (r
+ * fastest, then theta, then z), bounds (-1:nr+2, -1:ntheta+2, -1:nz+2)
+ * (src/mod_workspace.f90:124-125), i.e. (nr+4)*(ntheta+4)*(nz+4) doubles.
+ *
+ * Conventions
+ *  - Return value: EAFP_OK (0) on success, a negative EAFP_ERR_* for invalid
+ *    input, a positive hipError_t value for a HIP failure.  The Fortran
+ *    shim turns any nonzero status into residual = NaN, the failure signal
+ *    the reference's caller already consumes (src/mod_pressure_3d.f90:322).
+ *  - Every call is synchronous on return and runs on the library's own HIP
+ *    stream.  Handles are not thread-safe; one handle, one host thread.
+ *  - One GPU (the current HIP device); no MPI inside the solve.
+ */
+#ifndef EAFP_H
+#define EAFP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EAFP_OK 0
+#define EAFP_ERR_ARG (-1)          /* bad pointer, size or option */
+#define EAFP_ERR_UNSUPPORTED (-2)  /* is_parallel, halo != 2, lbound != -1, ... */
+#define EAFP_ERR_COMPONENTS (-3)   /* more singular components than the projection holds */
+#define EAFP_ERR_NO_DEVICE (-4)    /* no HIP device visible */
+#define EAFP_ERR_STATE (-5)        /* solve before upload, index out of range */
+
+/* Mesh descriptor: the fields of mesh_t that cg_3d_mpi reads
+ * (src/mod_types_3d.f90:143-193; bounds() src/mod_solver_3d.f90:573-585). */
+typedef struct eafp_desc {
+    int nr, ntheta, nz;   /* owned cells (serial: the whole mesh) */
+    int halo;             /* halo depth of the host arrays: must be 2 */
+    int lbound;           /* Fortran lower bound of the host arrays: must be -1 */
+    int is_parallel;      /* mesh_t%is_parallel: must be 0 (one GPU) */
+    int periodic_theta;   /* topo%periodic_theta: must be 1 (EAF cylinder) */
+} eafp_desc;
+
+/* Outcome of one solve. */
+typedef struct eafp_stats {
+    double residual;      /* FP64 CG: relative recurrence residual, as cg_3d_mpi's
+                             `residual`; mixed: true relative residual ||b-Ax||/||b|| */
+    int n_iter;           /* FP64 CG: iterations, as cg_3d_mpi's `n_iter`;
+                             mixed: total inner iterations */
+    int n_refine;         /* mixed: FP64 refinement steps (0 for FP64 CG) */
+    int status;           /* EAFP_STOP_* */
+    int pad_;
+    double true_rel_l2;   /* ||b - A x||_2 / ||b||_2 with the FP64 operator */
+    double true_rel_l1;   /* sum|b - A x| / sum|b| (the L1 form of res_cont) */
+    double seconds;       /* wall time of the device work of this solve */
+} eafp_stats;
+
+#define EAFP_STOP_RUNNING 0
+#define EAFP_STOP_CONVERGED 1    /* residual < tol */
+#define EAFP_STOP_BREAKDOWN 2    /* |p.q| < 1e-30 (src/mod_solver_3d.f90:535) */
+#define EAFP_STOP_MAX_ITER 3     /* max_iter reached */
+#define EAFP_STOP_ZERO_RHS 4     /* ||Su|| < 1e-30 (src/mod_solver_3d.f90:516-521) */
+
+/* Coefficient storage of the mixed-precision inner solver. */
+#define EAFP_COEF_FP32 0
+#define EAFP_COEF_BF16 1
+#define EAFP_COEF_FP16 2         /* FP16 with a per-cell FP32 scale */
+
+typedef struct eafp_mixed_opts {
+    double tol;           /* outer target: true relative residual (e.g. 1e-8) */
+    double inner_tol;     /* inner FP32 CG relative residual per refinement step */
+    int max_inner;        /* inner iteration cap per refinement step */
+    int max_refine;       /* refinement step cap */
+    int coef_format;      /* EAFP_COEF_* */
+    int chunk;            /* iterations per captured graph (0: library default) */
+} eafp_mixed_opts;
+
+/* ---------------------------------------------------------------------------
+ * Drop-in: cg_3d_mpi(aW,aE,aS,aN,aB,aT,aP,Su,phi,m,max_iter,tol,residual,n_iter)
+ * (src/mod_solver_3d.f90:485-496).  Jacobi-PCG in FP64 on the GPU with the
+ * reference's semantics: x0 = phi, recurrence-residual stop test, |p.q| <
+ * 1e-30 exit, ||Su|| < 1e-30 early return, phi updated on the interior and
+ * its theta halos (j = -1, 0, ntheta+1, ntheta+2) filled by periodic copy;
+ * r/z halos of phi are left as they were.  Device buffers are cached per
+ * mesh shape across calls (the role of ensure_workspace,
+ * src/mod_workspace.f90:120-144).
+ * ------------------------------------------------------------------------- */
+int eafp_cg_solve(const eafp_desc* desc,
+                  const double* aW, const double* aE, const double* aS, const double* aN,
+                  const double* aB, const double* aT, const double* aP, const double* Su,
+                  double* phi, int max_iter, double tol, double* residual, int* n_iter);
+
+/* Releases the device buffers cached by eafp_cg_solve.  No reference
+ * counterpart (the reference's workspace lives until exit). */
+int eafp_release(void);
+
+/* ---------------------------------------------------------------------------
+ * Device-resident systems.  A batch holds nsys systems of any shapes; nsys
+ * = 1 is the single-system case.  Upload once, solve many times: every
+ * solve restarts from the uploaded phi (the reference zeroes sh%pp before
+ * each call, src/mod_pressure_3d.f90:174).  These entry points have no
+ * one-to-one reference counterpart: they split cg_3d_mpi's implicit
+ * copy-in / solve / copy-out so operands can stay resident in HBM.
+ * ------------------------------------------------------------------------- */
+typedef struct eafp_batch eafp_batch;
+
+int eafp_batch_create(int nsys, const eafp_desc* descs, eafp_batch** out);
+
+/* Uploads system s.  Arrays are padded Fortran layout; arrays_on_device = 0
+ * for host pointers, 1 for device pointers.  phi0 may be NULL (zero). */
+int eafp_batch_upload(eafp_batch* b, int s,
+                      const double* aW, const double* aE, const double* aS, const double* aN,
+                      const double* aB, const double* aT, const double* aP, const double* Su,
+                      const double* phi0, int arrays_on_device);
+
+/* The reference algorithm (FP64 Jacobi-PCG, as eafp_cg_solve) on every
+ * system of the batch in one launch sequence with per-system convergence
+ * masks.  stats: nsys entries (may be NULL). */
+int eafp_batch_solve_fp64(eafp_batch* b, int max_iter, double tol, eafp_stats* stats);
+
+/* Mixed precision: FP32 inner Jacobi-PCG on low-precision symmetric face
+ * coefficients, wrapped in FP64 iterative refinement (FP64 true residual,
+ * FP64 dot accumulation, FP64 zero-mean projection of every singular
+ * Neumann component).  The result is gauge-fixed: zero mean per singular
+ * component.  stats: nsys entries (may be NULL). */
+int eafp_batch_solve_mixed(eafp_batch* b, const eafp_mixed_opts* opts, eafp_stats* stats);
+
+/* Copies system s's phi (padded Fortran layout: interior from the last
+ * solve, theta halos periodic copies, r/z halos as uploaded) to phi;
+ * to_device = 1 if phi is a device pointer. */
+int eafp_batch_download(eafp_batch* b, int s, double* phi, int to_device);
+
+/* Launch shape in use: blocks per launch and cells per tile (for reports). */
+int eafp_batch_info(const eafp_batch* b, int* grid, int* tile_cells, long* ncells);
+
+int eafp_batch_destroy(eafp_batch* b);
+
+/* Library version string. */
+const char* eafp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EAFP_H */

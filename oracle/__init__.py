@@ -1,0 +1,189 @@
+"""TEST INFRASTRUCTURE ONLY: the FP64 CPU oracle of EAF3D's p' CG solve.
+
+Only ``tests/``, the smoke check and the benchmark's CPU-baseline leg import
+this package.  It is the checker, never the thing measured or shipped.
+
+Contents
+--------
+* ctypes bindings to ``liboracle.so`` (``cg_oracle.c``): a plain-C FP64
+  restatement of ``cg_3d_mpi`` (``src/mod_solver_3d.f90:485-620``), its
+  ``matvec`` (``:588-605``), ``dot`` (``:607-618``) and the serial theta halo
+  copy (``src/mod_mpi_topology.f90:303-310``), on the reference's padded
+  Fortran layout (bounds ``(-1:n+2)``, r fastest).
+* numpy helpers that are independent of the C code and are used to pin it:
+  a scipy CSR operator built from the interior coefficient arrays, the
+  per-component zero-mean projection, and gauge-fixed error metrics.
+
+Pinning (tests/test_oracle.py): scipy ``spsolve`` on a gauge-fixed system,
+manufactured solutions ``b = A x*``, the closed form ``A 1 = aP - sum(a_nb)``,
+theta-roll equivariance of the operator, ``math.fsum`` dots, and the CG
+invariants (identity rows stay 0, ``b = 0`` returns at once, breakdown
+exit, ``n_iter <= max_iter``).  The reference's exact gfortran residual
+history is not available (no Fortran MPI toolchain in this image), so the
+iteration-by-iteration history is "parity unpinned" beyond these checks.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def build(force: bool = False) -> str:
+    """Compile ``liboracle.so`` with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        i = ctypes.c_int
+        L.oracle_cg.argtypes = [i, i, i] + [_dp] * 8 + [_dp, i, ctypes.c_double, _dp,
+                                                         ctypes.POINTER(ctypes.c_int), _dp]
+        L.oracle_cg.restype = ctypes.c_int
+        L.oracle_matvec.argtypes = [i, i, i] + [_dp] * 7 + [_dp, _dp]
+        L.oracle_matvec.restype = None
+        L.oracle_theta_halo.argtypes = [i, i, i, _dp]
+        L.oracle_theta_halo.restype = None
+        L.oracle_dot.argtypes = [i, i, i, _dp, _dp]
+        L.oracle_dot.restype = ctypes.c_double
+        L.oracle_true_residual.argtypes = [i, i, i] + [_dp] * 9 + [_dp, _dp]
+        L.oracle_true_residual.restype = None
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["F_CONTIGUOUS"], "padded arrays are Fortran-ordered float64"
+    return a.ctypes.data_as(_dp)
+
+
+COEF = ("aW", "aE", "aS", "aN", "aB", "aT", "aP")
+
+
+def cg(ps, phi=None, max_iter=100000, tol=1e-8, history=False):
+    """Run the restated ``cg_3d_mpi`` on system ``ps`` (padded layout).
+
+    Returns ``(phi, residual, n_iter, hist)``; ``phi`` is a new padded array
+    (x0 = ``phi`` or zero, as the reference's caller zeroes ``sh%pp``).
+    """
+    nr, nth, nz = ps.shape
+    phi = np.zeros((nr + 4, nth + 4, nz + 4), order="F") if phi is None else np.array(phi, order="F", dtype=np.float64)
+    res = ctypes.c_double()
+    nit = ctypes.c_int()
+    hist = np.full(max_iter + 1, np.nan) if history else None
+    rc = lib().oracle_cg(nr, nth, nz, *[_p(ps.padded(n)) for n in COEF], _p(ps.padded("Su")),
+                         _p(phi), int(max_iter), float(tol), ctypes.byref(res), ctypes.byref(nit),
+                         hist.ctypes.data_as(_dp) if history else None)
+    if rc != 0:
+        raise MemoryError("oracle_cg could not allocate its work arrays")
+    if history:
+        hist = hist[: nit.value + 1]
+    return phi, res.value, nit.value, hist
+
+
+def matvec(ps, x):
+    """y = A x with the reference's halo copy of x (x is modified in place)."""
+    nr, nth, nz = ps.shape
+    y = np.zeros_like(x, order="F")
+    lib().oracle_matvec(nr, nth, nz, *[_p(ps.padded(n)) for n in COEF], _p(x), _p(y))
+    return y
+
+
+def theta_halo(shape, x):
+    lib().oracle_theta_halo(*shape, _p(x))
+    return x
+
+
+def dot(shape, a, b):
+    return lib().oracle_dot(*shape, _p(a), _p(b))
+
+
+def true_residual(ps, x):
+    """(||b - A x||/||b||, sum|b - A x| / sum|b|) with x in padded layout."""
+    nr, nth, nz = ps.shape
+    l2 = ctypes.c_double()
+    l1 = ctypes.c_double()
+    lib().oracle_true_residual(nr, nth, nz, *[_p(ps.padded(n)) for n in COEF], _p(ps.padded("Su")),
+                               _p(np.asfortranarray(x)), ctypes.byref(l2), ctypes.byref(l1))
+    return l2.value, l1.value
+
+
+# ---------------------------------------------------------------------------
+# numpy / scipy helpers, independent of cg_oracle.c
+# ---------------------------------------------------------------------------
+
+def csr(ps):
+    """scipy CSR matrix of the interior operator, cells ordered z fastest.
+
+    Row (i,j,k) holds aP on the diagonal and -a_nb on the neighbour columns,
+    with the theta neighbour wrapping periodically (the halo copy made
+    explicit) and r/z links at the walls dropped (their halo partners are 0).
+    """
+    import scipy.sparse as sp
+    nr, nth, nz = ps.shape
+    n = nr * nth * nz
+    idx = np.arange(n).reshape(nr, nth, nz)
+    rows = [idx.ravel()]
+    cols = [idx.ravel()]
+    vals = [ps.interior("aP").ravel()]
+    for name, shift, axis in (("aW", 1, 0), ("aE", -1, 0), ("aS", 1, 1), ("aN", -1, 1),
+                              ("aB", 1, 2), ("aT", -1, 2)):
+        a = ps.interior(name)
+        nb = np.roll(idx, shift, axis=axis)
+        m = a != 0.0
+        if axis != 1:   # non-periodic axes: a wall link would read a zero halo
+            edge = np.zeros_like(m)
+            if shift == 1:
+                edge[(slice(None),) * axis + (0,)] = True
+            else:
+                edge[(slice(None),) * axis + (-1,)] = True
+            m = m & ~edge
+        rows.append(idx[m])
+        cols.append(nb[m])
+        vals.append(-a[m])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+
+
+def project(v, labels, ncomp):
+    """Per-component zero-mean projection (FP64), masked cells -> 0."""
+    v = np.asarray(v, dtype=np.float64).reshape(labels.shape)
+    out = np.zeros_like(v)
+    act = labels >= 0
+    lab = labels[act]
+    sums = np.bincount(lab, weights=v[act], minlength=ncomp)
+    cnt = np.bincount(lab, minlength=ncomp)
+    out[act] = v[act] - (sums / np.maximum(cnt, 1))[lab]
+    return out
+
+
+def fsum_dot(a, b):
+    """Correctly rounded dot product (math.fsum of exact products' roundings)."""
+    return math.fsum((np.asarray(a, dtype=np.float64) * np.asarray(b, dtype=np.float64)).ravel().tolist())
+
+
+def gauge_error(x, x_ref, labels, ncomp):
+    """||P x - P x_ref|| / ||P x_ref|| with P the per-component projection."""
+    px = project(x, labels, ncomp)
+    pr = project(x_ref, labels, ncomp)
+    den = np.linalg.norm(pr)
+    return float(np.linalg.norm(px - pr) / den) if den > 0 else float(np.linalg.norm(px - pr))
+
+
+def interior(padded):
+    return np.ascontiguousarray(padded[2:-2, 2:-2, 2:-2])

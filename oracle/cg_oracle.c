@@ -1,0 +1,228 @@
+/*
+ * oracle/cg_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement, in plain C and FP64, of EAF3D's p' conjugate-gradient
+ * solve.  It is the checker for the HIP library: only tests, the smoke
+ * check and the CPU-baseline leg of the benchmark load it.  Nothing in the
+ * product path links or calls it.
+ *
+ * Every array uses the reference's storage: Fortran order (r fastest, then
+ * theta, then z) with bounds (-1:n+2) in each axis, i.e.
+ *   off(i,j,k) = (i+1) + (nr+4)*((j+1) + (nth+4)*(k+1)),  i,j,k in [-1, n+2].
+ *
+ * Functions and the reference code they restate:
+ *   oracle_theta_halo  src/mod_mpi_topology.f90:303-310 (serial theta copy,
+ *                      four planes, in that order, over all i and k)
+ *   oracle_matvec      src/mod_solver_3d.f90:588-605 (halo copy of x, y = 0
+ *                      over the whole padded array, 7-point sweep k,j,i)
+ *   oracle_dot         src/mod_solver_3d.f90:607-618 (Fortran sum() over the
+ *                      interior, column-major sequential order)
+ *   oracle_cg          src/mod_solver_3d.f90:485-569 (Jacobi PCG with
+ *                      max(|aP|, SMALL), bnorm early return, |p.q| < SMALL
+ *                      breakdown exit, recurrence-residual stop, final theta
+ *                      halo copy of phi)
+ *   oracle_true_residual  no reference counterpart: ||b - A x||_2/||b||_2 and
+ *                      the L1 form sum|b - A x| / sum|b| for reporting.
+ *
+ * Build with -ffp-contract=off so that every multiply and add rounds on its
+ * own (no FMA), which gives the plain sequential evaluation order.
+ * Parity of this file is pinned by tests/test_oracle.py (scipy direct solve,
+ * manufactured solutions, an independent numpy CSR operator, the closed
+ * form A*1 = aP - sum(a_nb), invariants) -- see the test docstrings.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SMALL 1.0e-30   /* src/mod_constants.f90:53 */
+
+typedef struct {
+    int nr, nth, nz;
+    long sx, sy, sz;    /* strides of i, j, k */
+    long n;             /* padded length */
+} grid_t;
+
+static grid_t mkgrid(int nr, int nth, int nz) {
+    grid_t g;
+    g.nr = nr; g.nth = nth; g.nz = nz;
+    g.sx = 1; g.sy = nr + 4; g.sz = (long)(nr + 4) * (nth + 4);
+    g.n = g.sz * (nz + 4);
+    return g;
+}
+
+static inline long off(const grid_t* g, int i, int j, int k) {
+    return (long)(i + 1) + g->sy * (j + 1) + g->sz * (k + 1);
+}
+
+/* x(:,0,:)=x(:,nth,:); x(:,-1,:)=x(:,nth-1,:); x(:,nth+1,:)=x(:,1,:);
+ * x(:,nth+2,:)=x(:,2,:) -- applied plane by plane in this order. */
+static void theta_halo(const grid_t* g, double* x) {
+    const int dst[4] = {0, -1, g->nth + 1, g->nth + 2};
+    const int src[4] = {g->nth, g->nth - 1, 1, 2};
+    for (int p = 0; p < 4; ++p)
+        for (int k = -1; k <= g->nz + 2; ++k)
+            for (int i = -1; i <= g->nr + 2; ++i)
+                x[off(g, i, dst[p], k)] = x[off(g, i, src[p], k)];
+}
+
+void oracle_theta_halo(int nr, int nth, int nz, double* x) {
+    grid_t g = mkgrid(nr, nth, nz);
+    theta_halo(&g, x);
+}
+
+typedef struct {
+    const double *aW, *aE, *aS, *aN, *aB, *aT, *aP;
+} coef_t;
+
+static void matvec(const grid_t* g, const coef_t* c, double* x, double* y) {
+    theta_halo(g, x);
+    memset(y, 0, sizeof(double) * (size_t)g->n);
+    for (int k = 1; k <= g->nz; ++k)
+        for (int j = 1; j <= g->nth; ++j)
+            for (int i = 1; i <= g->nr; ++i) {
+                long o = off(g, i, j, k);
+                y[o] = c->aP[o] * x[o]
+                     - c->aW[o] * x[o - g->sx] - c->aE[o] * x[o + g->sx]
+                     - c->aS[o] * x[o - g->sy] - c->aN[o] * x[o + g->sy]
+                     - c->aB[o] * x[o - g->sz] - c->aT[o] * x[o + g->sz];
+            }
+}
+
+void oracle_matvec(int nr, int nth, int nz,
+                   const double* aW, const double* aE, const double* aS, const double* aN,
+                   const double* aB, const double* aT, const double* aP,
+                   double* x, double* y) {
+    grid_t g = mkgrid(nr, nth, nz);
+    coef_t c = {aW, aE, aS, aN, aB, aT, aP};
+    matvec(&g, &c, x, y);
+}
+
+static double dot(const grid_t* g, const double* a, const double* b) {
+    double s = 0.0;
+    for (int k = 1; k <= g->nz; ++k)
+        for (int j = 1; j <= g->nth; ++j)
+            for (int i = 1; i <= g->nr; ++i) {
+                long o = off(g, i, j, k);
+                s += a[o] * b[o];
+            }
+    return s;
+}
+
+double oracle_dot(int nr, int nth, int nz, const double* a, const double* b) {
+    grid_t g = mkgrid(nr, nth, nz);
+    return dot(&g, a, b);
+}
+
+/* Returns 0 on success, -1 if the work arrays cannot be allocated.
+ * res_hist (nullable, length max_iter+1) receives the relative recurrence
+ * residual before the loop and after every iteration that reached it. */
+int oracle_cg(int nr, int nth, int nz,
+              const double* aW, const double* aE, const double* aS, const double* aN,
+              const double* aB, const double* aT, const double* aP, const double* Su,
+              double* phi, int max_iter, double tol,
+              double* residual, int* n_iter, double* res_hist) {
+    grid_t g = mkgrid(nr, nth, nz);
+    coef_t c = {aW, aE, aS, aN, aB, aT, aP};
+    double* r = calloc((size_t)g.n, sizeof(double));
+    double* z = calloc((size_t)g.n, sizeof(double));
+    double* pv = calloc((size_t)g.n, sizeof(double));
+    double* q = calloc((size_t)g.n, sizeof(double));
+    if (!r || !z || !pv || !q) { free(r); free(z); free(pv); free(q); return -1; }
+
+    /* r = Su - A*phi on the interior */
+    matvec(&g, &c, phi, q);
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= nth; ++j)
+            for (int i = 1; i <= nr; ++i) {
+                long o = off(&g, i, j, k);
+                r[o] = Su[o] - q[o];
+            }
+
+    double bnorm = sqrt(dot(&g, Su, Su));
+    if (bnorm < SMALL) {
+        *residual = 0.0;
+        *n_iter = 0;
+        if (res_hist) res_hist[0] = 0.0;
+        free(r); free(z); free(pv); free(q);
+        return 0;
+    }
+
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= nth; ++j)
+            for (int i = 1; i <= nr; ++i) {
+                long o = off(&g, i, j, k);
+                z[o] = r[o] / fmax(fabs(aP[o]), SMALL);
+            }
+    memcpy(pv, z, sizeof(double) * (size_t)g.n);   /* pv = z (whole array) */
+    double rz = dot(&g, r, z);
+    *residual = sqrt(dot(&g, r, r)) / bnorm;
+    *n_iter = 0;
+    if (res_hist) res_hist[0] = *residual;
+
+    for (int iter = 1; iter <= max_iter; ++iter) {
+        matvec(&g, &c, pv, q);
+        double pq = dot(&g, pv, q);
+        if (fabs(pq) < SMALL) break;
+        double alpha = rz / pq;
+        for (int k = 1; k <= nz; ++k)
+            for (int j = 1; j <= nth; ++j)
+                for (int i = 1; i <= nr; ++i) {
+                    long o = off(&g, i, j, k);
+                    phi[o] = phi[o] + alpha * pv[o];
+                }
+        for (int k = 1; k <= nz; ++k)
+            for (int j = 1; j <= nth; ++j)
+                for (int i = 1; i <= nr; ++i) {
+                    long o = off(&g, i, j, k);
+                    r[o] = r[o] - alpha * q[o];
+                }
+        double rnorm = sqrt(dot(&g, r, r));
+        *residual = rnorm / bnorm;
+        *n_iter = iter;
+        if (res_hist) res_hist[iter] = *residual;
+        if (*residual < tol) break;
+        for (int k = 1; k <= nz; ++k)
+            for (int j = 1; j <= nth; ++j)
+                for (int i = 1; i <= nr; ++i) {
+                    long o = off(&g, i, j, k);
+                    z[o] = r[o] / fmax(fabs(aP[o]), SMALL);
+                }
+        double rz_new = dot(&g, r, z);
+        double beta = rz_new / fmax(rz, SMALL);
+        rz = rz_new;
+        for (int k = 1; k <= nz; ++k)
+            for (int j = 1; j <= nth; ++j)
+                for (int i = 1; i <= nr; ++i) {
+                    long o = off(&g, i, j, k);
+                    pv[o] = z[o] + beta * pv[o];
+                }
+    }
+
+    theta_halo(&g, phi);
+    free(r); free(z); free(pv); free(q);
+    return 0;
+}
+
+/* ||b - A x||_2 / ||b||_2 and sum|b - A x| / sum|b| over the interior.
+ * x's halos are used as they are (no theta copy is made here). */
+void oracle_true_residual(int nr, int nth, int nz,
+                          const double* aW, const double* aE, const double* aS, const double* aN,
+                          const double* aB, const double* aT, const double* aP, const double* Su,
+                          const double* x, double* rel_l2, double* rel_l1) {
+    grid_t g = mkgrid(nr, nth, nz);
+    double s2 = 0.0, b2 = 0.0, s1 = 0.0, b1 = 0.0;
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= nth; ++j)
+            for (int i = 1; i <= nr; ++i) {
+                long o = off(&g, i, j, k);
+                double ax = aP[o] * x[o]
+                          - aW[o] * x[o - g.sx] - aE[o] * x[o + g.sx]
+                          - aS[o] * x[o - g.sy] - aN[o] * x[o + g.sy]
+                          - aB[o] * x[o - g.sz] - aT[o] * x[o + g.sz];
+                double rr = Su[o] - ax;
+                s2 += rr * rr; b2 += Su[o] * Su[o];
+                s1 += fabs(rr); b1 += fabs(Su[o]);
+            }
+    *rel_l2 = b2 > 0.0 ? sqrt(s2) / sqrt(b2) : sqrt(s2);
+    *rel_l1 = b1 > 0.0 ? s1 / b1 : s1;
+}

@@ -1,0 +1,86 @@
+"""CPU checks of the C-ABI boundary: libeafp.so loads, exports every entry
+point that include/eafp.h declares, and rejects invalid descriptors before
+any GPU work (so these run without a GPU)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import eafp
+from eafp import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    src = open(os.path.join(ROOT, "include", "eafp.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(eafp_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_symbols_exported():
+    L = _lib.load()
+    names = header_functions()
+    assert set(names) == set(_lib.EXPORTS), names
+    for n in names:
+        assert hasattr(L, n), n
+    assert L.eafp_version().startswith(b"eafp")
+
+
+def test_struct_layouts_match_header():
+    # eafp_desc: 7 ints; eafp_stats: double, 4 ints, 3 doubles; eafp_mixed_opts: 2 doubles, 4 ints
+    assert ctypes.sizeof(_lib.Desc) == 28
+    assert ctypes.sizeof(_lib.Stats) == 48
+    assert ctypes.sizeof(_lib.MixedOpts) == 32
+
+
+def _arrays(nr=4, nth=6, nz=5):
+    return [np.zeros((nr + 4, nth + 4, nz + 4), order="F") for _ in range(9)]
+
+
+@pytest.mark.parametrize("field,value,rc", [
+    ("is_parallel", 1, _lib.EAFP_ERR_UNSUPPORTED),
+    ("halo", 1, _lib.EAFP_ERR_UNSUPPORTED),
+    ("lbound", 0, _lib.EAFP_ERR_UNSUPPORTED),
+    ("periodic_theta", 0, _lib.EAFP_ERR_UNSUPPORTED),
+    ("nr", 0, _lib.EAFP_ERR_ARG),
+])
+def test_descriptor_rejected(field, value, rc):
+    L = _lib.load()
+    d = _lib.Desc(4, 6, 5, 2, -1, 0, 1)
+    setattr(d, field, value)
+    a = _arrays()
+    res = ctypes.c_double(0.0)
+    nit = ctypes.c_int(0)
+    got = L.eafp_cg_solve(ctypes.byref(d), *[x.ctypes.data for x in a], 10, 1e-8, ctypes.byref(res), ctypes.byref(nit))
+    assert got == rc
+
+
+def test_null_pointer_rejected():
+    L = _lib.load()
+    d = _lib.Desc(4, 6, 5, 2, -1, 0, 1)
+    a = _arrays()
+    ptrs = [x.ctypes.data for x in a]
+    ptrs[3] = None
+    res = ctypes.c_double(0.0)
+    nit = ctypes.c_int(0)
+    assert L.eafp_cg_solve(ctypes.byref(d), *ptrs, 10, 1e-8, ctypes.byref(res), ctypes.byref(nit)) == _lib.EAFP_ERR_ARG
+
+
+def test_python_mirror_maps_failure_to_nan():
+    """Parallel meshes are refused; the mirror reports residual = NaN like the Fortran shim."""
+    a = _arrays()
+    m = eafp.Mesh(4, 6, 5, is_parallel=True)
+    res, nit = eafp.cg_3d_mpi(*a[:8], a[8], m, 10, 1e-8)
+    assert np.isnan(res) and nit == 0
+    assert eafp.solver.last_status == _lib.EAFP_ERR_UNSUPPORTED
+
+
+def test_layout_checked():
+    a = _arrays()
+    a[0] = np.zeros((8, 10, 9))  # C order
+    m = eafp.Mesh(4, 6, 5)
+    with pytest.raises(ValueError):
+        eafp.cg_3d_mpi(*a[:8], a[8], m, 10, 1e-8)
