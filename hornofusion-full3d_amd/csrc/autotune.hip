@@ -1,0 +1,52 @@
+// autotune.hip -- launch-shape autotuner for the iteration kernels.
+//
+// The two inner-iteration kernels of a solver are timed (HIP events on the
+// library stream, averaged over `reps` iterations) for every candidate
+// shape: cells per thread in {1, 2, 4, 8} (the tile is 256 * cpt cells)
+// and blocks per CU in {4, 8, 16} (capped by the tile count).  The fastest
+// shape is kept in the batch; solvers pick it up at their next solve.
+#include <cfloat>
+
+#include "batch.hpp"
+
+namespace eafp {
+
+int time_fp64_kernels(Batch& B, int reps, float* ms_spmv, float* ms_update);
+int time_mixed_kernels(Batch& B, int fmt, int reps, float* ms_spmv, float* ms_update);
+
+int time_kernels(Batch& B, int solver, int fmt, int reps, float* ms_spmv, float* ms_update) {
+    if (solver == 0) return time_fp64_kernels(B, reps, ms_spmv, ms_update);
+    if (solver == 1) return time_mixed_kernels(B, fmt, reps, ms_spmv, ms_update);
+    return EAFP_ERR_ARG;
+}
+
+int autotune(Batch& B, int solver, int fmt, int reps, int* best_cpt, int* best_blocks, float* best_ms) {
+    const int cus = device_cus();
+    const int cpts[] = {1, 2, 4, 8};
+    const int per_cu[] = {4, 8, 16};
+    float best = FLT_MAX;
+    int bc = B.cpt, bb = B.grid;
+    for (int cpt : cpts) {
+        for (int k : per_cu) {
+            EAFP_TRY_RC(B.set_tiling(cpt, 0));
+            const int blocks = std::min(B.ntiles, k * cus);
+            EAFP_TRY_RC(B.set_tiling(cpt, blocks));
+            float a = 0.f, b = 0.f;
+            // one untimed pass settles caches and code, then the timed reps
+            EAFP_TRY_RC(time_kernels(B, solver, fmt, 2, &a, &b));
+            EAFP_TRY_RC(time_kernels(B, solver, fmt, reps, &a, &b));
+            if (a + b < best) {
+                best = a + b;
+                bc = cpt;
+                bb = B.grid;
+            }
+        }
+    }
+    EAFP_TRY_RC(B.set_tiling(bc, bb));
+    if (best_cpt) *best_cpt = bc;
+    if (best_blocks) *best_blocks = bb;
+    if (best_ms) *best_ms = best;
+    return EAFP_OK;
+}
+
+}  // namespace eafp

@@ -1,7 +1,9 @@
 // batch.hip -- system container: tile table, uploads and downloads between
 // the reference's padded Fortran layout and the compact z-fastest layout.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "batch.hpp"
 
@@ -18,17 +20,66 @@ int validate_desc(const eafp_desc* d) {
     return EAFP_OK;
 }
 
-int choose_grid(int ntiles) {
+int device_cus() {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
             cus = v;
     }
+    return cus;
+}
+
+int choose_grid(int ntiles) {
     // Up to 8 blocks of 4 waves per CU; never more blocks than tiles.
-    int g = std::min(ntiles, cus * 8);
+    int g = std::min(ntiles, device_cus() * 8);
     g = (g + NXCD - 1) / NXCD * NXCD;
     return std::max(g, NXCD);
+}
+
+bool valid_cpt(int cpt) { return cpt == 1 || cpt == 2 || cpt == 4 || cpt == 8; }
+
+// Rebuild the tile table for `cells_per_thread` and the launch grid
+// (`blocks` = 0: one block per tile up to 8 per CU).
+int Batch::set_tiling(int cells_per_thread, int blocks) {
+    if (!valid_cpt(cells_per_thread) || blocks < 0) return EAFP_ERR_ARG;
+    const int tile = BLOCK * cells_per_thread;
+    tiles_h.clear();
+    int t = 0;
+    for (int s = 0; s < nsys; ++s) {
+        SysGeom& g = geom[s];
+        const int nt = (g.n + tile - 1) / tile;
+        g.tile_begin = t;
+        g.tile_end = t + nt;
+        for (int u = 0; u < nt; ++u) {
+            TileDesc d{};
+            d.sys = s;
+            d.tile_begin = g.tile_begin;
+            d.tile_end = g.tile_end;
+            d.n = g.n;
+            d.nr = g.nr;
+            d.nth = g.nth;
+            d.nz = g.nz;
+            d.off = g.off;
+            d.inv_nz = g.inv_nz;
+            d.inv_nth = g.inv_nth;
+            tiles_h.push_back(d);
+        }
+        t += nt;
+    }
+    ntiles = t;
+    cpt = cells_per_thread;
+    EAFP_TRY_RC(tiles.alloc(ntiles));
+    EAFP_TRY(hipMemcpy(tiles.p, tiles_h.data(), sizeof(TileDesc) * ntiles, hipMemcpyHostToDevice));
+    if (geom_d.p)
+        EAFP_TRY(hipMemcpy(geom_d.p, geom.data(), sizeof(SysGeom) * nsys, hipMemcpyHostToDevice));
+    if (blocks > 0) {
+        grid = std::max(NXCD, (blocks + NXCD - 1) / NXCD * NXCD);
+    } else {
+        grid = choose_grid(ntiles);
+    }
+    ++tiling_gen;
+    return EAFP_OK;
 }
 
 void free_cg64(Cg64Work* w);
@@ -48,9 +99,6 @@ int Batch::init(int n, const eafp_desc* descs) {
     nsys = n;
     geom.resize(n);
     long off = 0;
-    int t = 0;
-    tile_sys_h.clear();
-    tile_start_h.clear();
     for (int s = 0; s < n; ++s) {
         SysGeom& g = geom[s];
         g.nr = descs[s].nr;
@@ -60,28 +108,21 @@ int Batch::init(int n, const eafp_desc* descs) {
         g.off = off;
         g.inv_nz = 1.0f / (float)g.nz;
         g.inv_nth = 1.0f / (float)g.nth;
-        g.tile_begin = t;
-        for (int c = 0; c < g.n; c += TILE_CELLS) {
-            tile_sys_h.push_back(s);
-            tile_start_h.push_back(c);
-            ++t;
-        }
-        g.tile_end = t;
         // keep every system's first cell 256-byte aligned for FP64 arrays
         off += (g.n + 31) / 32 * 32;
     }
     ncells = off;
-    ntiles = t;
     EAFP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     EAFP_TRY_RC(geom_d.alloc(n));
-    EAFP_TRY(hipMemcpy(geom_d.p, geom.data(), sizeof(SysGeom) * n, hipMemcpyHostToDevice));
-    EAFP_TRY_RC(tile_sys.alloc(ntiles));
-    EAFP_TRY_RC(tile_start.alloc(ntiles));
-    EAFP_TRY(hipMemcpy(tile_sys.p, tile_sys_h.data(), sizeof(int) * ntiles, hipMemcpyHostToDevice));
-    EAFP_TRY(hipMemcpy(tile_start.p, tile_start_h.data(), sizeof(int) * ntiles, hipMemcpyHostToDevice));
+    EAFP_TRY_RC(set_tiling(2, 0));
     for (int c = 0; c < NCOEF; ++c) EAFP_TRY_RC(coef[c].alloc(ncells));
     EAFP_TRY_RC(b.alloc(ncells));
     EAFP_TRY_RC(x0.alloc(ncells));
+    EAFP_TRY_RC(fold.alloc(ncells));
+    EAFP_TRY_RC(comp.alloc(ncells));
+    EAFP_TRY(hipMemset(fold.p, 0, sizeof(double) * ncells));
+    EAFP_TRY(hipMemset(comp.p, NO_COMP, ncells));
+    comp_info.assign(n, CompInfo{});
     EAFP_TRY(hipMemset(b.p, 0, sizeof(double) * ncells));
     EAFP_TRY(hipMemset(x0.p, 0, sizeof(double) * ncells));
     for (int c = 0; c < NCOEF; ++c) EAFP_TRY(hipMemset(coef[c].p, 0, sizeof(double) * ncells));
@@ -92,15 +133,14 @@ int Batch::init(int n, const eafp_desc* descs) {
         EAFP_TRY(hipMemset(phi_pad[s]->p, 0, sizeof(double) * padded_len(s)));
     }
     uploaded.assign(n, 0);
-    grid = choose_grid(ntiles);
     return EAFP_OK;
 }
 
 // ---------------------------------------------------------------------------
 // Pack: padded Fortran (r fastest) -> compact (z fastest).  One thread per
-// compact cell; the wall links of phi0's r/z halos are folded into b:
-//   b = Su + [i=1] aW*phi(0) + [i=nr] aE*phi(nr+1) + [k=1] aB*phi(.,.,0)
-//          + [k=nz] aT*phi(.,.,nz+1)
+// compact cell.  The wall links of phi0's r/z halos are kept apart in
+//   fold = [i=1] aW*phi(0) + [i=nr] aE*phi(nr+1) + [k=1] aB*phi(.,.,0)
+//        + [k=nz] aT*phi(.,.,nz+1)
 // which is exactly the halo part of r = Su - A*phi at the first matvec
 // (src/mod_solver_3d.f90:510-513); later CG vectors have zero r/z halos.
 // ---------------------------------------------------------------------------
@@ -109,6 +149,7 @@ struct PackArgs {
     const double* phi;      // padded phi0 (device)
     double* dst[7];         // compact coefficient arenas (at system offset)
     double* b;
+    double* fold;
     double* x0;
 };
 
@@ -120,13 +161,14 @@ __global__ void k_pack(PackArgs a, SysGeom g) {
     const long o = (id.i + 2) + sy * (id.j + 2) + sz * (id.k + 2);
 #pragma unroll
     for (int q = 0; q < 7; ++q) a.dst[q][c] = a.src[q][o];
-    double bb = a.src[7][o];
     const double* ph = a.phi;
-    if (id.i == 0) bb += a.src[C_AW][o] * ph[o - 1];
-    if (id.i == g.nr - 1) bb += a.src[C_AE][o] * ph[o + 1];
-    if (id.k == 0) bb += a.src[C_AB][o] * ph[o - sz];
-    if (id.k == g.nz - 1) bb += a.src[C_AT][o] * ph[o + sz];
-    a.b[c] = bb;
+    double f = 0.0;
+    if (id.i == 0) f += a.src[C_AW][o] * ph[o - 1];
+    if (id.i == g.nr - 1) f += a.src[C_AE][o] * ph[o + 1];
+    if (id.k == 0) f += a.src[C_AB][o] * ph[o - sz];
+    if (id.k == g.nz - 1) f += a.src[C_AT][o] * ph[o + sz];
+    a.b[c] = a.src[7][o];
+    a.fold[c] = f;
     a.x0[c] = ph[o];
 }
 
@@ -149,11 +191,113 @@ int Batch::upload(int s, const double* const host[8], const double* phi0, int on
     a.phi = phi_pad[s]->p;
     for (int q = 0; q < 7; ++q) a.dst[q] = coef[q].p + g.off;
     a.b = b.p + g.off;
+    a.fold = fold.p + g.off;
     a.x0 = x0.p + g.off;
     k_pack<<<(g.n + 255) / 256, 256, 0, stream>>>(a, g);
     EAFP_TRY(hipGetLastError());
+
+    // singular components (host): needs the coefficients on the host
+    std::vector<double> hbuf;
+    const double* hc[7];
+    if (on_device) {
+        hbuf.resize((size_t)np * 7);
+        EAFP_TRY(hipMemcpyAsync(hbuf.data(), staging.p, sizeof(double) * np * 7, hipMemcpyDeviceToHost, stream));
+        EAFP_TRY(hipStreamSynchronize(stream));
+        for (int q = 0; q < 7; ++q) hc[q] = hbuf.data() + (size_t)q * np;
+    } else {
+        for (int q = 0; q < 7; ++q) hc[q] = host[q];
+    }
+    std::vector<unsigned char> slot(g.n);
+    EAFP_TRY_RC(analyze_components(g, hc, slot.data(), comp_info[s]));
+    EAFP_TRY(hipMemcpyAsync(comp.p + g.off, slot.data(), g.n, hipMemcpyHostToDevice, stream));
     EAFP_TRY(hipStreamSynchronize(stream));
     uploaded[s] = 1;
+    return EAFP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Connected components of the link graph (union-find over nonzero links,
+// either side) and their singularity: a component is singular (constant
+// null vector) when every row has aP equal to the sum of its interior
+// links up to rounding, i.e. no compliance, acoustic, Dirichlet or wall
+// term on its diagonal.  Rows without any link are never projected.
+// ---------------------------------------------------------------------------
+namespace {
+int uf_find(std::vector<int>& p, int a) {
+    while (p[a] != a) {
+        p[a] = p[p[a]];
+        a = p[a];
+    }
+    return a;
+}
+}  // namespace
+
+int analyze_components(const SysGeom& g, const double* const A[7], unsigned char* slot, CompInfo& info) {
+    const long sy = g.nr + 4, sz = (long)(g.nr + 4) * (g.nth + 4);
+    auto po = [&](int i, int j, int k) { return (long)(i + 2) + sy * (j + 2) + sz * (k + 2); };
+    auto cidx = [&](int i, int j, int k) { return (i * g.nth + j) * g.nz + k; };
+    std::vector<int> parent(g.n);
+    std::vector<char> linked(g.n, 0), neumann(g.n, 1);
+    for (int c = 0; c < g.n; ++c) parent[c] = c;
+    auto unite = [&](int a, int b) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a != b) parent[a < b ? b : a] = a < b ? a : b;
+    };
+    for (int i = 0; i < g.nr; ++i)
+        for (int j = 0; j < g.nth; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                const long o = po(i, j, k);
+                const int c = cidx(i, j, k);
+                const int jn = (j + 1) % g.nth;
+                double links = 0.0;
+                if (i + 1 < g.nr && (A[C_AE][o] != 0.0 || A[C_AW][po(i + 1, j, k)] != 0.0)) {
+                    unite(c, cidx(i + 1, j, k));
+                    linked[c] = linked[cidx(i + 1, j, k)] = 1;
+                }
+                if (A[C_AN][o] != 0.0 || A[C_AS][po(i, jn, k)] != 0.0) {
+                    unite(c, cidx(i, jn, k));
+                    linked[c] = linked[cidx(i, jn, k)] = 1;
+                }
+                if (k + 1 < g.nz && (A[C_AT][o] != 0.0 || A[C_AB][po(i, j, k + 1)] != 0.0)) {
+                    unite(c, cidx(i, j, k + 1));
+                    linked[c] = linked[cidx(i, j, k + 1)] = 1;
+                }
+                if (i > 0) links += A[C_AW][o];
+                if (i + 1 < g.nr) links += A[C_AE][o];
+                links += A[C_AS][o] + A[C_AN][o];
+                if (k > 0) links += A[C_AB][o];
+                if (k + 1 < g.nz) links += A[C_AT][o];
+                const double ap = A[C_AP][o];
+                const bool wall = (i == 0 && A[C_AW][o] != 0.0) || (i + 1 == g.nr && A[C_AE][o] != 0.0) ||
+                                  (k == 0 && A[C_AB][o] != 0.0) || (k + 1 == g.nz && A[C_AT][o] != 0.0);
+                neumann[c] = !wall && std::fabs(ap - links) <= 1e-10 * std::fabs(ap);
+            }
+    // a component is singular iff all its rows are Neumann rows
+    std::vector<char> root_sing(g.n, 1);
+    for (int c = 0; c < g.n; ++c) {
+        const int r = uf_find(parent, c);
+        if (!linked[c] || !neumann[c]) root_sing[r] = 0;
+    }
+    std::vector<int> root_slot(g.n, -1);
+    std::vector<long> cnt;
+    info = CompInfo{};
+    for (int c = 0; c < g.n; ++c) {
+        slot[c] = NO_COMP;
+        if (!linked[c]) continue;
+        const int r = uf_find(parent, c);
+        if (!root_sing[r]) continue;
+        if (root_slot[r] < 0) {
+            root_slot[r] = (int)cnt.size();
+            cnt.push_back(0);
+        }
+        const int sl = root_slot[r];
+        ++cnt[sl];
+        if (sl < MAX_COMP) slot[c] = (unsigned char)sl;
+    }
+    info.ncomp = (int)std::min<size_t>(cnt.size(), MAX_COMP);
+    info.overflow = cnt.size() > (size_t)MAX_COMP;
+    for (int q = 0; q < info.ncomp; ++q) info.inv_cnt[q] = 1.0 / (double)cnt[q];
     return EAFP_OK;
 }
 

@@ -54,6 +54,17 @@ struct DevBuf {
     }
 };
 
+// Singular (pure-Neumann) connected components of one system, found on the
+// host at upload: the mixed solver projects its residuals and its result to
+// zero mean on each of them.
+struct CompInfo {
+    int ncomp = 0;                 // projected components (<= MAX_COMP)
+    int overflow = 0;              // more singular components than MAX_COMP
+    double inv_cnt[MAX_COMP] = {}; // 1 / cells of each
+};
+
+int analyze_components(const SysGeom& g, const double* const padded[7], unsigned char* slot, CompInfo& info);
+
 struct Cg64Work;     // cg64.hip
 struct MixedWork;    // mixed.hip
 
@@ -65,11 +76,16 @@ struct Batch {
     DevBuf<SysGeom> geom_d;
     long ncells = 0;
     int ntiles = 0;
-    std::vector<int> tile_sys_h, tile_start_h;
-    DevBuf<int> tile_sys, tile_start;    // per tile: system id, first local cell
+    int cpt = 2;                         // cells per thread; tile = BLOCK*cpt cells
+    int tiling_gen = 0;                  // bumped whenever tiles or grid change
+    std::vector<TileDesc> tiles_h;
+    DevBuf<TileDesc> tiles;
     DevBuf<double> coef[NCOEF];          // FP64 operand arenas, compact
     DevBuf<double> b;                    // Su plus the folded wall-halo terms of phi0
     DevBuf<double> x0;                   // initial phi interior
+    DevBuf<double> fold;                 // wall-halo terms of phi0 (r = Su + fold - A_int*x)
+    DevBuf<unsigned char> comp;          // projected-component slot per cell (NO_COMP: none)
+    std::vector<CompInfo> comp_info;
     std::vector<DevBuf<double>*> phi_pad;  // padded phi per system (device)
     std::vector<char> uploaded;
     DevBuf<double> staging;              // padded staging for uploads
@@ -80,6 +96,7 @@ struct Batch {
 
     ~Batch();
     int init(int n, const eafp_desc* descs);
+    int set_tiling(int cells_per_thread, int blocks);
     int upload(int s, const double* const host[8], const double* phi0, int arrays_on_device);
     int download(int s, double* phi_host, int to_device);
     long padded_len(int s) const {
@@ -90,5 +107,7 @@ struct Batch {
 
 int validate_desc(const eafp_desc* d);
 int choose_grid(int ntiles);
+int device_cus();
+bool valid_cpt(int cpt);
 
 }  // namespace eafp

@@ -9,6 +9,8 @@
 namespace eafp {
 int solve_fp64(Batch& B, int max_iter, double tol, eafp_stats* stats, int chunk);
 int solve_mixed(Batch& B, const eafp_mixed_opts& o, eafp_stats* stats);
+int time_kernels(Batch& B, int solver, int fmt, int reps, float* ms_spmv, float* ms_update);
+int autotune(Batch& B, int solver, int fmt, int reps, int* best_cpt, int* best_blocks, float* best_ms);
 }  // namespace eafp
 
 struct eafp_batch {
@@ -102,10 +104,24 @@ int eafp_batch_solve_fp64(eafp_batch* b, int max_iter, double tol, eafp_stats* s
 
 int eafp_batch_solve_mixed(eafp_batch* b, const eafp_mixed_opts* opts, eafp_stats* stats) {
     if (!b || !opts) return EAFP_ERR_ARG;
-    if (!(opts->tol > 0.0) || !(opts->inner_tol > 0.0) || opts->max_inner < 1 || opts->max_refine < 1)
-        return EAFP_ERR_ARG;
-    if (opts->coef_format < EAFP_COEF_FP32 || opts->coef_format > EAFP_COEF_FP16) return EAFP_ERR_ARG;
     return eafp::solve_mixed(b->B, *opts, stats);
+}
+
+int eafp_batch_set_tiling(eafp_batch* b, int cells_per_thread, int blocks) {
+    if (!b) return EAFP_ERR_ARG;
+    return b->B.set_tiling(cells_per_thread, blocks);
+}
+
+int eafp_batch_time_kernels(eafp_batch* b, int solver, int coef_format, int reps, float* ms_spmv,
+                            float* ms_update) {
+    if (!b || reps < 1) return EAFP_ERR_ARG;
+    return eafp::time_kernels(b->B, solver, coef_format, reps, ms_spmv, ms_update);
+}
+
+int eafp_batch_autotune(eafp_batch* b, int solver, int coef_format, int reps, int* cells_per_thread,
+                        int* blocks, float* ms_iter) {
+    if (!b || reps < 1) return EAFP_ERR_ARG;
+    return eafp::autotune(b->B, solver, coef_format, reps, cells_per_thread, blocks, ms_iter);
 }
 
 int eafp_batch_download(eafp_batch* b, int s, double* phi, int to_device) {
@@ -116,7 +132,7 @@ int eafp_batch_download(eafp_batch* b, int s, double* phi, int to_device) {
 int eafp_batch_info(const eafp_batch* b, int* grid, int* tile_cells, long* ncells) {
     if (!b) return EAFP_ERR_ARG;
     if (grid) *grid = b->B.grid;
-    if (tile_cells) *tile_cells = eafp::TILE_CELLS;
+    if (tile_cells) *tile_cells = eafp::BLOCK * b->B.cpt;
     if (ncells) *ncells = b->B.ncells;
     return EAFP_OK;
 }

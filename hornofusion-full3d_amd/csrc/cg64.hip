@@ -3,62 +3,75 @@
 //
 // Per iteration two kernels, each one streaming pass over the cells:
 //
-//   k64_spmv    pv_new = z + beta*pv_old (recomputed at the 6 neighbours, so
-//               no separate xpay pass), q = A*pv_new, partial p.q
-//               finisher: |p.q| < SMALL -> breakdown, else alpha = rz/p.q
-//   k64_update  phi += alpha*pv_new, r -= alpha*q, z = r/max(|aP|,SMALL),
+//   k64_spmv    closes the previous iteration from the partial sums that
+//               k64_update left (residual, stop tests, beta), then
+//               pv_new = z + beta*pv_old (recomputed at the 6 neighbours,
+//               so no separate xpay pass), q = A*pv_new, partials p.q
+//   k64_update  from the p.q partials: breakdown test, alpha; then
+//               phi += alpha*pv_new, r -= alpha*q, z = r/max(|aP|,SMALL),
 //               partials r.r and r.z
-//               finisher: residual = ||r||/||Su||, n_iter = it, stop test,
-//               beta = rz_new/max(rz,SMALL)
 //
-// The scalar recurrences follow the reference line by line (same tests in
-// the same order, including the NaN behaviour: a NaN residual never passes
+// Every block reduces the partials it needs itself (same tile order, same
+// bits in every block), so no block waits on another inside a kernel.  The
+// scalar recurrences follow the reference line by line (same tests in the
+// same order, including the NaN behaviour: a NaN residual never passes
 // `residual < tol`).  pv is double-buffered because neighbours read the old
-// value while the owner writes the new one.  Iterations are replayed from
-// a captured hipGraph in chunks; every kernel of a finished system returns
-// at once, and the host polls the number of finished systems between
-// chunks.
+// value while the owner writes the new one; the buffer parity is a launch
+// argument (chunks hold an even number of iterations and every system
+// starts iterating at the same launch).  Iterations are replayed from a
+// captured hipGraph in chunks; every kernel of a finished system skips its
+// tiles, and a one-block kernel at the end of each chunk counts finished
+// systems for the host.
 //
 // Algorithmic HBM bytes per cell and iteration (FP64): k64_spmv reads 7
 // coefficients + z + pv_old and writes pv_new + q (88 B); k64_update reads
 // phi, r, pv_new, q, aP and writes phi, r, z (64 B): 152 B/cell/iter.
-#include <chrono>
 #include <vector>
 
 #include "batch.hpp"
 
 namespace eafp {
 
-struct CgState {
-    double bnorm, residual, rz, alpha, beta, tol;
-    int n_iter, max_iter, done, status;
-    unsigned counter;
-    unsigned pad;
+struct CgSlot {
+    double bnorm;      // ||Su||; < 0 until the first k64_spmv after k64_init
+    double residual;   // relative recurrence residual of the last completed iteration
+    double rz, alpha, beta;
+    int it;            // iteration in progress (set by k64_spmv)
+    int n_iter;        // completed iterations
+    int done, status;
+};
+
+struct CgConst {
+    double tol;
+    int max_iter;
+    int pad;
 };
 
 struct Cg64Args {
-    const SysGeom* geom;
-    const int* tile_sys;
-    const int* tile_start;
+    const TileDesc* tiles;
     int ntiles;
+    int nsys;
     const double *aW, *aE, *aS, *aN, *aB, *aT, *aP;
-    const double *b, *x0;
+    const double *b, *fold, *x0;
     double *x, *r, *z, *q;
     double* p[2];
-    double* partial;     // ntiles * 3
-    CgState* st;
-    int* ndone;
+    double* partA;       // ntiles:     p.q
+    double* partB;       // ntiles * 3: r.r, r.z, Su.Su
+    CgSlot* slot[2];
+    const CgConst* cst;
+    int* ndone;          // finished systems, counted after each chunk
 };
 
 struct Cg64Work {
-    DevBuf<double> x, r, z, q, p0, p1, partial;
-    DevBuf<CgState> st;
+    DevBuf<double> x, r, z, q, p0, p1, partA, partB;
+    DevBuf<CgSlot> slot0, slot1;
+    DevBuf<CgConst> cst;
     DevBuf<int> ndone;
     int* ndone_host = nullptr;     // pinned, 2 slots
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t t0 = nullptr, t1 = nullptr;
     hipGraphExec_t exec = nullptr;
-    int exec_chunk = 0;
+    int exec_chunk = 0, exec_gen = -1;
     Cg64Args args;
 };
 
@@ -75,8 +88,9 @@ void free_cg64(Cg64Work* w) {
 
 int download_x(Batch& B, int s, const double* x_compact);
 
-// Value of the stencil neighbour at compact offset o + d, or 0 outside
-// the r/z walls (the zeroed halo of the reference's work vectors).
+// Compact offsets of the six stencil neighbours; the theta pair wraps, the
+// r/z pairs are flagged off at the walls (the zeroed halo of the
+// reference's work vectors contributes nothing there).
 #define EAFP_NEIGHBOURS(id, g, o)                                           \
     const long sr_ = (long)(g).nth * (g).nz;                                \
     const long oW = (o) - sr_, oE = (o) + sr_;                              \
@@ -87,225 +101,231 @@ int download_x(Batch& B, int s, const double* x_compact);
     const bool hB = (id).k > 0, hT = (id).k < (g).nz - 1;
 
 // ---------------------------------------------------------------------------
-// r = Su - A*phi0, z = r/max(|aP|,SMALL), partials Su.Su, r.r, r.z
-// (src/mod_solver_3d.f90:509-530).
+// r = Su - A*phi0, z = r/max(|aP|,SMALL), pv_old = 0, partials r.r, r.z,
+// Su.Su (src/mod_solver_3d.f90:509-530).  Writes the fresh slot 1.
 // ---------------------------------------------------------------------------
+template <int CPT>
 __global__ __launch_bounds__(BLOCK) void k64_init(Cg64Args a) {
+    constexpr int TILE = BLOCK * CPT;
     __shared__ double sh[3 * (BLOCK / 64)];
-    __shared__ int flag;
-    for (TileWalk w(a.ntiles); w.valid(); w.next()) {
-        const int s = a.tile_sys[w.t];
-        const SysGeom g = a.geom[s];
-        const int start = a.tile_start[w.t];
+    const TileRange tr(a.ntiles);
+    for (int t = tr.t; t < tr.end;) {
+        const TileDesc T = a.tiles[t];
+        const int run_end = min(tr.end, T.tile_end);
+        if (t == T.tile_begin && threadIdx.x == 0) {
+            CgSlot n{};
+            n.bnorm = -1.0;
+            a.slot[1][T.sys] = n;
+        }
         double acc[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-        for (int m = 0; m < CELLS_PER_THREAD; ++m) {
-            const int c = start + m * BLOCK + (int)threadIdx.x;
-            if (c < g.n) {
-                const CellIdx id = decompose(c, g);
-                const long o = g.off + c;
-                EAFP_NEIGHBOURS(id, g, o)
-                const double* x = a.x0;
-                const double xW = hW ? x[oW] : 0.0, xE = hE ? x[oE] : 0.0;
-                const double xB = hB ? x[oB] : 0.0, xT = hT ? x[oT] : 0.0;
-                const double ap = a.aP[o];
-                const double ax = ap * x[o] - a.aW[o] * xW - a.aE[o] * xE - a.aS[o] * x[oS] -
-                                  a.aN[o] * x[oN] - a.aB[o] * xB - a.aT[o] * xT;
-                const double su = a.b[o];
-                const double rr = su - ax;
-                const double zz = rr / fmax(fabs(ap), SMALL);
-                a.x[o] = x[o];
-                a.r[o] = rr;
-                a.z[o] = zz;
-                acc[0] += su * su;
-                acc[1] += rr * rr;
-                acc[2] += rr * zz;
-            }
+        const int c_end = min(T.n, (run_end - T.tile_begin) * TILE);
+        for (int c = (t - T.tile_begin) * TILE + (int)threadIdx.x; c < c_end; c += BLOCK) {
+            const CellIdx id = decompose(c, T);
+            const long o = T.off + c;
+            EAFP_NEIGHBOURS(id, T, o)
+            const double* x = a.x0;
+            const double xW = hW ? x[oW] : 0.0, xE = hE ? x[oE] : 0.0;
+            const double xB = hB ? x[oB] : 0.0, xT = hT ? x[oT] : 0.0;
+            const double ap = a.aP[o];
+            const double ax = ap * x[o] - a.aW[o] * xW - a.aE[o] * xE - a.aS[o] * x[oS] -
+                              a.aN[o] * x[oN] - a.aB[o] * xB - a.aT[o] * xT;
+            const double su = a.b[o];
+            const double rr = su - (ax - a.fold[o]);
+            const double zz = rr / fmax(fabs(ap), SMALL);
+            a.x[o] = x[o];
+            a.r[o] = rr;
+            a.z[o] = zz;
+            a.p[0][o] = 0.0;
+            acc[0] += rr * rr;
+            acc[1] += rr * zz;
+            acc[2] += su * su;
         }
         block_sum<3>(acc, sh);
-        if (threadIdx.x == 0) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) a.partial[(long)w.t * 3 + c] = acc[c];
-        }
-        CgState& S = a.st[s];
-        if (arrive_last(&S.counter, (unsigned)(g.tile_end - g.tile_begin), &flag)) {
+        store_run<3>(a.partB, t, run_end, acc);
+        t = run_end;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Role 0 (reads slot 1, writes slot 0).  Closes the previous iteration --
+// residual, stop tests, beta (src/mod_solver_3d.f90:545-555) -- then
+// pv = z + beta*pv ; q = A*pv ; p.q  (:533-534, 556-558).
+// ---------------------------------------------------------------------------
+template <int CPT>
+__global__ __launch_bounds__(BLOCK) void k64_spmv(Cg64Args a, int par) {
+    constexpr int TILE = BLOCK * CPT;
+    __shared__ double sh[3 * (BLOCK / 64)];
+    const double* __restrict__ pold = a.p[par];
+    double* __restrict__ pnew = a.p[par ^ 1];
+    const double* __restrict__ z = a.z;
+    const TileRange tr(a.ntiles);
+    for (int t = tr.t; t < tr.end;) {
+        const TileDesc T = a.tiles[t];
+        const int s = T.sys;
+        const int run_end = min(tr.end, T.tile_end);
+        const CgSlot P = a.slot[1][s];
+        CgSlot N = P;
+        if (!P.done) {
+            const CgConst K = a.cst[s];
             double tot[3];
-            reduce_tiles<3>(a.partial, 3, g.tile_begin, g.tile_end, tot, sh);
-            if (threadIdx.x == 0) {
-                S.counter = 0;
-                const double bnorm = sqrt(tot[0]);
-                S.bnorm = bnorm;
-                S.n_iter = 0;
-                S.beta = 0.0;
-                if (bnorm < SMALL) {
-                    S.residual = 0.0;
-                    S.done = 1;
-                    S.status = EAFP_STOP_ZERO_RHS;
-                    atomicAdd(a.ndone, 1);
+            reduce_tiles<3>(a.partB, T.tile_begin, T.tile_end, tot, sh);
+            if (P.bnorm < 0.0) {           // first iteration after k64_init
+                N.bnorm = sqrt(tot[2]);
+                N.n_iter = 0;
+                N.beta = 0.0;
+                if (N.bnorm < SMALL) {
+                    N.residual = 0.0;
+                    N.done = 1;
+                    N.status = EAFP_STOP_ZERO_RHS;
                 } else {
-                    S.rz = tot[2];
-                    S.residual = sqrt(tot[1]) / bnorm;
-                    if (S.max_iter <= 0) {
-                        S.done = 1;
-                        S.status = EAFP_STOP_MAX_ITER;
-                        atomicAdd(a.ndone, 1);
+                    N.residual = sqrt(tot[0]) / N.bnorm;
+                    N.rz = tot[1];
+                    if (K.max_iter <= 0) {
+                        N.done = 1;
+                        N.status = EAFP_STOP_MAX_ITER;
                     }
                 }
+            } else {                       // iteration P.it has just been updated
+                N.residual = sqrt(tot[0]) / P.bnorm;
+                N.n_iter = P.it;
+                if (N.residual < K.tol) {
+                    N.done = 1;
+                    N.status = EAFP_STOP_CONVERGED;
+                } else if (P.it >= K.max_iter) {
+                    N.done = 1;
+                    N.status = EAFP_STOP_MAX_ITER;
+                } else {
+                    N.beta = tot[1] / fmax(P.rz, SMALL);
+                    N.rz = tot[1];
+                }
             }
+            N.it = P.it + 1;
         }
+        if (t == T.tile_begin && threadIdx.x == 0) a.slot[0][s] = N;
+        if (!N.done) {
+            const double beta = N.beta;
+            double acc[1] = {0.0};
+            const int c_end = min(T.n, (run_end - T.tile_begin) * TILE);
+            for (int c0 = (t - T.tile_begin) * TILE + (int)threadIdx.x; c0 < c_end; c0 += TILE) {
+#pragma unroll
+                for (int m = 0; m < CPT; ++m) {
+                    const int c = c0 + m * BLOCK;
+                    if (c >= c_end) break;
+                    const CellIdx id = decompose(c, T);
+                    const long o = T.off + c;
+                    EAFP_NEIGHBOURS(id, T, o)
+                    auto pv = [&](long q) { return z[q] + beta * pold[q]; };
+                    const double pc = pv(o);
+                    const double pW = hW ? pv(oW) : 0.0, pE = hE ? pv(oE) : 0.0;
+                    const double pB = hB ? pv(oB) : 0.0, pT = hT ? pv(oT) : 0.0;
+                    const double pS = pv(oS), pN = pv(oN);
+                    const double y = a.aP[o] * pc - a.aW[o] * pW - a.aE[o] * pE - a.aS[o] * pS -
+                                     a.aN[o] * pN - a.aB[o] * pB - a.aT[o] * pT;
+                    pnew[o] = pc;
+                    a.q[o] = y;
+                    acc[0] += pc * y;
+                }
+            }
+            block_sum<1>(acc, sh);
+            store_run<1>(a.partA, t, run_end, acc);
+        }
+        t = run_end;
     }
 }
 
 // ---------------------------------------------------------------------------
-// pv = z + beta*pv ; q = A*pv ; p.q  (src/mod_solver_3d.f90:533-537,556-558)
+// Role 1 (reads slot 0, writes slot 1).  p.q, breakdown exit, alpha
+// (src/mod_solver_3d.f90:534-536), then phi += alpha*pv ; r -= alpha*q ;
+// z = r/max(|aP|,SMALL) ; partials r.r, r.z  (:538-553).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k64_spmv(Cg64Args a) {
-    __shared__ double sh[1 * (BLOCK / 64)];
-    __shared__ int flag;
-    for (TileWalk w(a.ntiles); w.valid(); w.next()) {
-        const int s = a.tile_sys[w.t];
-        CgState& S = a.st[s];
-        if (S.done) continue;
-        const SysGeom g = a.geom[s];
-        const int start = a.tile_start[w.t];
-        const int it = S.n_iter + 1;
-        const double beta = S.beta;
-        const bool first = (it == 1);
-        const double* __restrict__ pold = a.p[(it - 1) & 1];
-        double* __restrict__ pnew = a.p[it & 1];
-        const double* __restrict__ z = a.z;
-        double acc[1] = {0.0};
-#pragma unroll
-        for (int m = 0; m < CELLS_PER_THREAD; ++m) {
-            const int c = start + m * BLOCK + (int)threadIdx.x;
-            if (c < g.n) {
-                const CellIdx id = decompose(c, g);
-                const long o = g.off + c;
-                EAFP_NEIGHBOURS(id, g, o)
-                auto pv = [&](long q) { return first ? z[q] : z[q] + beta * pold[q]; };
-                const double pc = pv(o);
-                const double pW = hW ? pv(oW) : 0.0, pE = hE ? pv(oE) : 0.0;
-                const double pB = hB ? pv(oB) : 0.0, pT = hT ? pv(oT) : 0.0;
-                const double pS = pv(oS), pN = pv(oN);
-                const double y = a.aP[o] * pc - a.aW[o] * pW - a.aE[o] * pE - a.aS[o] * pS -
-                                 a.aN[o] * pN - a.aB[o] * pB - a.aT[o] * pT;
-                pnew[o] = pc;
-                a.q[o] = y;
-                acc[0] += pc * y;
-            }
-        }
-        block_sum<1>(acc, sh);
-        if (threadIdx.x == 0) a.partial[(long)w.t * 3] = acc[0];
-        if (arrive_last(&S.counter, (unsigned)(g.tile_end - g.tile_begin), &flag)) {
+template <int CPT>
+__global__ __launch_bounds__(BLOCK) void k64_update(Cg64Args a, int par) {
+    constexpr int TILE = BLOCK * CPT;
+    __shared__ double sh[3 * (BLOCK / 64)];
+    const double* __restrict__ pv = a.p[par ^ 1];
+    const TileRange tr(a.ntiles);
+    for (int t = tr.t; t < tr.end;) {
+        const TileDesc T = a.tiles[t];
+        const int s = T.sys;
+        const int run_end = min(tr.end, T.tile_end);
+        const CgSlot P = a.slot[0][s];
+        CgSlot N = P;
+        if (!P.done) {
             double tot[1];
-            reduce_tiles<1>(a.partial, 3, g.tile_begin, g.tile_end, tot, sh);
-            if (threadIdx.x == 0) {
-                S.counter = 0;
-                const double pq = tot[0];
-                if (fabs(pq) < SMALL) {
-                    S.done = 1;
-                    S.status = EAFP_STOP_BREAKDOWN;
-                    atomicAdd(a.ndone, 1);
-                } else {
-                    S.alpha = S.rz / pq;
-                }
+            reduce_tiles<1>(a.partA, T.tile_begin, T.tile_end, tot, sh);
+            const double pq = tot[0];
+            if (fabs(pq) < SMALL) {
+                N.done = 1;
+                N.status = EAFP_STOP_BREAKDOWN;
+            } else {
+                N.alpha = P.rz / pq;
             }
         }
+        if (t == T.tile_begin && threadIdx.x == 0) a.slot[1][s] = N;
+        if (!N.done) {
+            const double alpha = N.alpha;
+            double acc[3] = {0.0, 0.0, 0.0};
+            const int c_end = min(T.n, (run_end - T.tile_begin) * TILE);
+            for (int c0 = (t - T.tile_begin) * TILE + (int)threadIdx.x; c0 < c_end; c0 += TILE) {
+#pragma unroll
+                for (int m = 0; m < CPT; ++m) {
+                    const int c = c0 + m * BLOCK;
+                    if (c >= c_end) break;
+                    const long o = T.off + c;
+                    a.x[o] = a.x[o] + alpha * pv[o];
+                    const double rr = a.r[o] - alpha * a.q[o];
+                    const double zz = rr / fmax(fabs(a.aP[o]), SMALL);
+                    a.r[o] = rr;
+                    a.z[o] = zz;
+                    acc[0] += rr * rr;
+                    acc[1] += rr * zz;
+                }
+            }
+            block_sum<3>(acc, sh);
+            store_run<3>(a.partB, t, run_end, acc);
+        }
+        t = run_end;
     }
 }
 
-// ---------------------------------------------------------------------------
-// phi += alpha*pv ; r -= alpha*q ; residual ; z = r/max(|aP|,SMALL) ; r.z ;
-// beta  (src/mod_solver_3d.f90:538-555)
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k64_update(Cg64Args a) {
-    __shared__ double sh[2 * (BLOCK / 64)];
-    __shared__ int flag;
-    for (TileWalk w(a.ntiles); w.valid(); w.next()) {
-        const int s = a.tile_sys[w.t];
-        CgState& S = a.st[s];
-        if (S.done) continue;
-        const SysGeom g = a.geom[s];
-        const int start = a.tile_start[w.t];
-        const int it = S.n_iter + 1;
-        const double alpha = S.alpha;
-        const double* __restrict__ pv = a.p[it & 1];
-        double acc[2] = {0.0, 0.0};
-#pragma unroll
-        for (int m = 0; m < CELLS_PER_THREAD; ++m) {
-            const int c = start + m * BLOCK + (int)threadIdx.x;
-            if (c < g.n) {
-                const long o = g.off + c;
-                a.x[o] = a.x[o] + alpha * pv[o];
-                const double rr = a.r[o] - alpha * a.q[o];
-                const double zz = rr / fmax(fabs(a.aP[o]), SMALL);
-                a.r[o] = rr;
-                a.z[o] = zz;
-                acc[0] += rr * rr;
-                acc[1] += rr * zz;
-            }
-        }
-        block_sum<2>(acc, sh);
-        if (threadIdx.x == 0) {
-            a.partial[(long)w.t * 3 + 1] = acc[0];
-            a.partial[(long)w.t * 3 + 2] = acc[1];
-        }
-        if (arrive_last(&S.counter, (unsigned)(g.tile_end - g.tile_begin), &flag)) {
-            double tot[2];
-            // partial column 1..2 of every tile
-            tot[0] = 0.0;
-            tot[1] = 0.0;
-            for (int t = g.tile_begin + (int)threadIdx.x; t < g.tile_end; t += BLOCK) {
-                tot[0] += a.partial[(long)t * 3 + 1];
-                tot[1] += a.partial[(long)t * 3 + 2];
-            }
-            block_sum<2>(tot, sh);
-            if (threadIdx.x == 0) {
-                S.counter = 0;
-                const double res = sqrt(tot[0]) / S.bnorm;
-                S.residual = res;
-                S.n_iter = it;
-                if (res < S.tol) {
-                    S.done = 1;
-                    S.status = EAFP_STOP_CONVERGED;
-                    atomicAdd(a.ndone, 1);
-                } else {
-                    S.beta = tot[1] / fmax(S.rz, SMALL);
-                    S.rz = tot[1];
-                    if (it >= S.max_iter) {
-                        S.done = 1;
-                        S.status = EAFP_STOP_MAX_ITER;
-                        atomicAdd(a.ndone, 1);
-                    }
-                }
-            }
-        }
-    }
+// Number of finished systems in slot 1 (one block).
+__global__ __launch_bounds__(BLOCK) void k64_count(Cg64Args a) {
+    __shared__ double sh[BLOCK / 64];
+    double v[1] = {0.0};
+    for (int s = threadIdx.x; s < a.nsys; s += BLOCK) v[0] += a.slot[1][s].done ? 1.0 : 0.0;
+    block_sum<1>(v, sh);
+    if (threadIdx.x == 0) *a.ndone = (int)v[0];
 }
 
 static int ensure_cg64(Batch& B) {
-    if (B.cg64) return EAFP_OK;
-    Cg64Work* w = new Cg64Work();
-    B.cg64 = w;
-    const size_t n = B.ncells;
-    EAFP_TRY_RC(w->x.alloc(n));
-    EAFP_TRY_RC(w->r.alloc(n));
-    EAFP_TRY_RC(w->z.alloc(n));
-    EAFP_TRY_RC(w->q.alloc(n));
-    EAFP_TRY_RC(w->p0.alloc(n));
-    EAFP_TRY_RC(w->p1.alloc(n));
-    EAFP_TRY_RC(w->partial.alloc((size_t)B.ntiles * 3));
-    EAFP_TRY_RC(w->st.alloc(B.nsys));
-    EAFP_TRY_RC(w->ndone.alloc(1));
-    EAFP_TRY(hipHostMalloc((void**)&w->ndone_host, 2 * sizeof(int), hipHostMallocDefault));
-    for (auto& e : w->ev) EAFP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    EAFP_TRY(hipEventCreate(&w->t0));
-    EAFP_TRY(hipEventCreate(&w->t1));
+    if (!B.cg64) {
+        Cg64Work* w = new Cg64Work();
+        B.cg64 = w;
+        const size_t n = B.ncells;
+        EAFP_TRY_RC(w->x.alloc(n));
+        EAFP_TRY_RC(w->r.alloc(n));
+        EAFP_TRY_RC(w->z.alloc(n));
+        EAFP_TRY_RC(w->q.alloc(n));
+        EAFP_TRY_RC(w->p0.alloc(n));
+        EAFP_TRY_RC(w->p1.alloc(n));
+        EAFP_TRY_RC(w->slot0.alloc(B.nsys));
+        EAFP_TRY_RC(w->slot1.alloc(B.nsys));
+        EAFP_TRY_RC(w->cst.alloc(B.nsys));
+        EAFP_TRY_RC(w->ndone.alloc(1));
+        EAFP_TRY(hipHostMalloc((void**)&w->ndone_host, 2 * sizeof(int), hipHostMallocDefault));
+        for (auto& e : w->ev) EAFP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        EAFP_TRY(hipEventCreate(&w->t0));
+        EAFP_TRY(hipEventCreate(&w->t1));
+        EAFP_TRY(hipMemset(w->p0.p, 0, n * sizeof(double)));
+        EAFP_TRY(hipMemset(w->p1.p, 0, n * sizeof(double)));
+    }
+    Cg64Work* w = B.cg64;
+    EAFP_TRY_RC(w->partA.alloc((size_t)B.ntiles));
+    EAFP_TRY_RC(w->partB.alloc((size_t)B.ntiles * 3));
     Cg64Args& a = w->args;
-    a.geom = B.geom_d.p;
-    a.tile_sys = B.tile_sys.p;
-    a.tile_start = B.tile_start.p;
+    a.tiles = B.tiles.p;
     a.ntiles = B.ntiles;
+    a.nsys = B.nsys;
     a.aW = B.coef[C_AW].p;
     a.aE = B.coef[C_AE].p;
     a.aS = B.coef[C_AS].p;
@@ -314,6 +334,7 @@ static int ensure_cg64(Batch& B) {
     a.aT = B.coef[C_AT].p;
     a.aP = B.coef[C_AP].p;
     a.b = B.b.p;
+    a.fold = B.fold.p;
     a.x0 = B.x0.p;
     a.x = w->x.p;
     a.r = w->r.p;
@@ -321,19 +342,39 @@ static int ensure_cg64(Batch& B) {
     a.q = w->q.p;
     a.p[0] = w->p0.p;
     a.p[1] = w->p1.p;
-    a.partial = w->partial.p;
-    a.st = w->st.p;
+    a.partA = w->partA.p;
+    a.partB = w->partB.p;
+    a.slot[0] = w->slot0.p;
+    a.slot[1] = w->slot1.p;
+    a.cst = w->cst.p;
     a.ndone = w->ndone.p;
-    // zero halos of nothing: compact vectors have no halos, but keep the
-    // unused tail cells of each system (alignment padding) at zero
-    EAFP_TRY(hipMemset(w->p0.p, 0, n * sizeof(double)));
-    EAFP_TRY(hipMemset(w->p1.p, 0, n * sizeof(double)));
     return EAFP_OK;
 }
 
+template <int CPT>
+static void launch_spmv64(Batch& B, int par) {
+    k64_spmv<CPT><<<B.grid, BLOCK, 0, B.stream>>>(B.cg64->args, par);
+}
+template <int CPT>
+static void launch_update64(Batch& B, int par) {
+    k64_update<CPT><<<B.grid, BLOCK, 0, B.stream>>>(B.cg64->args, par);
+}
+template <int CPT>
+static void launch_init64_t(Batch& B) {
+    k64_init<CPT><<<B.grid, BLOCK, 0, B.stream>>>(B.cg64->args);
+}
+
+#define EAFP_CPT_DISPATCH(fn, ...)                  \
+    switch (B.cpt) {                                \
+        case 1: fn<1>(__VA_ARGS__); break;          \
+        case 4: fn<4>(__VA_ARGS__); break;          \
+        case 8: fn<8>(__VA_ARGS__); break;          \
+        default: fn<2>(__VA_ARGS__); break;         \
+    }
+
 static int cg64_graph(Batch& B, int chunk) {
     Cg64Work* w = B.cg64;
-    if (w->exec && w->exec_chunk == chunk) return EAFP_OK;
+    if (w->exec && w->exec_chunk == chunk && w->exec_gen == B.tiling_gen) return EAFP_OK;
     if (w->exec) {
         (void)hipGraphExecDestroy(w->exec);
         w->exec = nullptr;
@@ -341,15 +382,17 @@ static int cg64_graph(Batch& B, int chunk) {
     hipGraph_t graph = nullptr;
     EAFP_TRY(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
     for (int c = 0; c < chunk; ++c) {
-        k64_spmv<<<B.grid, BLOCK, 0, B.stream>>>(w->args);
-        k64_update<<<B.grid, BLOCK, 0, B.stream>>>(w->args);
+        EAFP_CPT_DISPATCH(launch_spmv64, B, c & 1)
+        EAFP_CPT_DISPATCH(launch_update64, B, c & 1)
     }
+    k64_count<<<1, BLOCK, 0, B.stream>>>(w->args);
     hipError_t e = hipStreamEndCapture(B.stream, &graph);
     if (e != hipSuccess) return (int)e;
     e = hipGraphInstantiate(&w->exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return (int)e;
     w->exec_chunk = chunk;
+    w->exec_gen = B.tiling_gen;
     return EAFP_OK;
 }
 
@@ -359,23 +402,25 @@ int solve_fp64(Batch& B, int max_iter, double tol, eafp_stats* stats, int chunk)
     EAFP_TRY_RC(ensure_cg64(B));
     Cg64Work* w = B.cg64;
     if (chunk <= 0) chunk = 16;
+    chunk = (chunk + 1) & ~1;    // even: the pv buffer parity restarts every chunk
     EAFP_TRY_RC(cg64_graph(B, chunk));
 
-    std::vector<CgState> st(B.nsys);
-    for (auto& S : st) {
-        S = CgState{};
-        S.tol = tol;
-        S.max_iter = max_iter;
+    std::vector<CgConst> cst(B.nsys);
+    for (auto& K : cst) {
+        K = CgConst{};
+        K.tol = tol;
+        K.max_iter = max_iter;
     }
-    EAFP_TRY(hipMemcpyAsync(w->st.p, st.data(), sizeof(CgState) * B.nsys, hipMemcpyHostToDevice, B.stream));
-    EAFP_TRY(hipMemsetAsync(w->ndone.p, 0, sizeof(int), B.stream));
+    EAFP_TRY(hipMemcpyAsync(w->cst.p, cst.data(), sizeof(CgConst) * B.nsys, hipMemcpyHostToDevice, B.stream));
     EAFP_TRY(hipEventRecord(w->t0, B.stream));
-    k64_init<<<B.grid, BLOCK, 0, B.stream>>>(w->args);
+    EAFP_CPT_DISPATCH(launch_init64_t, B)
     EAFP_TRY(hipGetLastError());
 
     // Chunks of `chunk` iterations; poll the finished count of the chunk
-    // before the one just launched so the queue never drains.
-    const long max_chunks = (long)(max_iter > 0 ? max_iter : 0) / chunk + 2;
+    // before the one just launched so the queue never drains.  A system is
+    // marked finished by the k64_spmv after its last update, so one chunk
+    // past max_iter is always enough.
+    const long max_chunks = (long)(max_iter > 0 ? max_iter : 0) / chunk + 3;
     long launched = 0;
     for (;;) {
         if (launched < max_chunks) {
@@ -385,20 +430,20 @@ int solve_fp64(Batch& B, int max_iter, double tol, eafp_stats* stats, int chunk)
             EAFP_TRY(hipEventRecord(w->ev[slot], B.stream));
             ++launched;
         }
-        const int prev = (int)((launched - 1) & 1);
-        const int older = (int)(launched & 1);
         if (launched >= 2) {
+            const int older = (int)(launched & 1);
             EAFP_TRY(hipEventSynchronize(w->ev[older]));
             if (w->ndone_host[older] >= B.nsys) break;
         }
         if (launched >= max_chunks) {
-            EAFP_TRY(hipEventSynchronize(w->ev[prev]));
+            EAFP_TRY(hipEventSynchronize(w->ev[(launched - 1) & 1]));
             break;
         }
     }
     for (int s = 0; s < B.nsys; ++s) EAFP_TRY_RC(download_x(B, s, w->x.p));
     EAFP_TRY(hipEventRecord(w->t1, B.stream));
-    EAFP_TRY(hipMemcpyAsync(st.data(), w->st.p, sizeof(CgState) * B.nsys, hipMemcpyDeviceToHost, B.stream));
+    std::vector<CgSlot> st(B.nsys);
+    EAFP_TRY(hipMemcpyAsync(st.data(), w->slot1.p, sizeof(CgSlot) * B.nsys, hipMemcpyDeviceToHost, B.stream));
     EAFP_TRY(hipStreamSynchronize(B.stream));
     float ms = 0.f;
     EAFP_TRY(hipEventElapsedTime(&ms, w->t0, w->t1));
@@ -415,6 +460,46 @@ int solve_fp64(Batch& B, int max_iter, double tol, eafp_stats* stats, int chunk)
             o.seconds = ms * 1e-3;
         }
     }
+    return EAFP_OK;
+}
+
+// Average durations of the two iteration kernels over `reps` iterations
+// on the uploaded systems (kept running: tol = 0), each launch bracketed
+// by HIP events on the library stream.
+int time_fp64_kernels(Batch& B, int reps, float* ms_spmv, float* ms_update) {
+    for (int s = 0; s < B.nsys; ++s)
+        if (!B.uploaded[s]) return EAFP_ERR_STATE;
+    if (reps < 1) return EAFP_ERR_ARG;
+    EAFP_TRY_RC(ensure_cg64(B));
+    Cg64Work* w = B.cg64;
+    std::vector<CgConst> cst(B.nsys);
+    for (auto& K : cst) {
+        K = CgConst{};
+        K.tol = 0.0;
+        K.max_iter = 1 << 30;
+    }
+    EAFP_TRY(hipMemcpyAsync(w->cst.p, cst.data(), sizeof(CgConst) * B.nsys, hipMemcpyHostToDevice, B.stream));
+    EAFP_CPT_DISPATCH(launch_init64_t, B)
+    hipEvent_t e[3];
+    for (auto& x : e) EAFP_TRY(hipEventCreate(&x));
+    float tot_s = 0.f, tot_u = 0.f;
+    for (int r = 0; r < reps; ++r) {
+        const int par = r & 1;
+        EAFP_TRY(hipEventRecord(e[0], B.stream));
+        EAFP_CPT_DISPATCH(launch_spmv64, B, par)
+        EAFP_TRY(hipEventRecord(e[1], B.stream));
+        EAFP_CPT_DISPATCH(launch_update64, B, par)
+        EAFP_TRY(hipEventRecord(e[2], B.stream));
+        EAFP_TRY(hipEventSynchronize(e[2]));
+        float ta = 0.f, tb = 0.f;
+        EAFP_TRY(hipEventElapsedTime(&ta, e[0], e[1]));
+        EAFP_TRY(hipEventElapsedTime(&tb, e[1], e[2]));
+        tot_s += ta;
+        tot_u += tb;
+    }
+    for (auto& x : e) (void)hipEventDestroy(x);
+    if (ms_spmv) *ms_spmv = tot_s / (float)reps;
+    if (ms_update) *ms_update = tot_u / (float)reps;
     return EAFP_OK;
 }
 

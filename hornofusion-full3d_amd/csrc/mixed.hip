@@ -1,7 +1,810 @@
-// mixed.hip -- placeholder, replaced by the mixed-precision solver.
+// mixed.hip -- mixed-precision p' solver: FP32 Jacobi-PCG on compressed
+// symmetric face weights, inside FP64 iterative refinement.
+//
+// Setup (one pass, k_compress): for every cell the three faces it owns
+// (+r, +theta, +z) get one symmetric weight w = (a_own + a_neighbour)/2,
+// rounded to the storage format (FP32, BF16, or FP16 scaled by the owner's
+// diagonal aP); the diagonal excess e = aP - sum(interior links) (compliance,
+// acoustic, Dirichlet and wall terms; identity rows give e = aP) is kept in
+// FP32, and the Jacobi inverse 1/(sum of the six rounded weights + e).
+//
+// Inner operator, flux form:  (A_lo p)_c = sum_f w_f (p_c - p_nb) + e_c p_c.
+// It is an exact weighted graph Laplacian plus a diagonal: rounding the
+// weights keeps it symmetric and keeps the constant null vector of every
+// pure-Neumann component exact, so (1-d)A <= A_lo <= (1+d)A with d the
+// storage rounding.
+//
+// Refinement (FP64), every `interval` inner iterations for all systems at
+// once (k_res, k_mreplace): x += d, r = Su - A*x with the caller's seven
+// FP64 coefficient arrays, FP64 norms, stop test, per-component sums; the
+// inner residual is replaced by fl32(r - mean) on every singular
+// (pure-Neumann) component and d restarts from zero.  The inner CG keeps
+// its search direction across the replacement (the reliable-update form
+// of refinement: no Krylov restart, so the inner iteration count stays
+// close to the FP64 CG's), or restarts it (classical refinement) on
+// request.  Every dot product is accumulated in FP64.
+//
+// Inner iteration: two kernels as in cg64.hip, FP32 vectors.
+//   k_mspmv   beta, p = z + beta*p (recomputed at neighbours), q = A_lo p, p.q
+//   k_mupdate alpha, d += alpha p, r -= alpha q, z = r/D, r.r and r.z
+// Algorithmic bytes per cell and inner iteration: k_mspmv reads z, p_old,
+// 3 weights (+1 FP16 scale), e; writes p, q: 8+8+3w(+4)+4 B; k_mupdate
+// reads d, r, p, q, 1/D, writes d, r, z: 32 B.  FP32 weights: 64 B,
+// BF16: 58 B, FP16+scale: 62 B.
+//
+// One captured graph runs a whole refinement step for every system:
+// interval x (k_mspmv, k_mupdate), k_res, k_mreplace, k_mcount.
+#include <cmath>
+#include <vector>
+
 #include "batch.hpp"
+
 namespace eafp {
-struct MixedWork {};
-void free_mixed(MixedWork* w) { delete w; }
-int solve_mixed(Batch&, const eafp_mixed_opts&, eafp_stats*) { return EAFP_ERR_UNSUPPORTED; }
+
+struct MxSlot {
+    double bnorm, b1, res_l2, res_l1;
+    double mean_x[MAX_COMP];
+    double rz, alpha, beta;
+    int n_refine, done, status, xbuf;
+    int fresh;          // next k_mspmv starts a new Krylov sequence (beta = 0)
+    int broken;         // inner breakdown: idle until the next refinement step
+    int total_inner, pad;
+    double spare;       // size to a multiple of 16 bytes
+};
+
+struct MxConst {
+    double tol;
+    double inv_cnt[MAX_COMP];
+    int ncomp, max_steps, restart, pad;
+};
+
+constexpr int NRES = 4 + 2 * MAX_COMP;   // r.r, |r|, Su.Su, |Su|, sum r per comp, sum x per comp
+
+struct MxArgs {
+    const TileDesc* tiles;
+    int ntiles;
+    int nsys;
+    // FP64 operand
+    const double *aW, *aE, *aS, *aN, *aB, *aT, *aP, *b, *fold, *x0;
+    const unsigned char* comp;
+    double* X[2];        // FP64 iterate, ping-pong per refinement step
+    double* r64;         // FP64 residual of the current refinement step
+    double* xout;        // projected result
+    // compressed operand
+    void* w[3];          // +r, +theta, +z face weights (format dependent)
+    float* scale;        // FP16 only: per-cell scale (the owner's aP)
+    float* e;            // diagonal excess
+    float* dinv;         // Jacobi inverse of the rounded operator
+    // inner FP32 vectors
+    float *d, *r, *z, *q;
+    float* p[2];
+    double* partR;       // ntiles * NRES (k_res)
+    double* partA;       // ntiles        (k_mspmv: p.q)
+    double* partB;       // ntiles * 2    (k_mreplace, k_mupdate: r.r, r.z)
+    MxSlot* slot[2];
+    const MxConst* cst;
+    int* ndone;
+};
+
+// ---------------------------------------------------------------------------
+// Weight storage formats
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned short f32_to_bf16_rne(float v) {
+    unsigned u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+template <int F>
+struct WFmt;
+
+template <>
+struct WFmt<EAFP_COEF_FP32> {
+    using T = float;
+    __device__ static float load(const void* w, const float*, long o) { return ((const float*)w)[o]; }
+    __device__ static T enc(float v, float) { return v; }
+    __device__ static float dec(T h, float) { return h; }
+};
+
+template <>
+struct WFmt<EAFP_COEF_BF16> {
+    using T = unsigned short;
+    __device__ static float load(const void* w, const float*, long o) {
+        return __uint_as_float((unsigned)((const unsigned short*)w)[o] << 16);
+    }
+    __device__ static T enc(float v, float) { return f32_to_bf16_rne(v); }
+    __device__ static float dec(T h, float) { return __uint_as_float((unsigned)h << 16); }
+};
+
+template <>
+struct WFmt<EAFP_COEF_FP16> {
+    using T = _Float16;
+    __device__ static float load(const void* w, const float* sc, long o) {
+        return (float)((const _Float16*)w)[o] * sc[o];
+    }
+    __device__ static T enc(float v, float s) { return (_Float16)(v / s); }
+    __device__ static float dec(T h, float s) { return (float)h * s; }
+};
+
+size_t wbytes(int fmt) { return fmt == EAFP_COEF_FP32 ? 4 : 2; }
+
+#define EAFP_NB(id, g, o)                                                   \
+    const long sr_ = (long)(g).nth * (g).nz;                                \
+    const long oW = (o) - sr_, oE = (o) + sr_;                              \
+    const long oS = (id).j > 0 ? (o) - (g).nz : (o) + (long)((g).nth - 1) * (g).nz; \
+    const long oN = (id).j < (g).nth - 1 ? (o) + (g).nz : (o) - (long)((g).nth - 1) * (g).nz; \
+    const long oB = (o) - 1, oT = (o) + 1;                                  \
+    const bool hW = (id).i > 0, hE = (id).i < (g).nr - 1;                   \
+    const bool hB = (id).k > 0, hT = (id).k < (g).nz - 1;
+
+// Copy of a slot by one thread in 16-byte words (the struct never needs a
+// private-memory copy).
+__device__ __forceinline__ void copy_slot(MxSlot* dst, const MxSlot* src) {
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    uint4* d = reinterpret_cast<uint4*>(dst);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(MxSlot) / sizeof(uint4)); ++i) d[i] = s[i];
+}
+static_assert(sizeof(MxSlot) % sizeof(uint4) == 0, "slot copy in 16-byte words");
+
+// ---------------------------------------------------------------------------
+// Setup: symmetric face weights, excess, Jacobi inverse.
+// ---------------------------------------------------------------------------
+template <int F>
+__global__ __launch_bounds__(BLOCK) void k_compress(MxArgs a, int tile) {
+    using W = WFmt<F>;
+    const TileRange tr(a.ntiles);
+    for (int t = tr.t; t < tr.end;) {
+        const TileDesc T = a.tiles[t];
+        const int run_end = min(tr.end, T.tile_end);
+        const int c_end = min(T.n, (run_end - T.tile_begin) * tile);
+        for (int c = (t - T.tile_begin) * tile + (int)threadIdx.x; c < c_end; c += BLOCK) {
+            const CellIdx id = decompose(c, T);
+            const long o = T.off + c;
+            EAFP_NB(id, T, o)
+            const double ap = a.aP[o];
+            const double wr = hE ? 0.5 * (a.aE[o] + a.aW[oE]) : 0.0;
+            const double wrm = hW ? 0.5 * (a.aE[oW] + a.aW[o]) : 0.0;
+            const double wt = 0.5 * (a.aN[o] + a.aS[oN]);
+            const double wtm = 0.5 * (a.aN[oS] + a.aS[o]);
+            const double wz = hT ? 0.5 * (a.aT[o] + a.aB[oT]) : 0.0;
+            const double wzm = hB ? 0.5 * (a.aT[oB] + a.aB[o]) : 0.0;
+            const double e = ap - (wr + wrm + wt + wtm + wz + wzm);
+            // scale of a cell's own weights (FP16 only): its diagonal
+            auto sc = [&](long q) {
+                const float v = (float)a.aP[q];
+                return v > 0.f ? v : 1.f;
+            };
+            const float s_o = sc(o);
+            const typename W::T er = W::enc((float)wr, s_o), et = W::enc((float)wt, s_o),
+                                 ez = W::enc((float)wz, s_o);
+            // the rounded weights exactly as the inner operator sees them
+            float D = W::dec(er, s_o) + W::dec(et, s_o) + W::dec(ez, s_o);
+            if (hW) D += W::dec(W::enc((float)wrm, sc(oW)), sc(oW));
+            D += W::dec(W::enc((float)wtm, sc(oS)), sc(oS));
+            if (hB) D += W::dec(W::enc((float)wzm, sc(oB)), sc(oB));
+            const float e32 = (float)e;
+            D += e32;
+            ((typename W::T*)a.w[0])[o] = er;
+            ((typename W::T*)a.w[1])[o] = et;
+            ((typename W::T*)a.w[2])[o] = ez;
+            if (F == EAFP_COEF_FP16) a.scale[o] = s_o;
+            a.e[o] = e32;
+            a.dinv[o] = D > 0.f ? 1.0f / D : 0.0f;
+        }
+        t = run_end;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Refinement, part 1 (reads slot 1, writes slot 0): x_new = x + d,
+// r = Su - A*x_new with the caller's FP64 operator (cg_3d_mpi's matvec,
+// src/mod_solver_3d.f90:588-605), partials of norms and per-component sums.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_res(MxArgs a, int tile) {
+    __shared__ double sh[NRES * (BLOCK / 64)];
+    const TileRange tr(a.ntiles);
+    const float* __restrict__ d = a.d;
+    for (int t = tr.t; t < tr.end;) {
+        const TileDesc T = a.tiles[t];
+        const int s = T.sys;
+        const int run_end = min(tr.end, T.tile_end);
+        const MxSlot& P = a.slot[1][s];
+        const int done = P.done;
+        const int rnd = P.n_refine;
+        if (t == T.tile_begin && threadIdx.x == 0) copy_slot(&a.slot[0][s], &P);
+        if (!done) {
+            const bool first = (rnd == 0);
+            const double* __restrict__ xo = first ? a.x0 : a.X[rnd & 1];
+            double* __restrict__ xn = a.X[(rnd + 1) & 1];
+            double acc[NRES];
+#pragma unroll
+            for (int c = 0; c < NRES; ++c) acc[c] = 0.0;
+            const int c_end = min(T.n, (run_end - T.tile_begin) * tile);
+            for (int c = (t - T.tile_begin) * tile + (int)threadIdx.x; c < c_end; c += BLOCK) {
+                const CellIdx id = decompose(c, T);
+                const long o = T.off + c;
+                EAFP_NB(id, T, o)
+                auto xv = [&](long q) { return first ? xo[q] : xo[q] + (double)d[q]; };
+                const double xc = xv(o);
+                const double xW = hW ? xv(oW) : 0.0, xE = hE ? xv(oE) : 0.0;
+                const double xB = hB ? xv(oB) : 0.0, xT = hT ? xv(oT) : 0.0;
+                const double ax = a.aP[o] * xc - a.aW[o] * xW - a.aE[o] * xE - a.aS[o] * xv(oS) -
+                                  a.aN[o] * xv(oN) - a.aB[o] * xB - a.aT[o] * xT;
+                const double su = a.b[o];
+                const double res = su - (ax - a.fold[o]);
+                xn[o] = xc;
+                a.r64[o] = res;
+                acc[0] += res * res;
+                acc[1] += fabs(res);
+                acc[2] += su * su;
+                acc[3] += fabs(su);
+                const int slot = a.comp[o];
+#pragma unroll
+                for (int q = 0; q < MAX_COMP; ++q) {
+                    acc[4 + q] += slot == q ? res : 0.0;
+                    acc[4 + MAX_COMP + q] += slot == q ? xc : 0.0;
+                }
+            }
+            block_sum<NRES>(acc, sh);
+            store_run<NRES>(a.partR, t, run_end, acc);
+        }
+        t = run_end;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Refinement, part 2 (reads slot 0, writes slot 1): true residual, stop
+// tests, component means; then the inner residual is replaced by
+// r = fl32(r64 - mean per component) (the compatibility projection),
+// z = r/D, d = 0; the search direction is kept (or zeroed on a restart).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_mreplace(MxArgs a, int tile) {
+    __shared__ double sh[NRES * (BLOCK / 64)];
+    const TileRange tr(a.ntiles);
+    for (int t = tr.t; t < tr.end;) {
+        const TileDesc T = a.tiles[t];
+        const int s = T.sys;
+        const int run_end = min(tr.end, T.tile_end);
+        const MxSlot& P = a.slot[0][s];
+        bool done = P.done != 0;
+        bool zero_p = false;
+        double mean_r[MAX_COMP] = {};
+        if (!done) {
+            const MxConst& K = a.cst[s];
+            double tot[NRES];
+            reduce_tiles<NRES>(a.partR, T.tile_begin, T.tile_end, tot, sh);
+            const int rnd = P.n_refine;
+            const double bnorm = rnd == 0 ? sqrt(tot[2]) : P.bnorm;
+            const double b1 = rnd == 0 ? tot[3] : P.b1;
+            int status = EAFP_STOP_RUNNING;
+            double res_l2 = 0.0, res_l1 = 0.0;
+            if (bnorm < SMALL) {
+                status = EAFP_STOP_ZERO_RHS;
+            } else {
+                res_l2 = sqrt(tot[0]) / bnorm;
+                res_l1 = b1 > 0.0 ? tot[1] / b1 : tot[1];
+                if (res_l2 < K.tol) status = EAFP_STOP_CONVERGED;
+                else if (rnd >= K.max_steps) status = EAFP_STOP_MAX_ITER;
+            }
+            done = status != EAFP_STOP_RUNNING;
+            zero_p = (rnd == 0) || K.restart || P.broken;
+#pragma unroll
+            for (int q = 0; q < MAX_COMP; ++q) mean_r[q] = q < K.ncomp ? tot[4 + q] * K.inv_cnt[q] : 0.0;
+            if (t == T.tile_begin && threadIdx.x == 0) {
+                MxSlot* N = &a.slot[1][s];
+                copy_slot(N, &P);
+                N->bnorm = bnorm;
+                N->b1 = b1;
+                N->res_l2 = res_l2;
+                N->res_l1 = res_l1;
+                N->xbuf = (rnd + 1) & 1;
+#pragma unroll
+                for (int q = 0; q < MAX_COMP; ++q)
+                    N->mean_x[q] = q < K.ncomp ? tot[4 + MAX_COMP + q] * K.inv_cnt[q] : 0.0;
+                if (done) {
+                    N->done = 1;
+                    N->status = status;
+                } else {
+                    N->n_refine = rnd + 1;
+                    N->fresh = zero_p ? 1 : 0;
+                    N->broken = 0;
+                }
+            }
+        } else if (t == T.tile_begin && threadIdx.x == 0) {
+            copy_slot(&a.slot[1][s], &P);
+        }
+        if (!done) {
+            double acc[2] = {0.0, 0.0};
+            const int c_end = min(T.n, (run_end - T.tile_begin) * tile);
+            for (int c = (t - T.tile_begin) * tile + (int)threadIdx.x; c < c_end; c += BLOCK) {
+                const long o = T.off + c;
+                const int slot = a.comp[o];
+                double mu = 0.0;
+#pragma unroll
+                for (int q = 0; q < MAX_COMP; ++q) mu = slot == q ? mean_r[q] : mu;
+                const float rr = (float)(a.r64[o] - mu);
+                const float zz = a.dinv[o] * rr;
+                a.r[o] = rr;
+                a.z[o] = zz;
+                a.d[o] = 0.0f;
+                if (zero_p) a.p[0][o] = 0.0f;
+                acc[0] += (double)rr * (double)rr;
+                acc[1] += (double)rr * (double)zz;
+            }
+            block_sum<2>(acc, sh);
+            store_run<2>(a.partB, t, run_end, acc);
+        }
+        t = run_end;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Inner iteration, role 0 (reads slot 1, writes slot 0): beta from the r.z
+// partials, p = z + beta*p, q = A_lo p (flux form), p.q.
+// ---------------------------------------------------------------------------
+template <int F, int CPT>
+__global__ __launch_bounds__(BLOCK) void k_mspmv(MxArgs a, int par) {
+    using W = WFmt<F>;
+    constexpr int TILE = BLOCK * CPT;
+    __shared__ double sh[2 * (BLOCK / 64)];
+    const float* __restrict__ pold = a.p[par];
+    float* __restrict__ pnew = a.p[par ^ 1];
+    const float* __restrict__ z = a.z;
+    const TileRange tr(a.ntiles);
+    for (int t = tr.t; t < tr.end;) {
+        const TileDesc T = a.tiles[t];
+        const int s = T.sys;
+        const int run_end = min(tr.end, T.tile_end);
+        const MxSlot& P = a.slot[1][s];
+        bool skip = P.done || P.broken;
+        double beta = 0.0, rz = P.rz;
+        if (!skip) {
+            double tot[2];
+            reduce_tiles<2>(a.partB, T.tile_begin, T.tile_end, tot, sh);
+            if (!(tot[1] > 0.0)) {
+                skip = true;     // r.z lost positivity: wait for the next refinement step
+            } else {
+                beta = P.fresh ? 0.0 : tot[1] / P.rz;
+                rz = tot[1];
+            }
+        }
+        if (t == T.tile_begin && threadIdx.x == 0) {
+            MxSlot* N = &a.slot[0][s];
+            copy_slot(N, &P);
+            if (!P.done && !P.broken) {
+                N->beta = beta;
+                N->rz = rz;
+                N->fresh = 0;
+                N->broken = skip ? 1 : 0;
+            }
+        }
+        if (!skip) {
+            const float fb = (float)beta;
+            double acc[1] = {0.0};
+            const int c_end = min(T.n, (run_end - T.tile_begin) * TILE);
+            for (int c0 = (t - T.tile_begin) * TILE + (int)threadIdx.x; c0 < c_end; c0 += TILE) {
+#pragma unroll
+                for (int m = 0; m < CPT; ++m) {
+                    const int c = c0 + m * BLOCK;
+                    if (c >= c_end) break;
+                    const CellIdx id = decompose(c, T);
+                    const long o = T.off + c;
+                    EAFP_NB(id, T, o)
+                    auto pv = [&](long q) { return z[q] + fb * pold[q]; };
+                    const float pc = pv(o);
+                    float y = a.e[o] * pc;
+                    y += W::load(a.w[1], a.scale, o) * (pc - pv(oN));
+                    y += W::load(a.w[1], a.scale, oS) * (pc - pv(oS));
+                    if (hE) y += W::load(a.w[0], a.scale, o) * (pc - pv(oE));
+                    if (hW) y += W::load(a.w[0], a.scale, oW) * (pc - pv(oW));
+                    if (hT) y += W::load(a.w[2], a.scale, o) * (pc - pv(oT));
+                    if (hB) y += W::load(a.w[2], a.scale, oB) * (pc - pv(oB));
+                    pnew[o] = pc;
+                    a.q[o] = y;
+                    acc[0] += (double)pc * (double)y;
+                }
+            }
+            block_sum<1>(acc, sh);
+            store_run<1>(a.partA, t, run_end, acc);
+        }
+        t = run_end;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Inner iteration, role 1 (reads slot 0, writes slot 1): alpha from the
+// p.q partials, d += alpha p, r -= alpha q, z = r/D, partials r.r and r.z.
+// ---------------------------------------------------------------------------
+template <int CPT>
+__global__ __launch_bounds__(BLOCK) void k_mupdate(MxArgs a, int par) {
+    constexpr int TILE = BLOCK * CPT;
+    __shared__ double sh[2 * (BLOCK / 64)];
+    const float* __restrict__ pv = a.p[par ^ 1];
+    const TileRange tr(a.ntiles);
+    for (int t = tr.t; t < tr.end;) {
+        const TileDesc T = a.tiles[t];
+        const int s = T.sys;
+        const int run_end = min(tr.end, T.tile_end);
+        const MxSlot& P = a.slot[0][s];
+        bool skip = P.done || P.broken;
+        double alpha = 0.0;
+        if (!skip) {
+            double tot[1];
+            reduce_tiles<1>(a.partA, T.tile_begin, T.tile_end, tot, sh);
+            if (!(fabs(tot[0]) > SMALL)) skip = true;
+            else alpha = P.rz / tot[0];
+        }
+        if (t == T.tile_begin && threadIdx.x == 0) {
+            MxSlot* N = &a.slot[1][s];
+            copy_slot(N, &P);
+            if (!P.done && !P.broken) {
+                N->alpha = alpha;
+                N->broken = skip ? 1 : 0;
+                if (!skip) N->total_inner = P.total_inner + 1;
+            }
+        }
+        if (!skip) {
+            const float fa = (float)alpha;
+            double acc[2] = {0.0, 0.0};
+            const int c_end = min(T.n, (run_end - T.tile_begin) * TILE);
+            for (int c0 = (t - T.tile_begin) * TILE + (int)threadIdx.x; c0 < c_end; c0 += TILE) {
+#pragma unroll
+                for (int m = 0; m < CPT; ++m) {
+                    const int c = c0 + m * BLOCK;
+                    if (c >= c_end) break;
+                    const long o = T.off + c;
+                    a.d[o] = a.d[o] + fa * pv[o];
+                    const float rr = a.r[o] - fa * a.q[o];
+                    const float zz = a.dinv[o] * rr;
+                    a.r[o] = rr;
+                    a.z[o] = zz;
+                    acc[0] += (double)rr * (double)rr;
+                    acc[1] += (double)rr * (double)zz;
+                }
+            }
+            block_sum<2>(acc, sh);
+            store_run<2>(a.partB, t, run_end, acc);
+        }
+        t = run_end;
+    }
+}
+
+// Finished systems in slot 1 (one block).
+__global__ __launch_bounds__(BLOCK) void k_mcount(MxArgs a) {
+    __shared__ double sh[BLOCK / 64];
+    double v[1] = {0.0};
+    for (int s = threadIdx.x; s < a.nsys; s += BLOCK) v[0] += a.slot[1][s].done ? 1.0 : 0.0;
+    block_sum<1>(v, sh);
+    if (threadIdx.x == 0) *a.ndone = (int)v[0];
+}
+
+// ---------------------------------------------------------------------------
+// Result: x - mean(x) on every singular component (the gauge fix).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_mproject(MxArgs a, int tile) {
+    const TileRange tr(a.ntiles);
+    for (int t = tr.t; t < tr.end;) {
+        const TileDesc T = a.tiles[t];
+        const int run_end = min(tr.end, T.tile_end);
+        const MxSlot& S = a.slot[1][T.sys];
+        const double* __restrict__ x = a.X[S.xbuf];
+        double mean[MAX_COMP];
+#pragma unroll
+        for (int q = 0; q < MAX_COMP; ++q) mean[q] = S.mean_x[q];
+        const int c_end = min(T.n, (run_end - T.tile_begin) * tile);
+        for (int c = (t - T.tile_begin) * tile + (int)threadIdx.x; c < c_end; c += BLOCK) {
+            const long o = T.off + c;
+            const int slot = a.comp[o];
+            double mu = 0.0;
+#pragma unroll
+            for (int q = 0; q < MAX_COMP; ++q) mu = slot == q ? mean[q] : mu;
+            a.xout[o] = x[o] - mu;
+        }
+        t = run_end;
+    }
+}
+
+struct MixedWork {
+    int fmt = -1;
+    DevBuf<double> X0, X1, r64, xout, partR, partA, partB;
+    DevBuf<unsigned char> w[3];
+    DevBuf<float> scale, e, dinv, d, r, z, q, p0, p1;
+    DevBuf<MxSlot> slot0, slot1;
+    DevBuf<MxConst> cst;
+    DevBuf<int> ndone;
+    int* host_cnt = nullptr;        // pinned, 2 slots
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int exec_interval = 0, exec_fmt = -1, exec_gen = -1;
+    MxArgs args;
+};
+
+void free_mixed(MixedWork* w) {
+    if (!w) return;
+    if (w->exec) (void)hipGraphExecDestroy(w->exec);
+    for (auto& e : w->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (w->t0) (void)hipEventDestroy(w->t0);
+    if (w->t1) (void)hipEventDestroy(w->t1);
+    if (w->host_cnt) (void)hipHostFree(w->host_cnt);
+    delete w;
+}
+
+int download_x(Batch& B, int s, const double* x_compact);
+
+static int ensure_mixed(Batch& B, int fmt) {
+    if (!B.mixed) {
+        MixedWork* w = new MixedWork();
+        B.mixed = w;
+        const size_t n = B.ncells;
+        EAFP_TRY_RC(w->X0.alloc(n));
+        EAFP_TRY_RC(w->X1.alloc(n));
+        EAFP_TRY_RC(w->r64.alloc(n));
+        EAFP_TRY_RC(w->xout.alloc(n));
+        EAFP_TRY_RC(w->e.alloc(n));
+        EAFP_TRY_RC(w->dinv.alloc(n));
+        EAFP_TRY_RC(w->d.alloc(n));
+        EAFP_TRY_RC(w->r.alloc(n));
+        EAFP_TRY_RC(w->z.alloc(n));
+        EAFP_TRY_RC(w->q.alloc(n));
+        EAFP_TRY_RC(w->p0.alloc(n));
+        EAFP_TRY_RC(w->p1.alloc(n));
+        EAFP_TRY(hipMemset(w->d.p, 0, n * sizeof(float)));
+        EAFP_TRY(hipMemset(w->p0.p, 0, n * sizeof(float)));
+        EAFP_TRY(hipMemset(w->p1.p, 0, n * sizeof(float)));
+        EAFP_TRY_RC(w->slot0.alloc(B.nsys));
+        EAFP_TRY_RC(w->slot1.alloc(B.nsys));
+        EAFP_TRY_RC(w->cst.alloc(B.nsys));
+        EAFP_TRY_RC(w->ndone.alloc(1));
+        EAFP_TRY(hipHostMalloc((void**)&w->host_cnt, 2 * sizeof(int), hipHostMallocDefault));
+        for (auto& e : w->ev) EAFP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        EAFP_TRY(hipEventCreate(&w->t0));
+        EAFP_TRY(hipEventCreate(&w->t1));
+    }
+    MixedWork* w = B.mixed;
+    if (w->fmt != fmt) {
+        const size_t n = B.ncells;
+        for (auto& wb : w->w) EAFP_TRY_RC(wb.alloc(n * wbytes(fmt)));
+        if (fmt == EAFP_COEF_FP16) EAFP_TRY_RC(w->scale.alloc(n));
+        w->fmt = fmt;
+    }
+    EAFP_TRY_RC(w->partR.alloc((size_t)B.ntiles * NRES));
+    EAFP_TRY_RC(w->partA.alloc((size_t)B.ntiles));
+    EAFP_TRY_RC(w->partB.alloc((size_t)B.ntiles * 2));
+    MxArgs& a = w->args;
+    a.tiles = B.tiles.p;
+    a.ntiles = B.ntiles;
+    a.nsys = B.nsys;
+    a.aW = B.coef[C_AW].p;
+    a.aE = B.coef[C_AE].p;
+    a.aS = B.coef[C_AS].p;
+    a.aN = B.coef[C_AN].p;
+    a.aB = B.coef[C_AB].p;
+    a.aT = B.coef[C_AT].p;
+    a.aP = B.coef[C_AP].p;
+    a.b = B.b.p;
+    a.fold = B.fold.p;
+    a.x0 = B.x0.p;
+    a.comp = B.comp.p;
+    a.X[0] = w->X0.p;
+    a.X[1] = w->X1.p;
+    a.r64 = w->r64.p;
+    a.xout = w->xout.p;
+    for (int c = 0; c < 3; ++c) a.w[c] = w->w[c].p;
+    a.scale = fmt == EAFP_COEF_FP16 ? w->scale.p : nullptr;
+    a.e = w->e.p;
+    a.dinv = w->dinv.p;
+    a.d = w->d.p;
+    a.r = w->r.p;
+    a.z = w->z.p;
+    a.q = w->q.p;
+    a.p[0] = w->p0.p;
+    a.p[1] = w->p1.p;
+    a.partR = w->partR.p;
+    a.partA = w->partA.p;
+    a.partB = w->partB.p;
+    a.slot[0] = w->slot0.p;
+    a.slot[1] = w->slot1.p;
+    a.cst = w->cst.p;
+    a.ndone = w->ndone.p;
+    return EAFP_OK;
+}
+
+template <int F, int CPT>
+static void launch_mspmv_t(Batch& B, int par) {
+    k_mspmv<F, CPT><<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, par);
+}
+template <int CPT>
+static void launch_mupdate_t(Batch& B, int par) {
+    k_mupdate<CPT><<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, par);
+}
+
+template <int F>
+static void launch_mspmv_f(Batch& B, int par) {
+    switch (B.cpt) {
+        case 1: launch_mspmv_t<F, 1>(B, par); break;
+        case 4: launch_mspmv_t<F, 4>(B, par); break;
+        case 8: launch_mspmv_t<F, 8>(B, par); break;
+        default: launch_mspmv_t<F, 2>(B, par); break;
+    }
+}
+
+static void launch_mspmv(Batch& B, int fmt, int par) {
+    if (fmt == EAFP_COEF_BF16) launch_mspmv_f<EAFP_COEF_BF16>(B, par);
+    else if (fmt == EAFP_COEF_FP16) launch_mspmv_f<EAFP_COEF_FP16>(B, par);
+    else launch_mspmv_f<EAFP_COEF_FP32>(B, par);
+}
+
+static void launch_mupdate(Batch& B, int par) {
+    switch (B.cpt) {
+        case 1: launch_mupdate_t<1>(B, par); break;
+        case 4: launch_mupdate_t<4>(B, par); break;
+        case 8: launch_mupdate_t<8>(B, par); break;
+        default: launch_mupdate_t<2>(B, par); break;
+    }
+}
+
+static void launch_refine(Batch& B) {
+    const int tile = BLOCK * B.cpt;
+    k_res<<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, tile);
+    k_mreplace<<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, tile);
+}
+
+static int mixed_graph(Batch& B, int fmt, int interval) {
+    MixedWork* w = B.mixed;
+    if (w->exec && w->exec_interval == interval && w->exec_fmt == fmt && w->exec_gen == B.tiling_gen)
+        return EAFP_OK;
+    if (w->exec) {
+        (void)hipGraphExecDestroy(w->exec);
+        w->exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    EAFP_TRY(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+    for (int c = 0; c < interval; ++c) {
+        launch_mspmv(B, fmt, c & 1);
+        launch_mupdate(B, c & 1);
+    }
+    launch_refine(B);
+    k_mcount<<<1, BLOCK, 0, B.stream>>>(w->args);
+    hipError_t e = hipStreamEndCapture(B.stream, &graph);
+    if (e != hipSuccess) return (int)e;
+    e = hipGraphInstantiate(&w->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return (int)e;
+    w->exec_interval = interval;
+    w->exec_fmt = fmt;
+    w->exec_gen = B.tiling_gen;
+    return EAFP_OK;
+}
+
+static int mixed_start(Batch& B, const eafp_mixed_opts& o, int fmt) {
+    MixedWork* w = B.mixed;
+    std::vector<MxConst> cst(B.nsys);
+    for (int s = 0; s < B.nsys; ++s) {
+        MxConst& K = cst[s];
+        K = MxConst{};
+        K.tol = o.tol;
+        K.max_steps = o.max_steps;
+        K.restart = o.restart ? 1 : 0;
+        K.ncomp = B.comp_info[s].ncomp;
+        for (int q = 0; q < MAX_COMP; ++q) K.inv_cnt[q] = B.comp_info[s].inv_cnt[q];
+    }
+    std::vector<MxSlot> st(B.nsys, MxSlot{});
+    EAFP_TRY(hipMemcpyAsync(w->cst.p, cst.data(), sizeof(MxConst) * B.nsys, hipMemcpyHostToDevice, B.stream));
+    EAFP_TRY(hipMemcpyAsync(w->slot1.p, st.data(), sizeof(MxSlot) * B.nsys, hipMemcpyHostToDevice, B.stream));
+    const int tile = BLOCK * B.cpt;
+    if (fmt == EAFP_COEF_BF16) k_compress<EAFP_COEF_BF16><<<B.grid, BLOCK, 0, B.stream>>>(w->args, tile);
+    else if (fmt == EAFP_COEF_FP16) k_compress<EAFP_COEF_FP16><<<B.grid, BLOCK, 0, B.stream>>>(w->args, tile);
+    else k_compress<EAFP_COEF_FP32><<<B.grid, BLOCK, 0, B.stream>>>(w->args, tile);
+    launch_refine(B);    // x = x0, first residual, first inner start
+    EAFP_TRY(hipGetLastError());
+    return EAFP_OK;
+}
+
+static int check_mixed(Batch& B, const eafp_mixed_opts& o) {
+    for (int s = 0; s < B.nsys; ++s) {
+        if (!B.uploaded[s]) return EAFP_ERR_STATE;
+        if (B.comp_info[s].overflow) return EAFP_ERR_COMPONENTS;
+    }
+    if (!(o.tol > 0.0) || o.max_steps < 1 || o.interval < 0) return EAFP_ERR_ARG;
+    if (o.coef_format < EAFP_COEF_FP32 || o.coef_format > EAFP_COEF_FP16) return EAFP_ERR_ARG;
+    return EAFP_OK;
+}
+
+int solve_mixed(Batch& B, const eafp_mixed_opts& o, eafp_stats* stats) {
+    EAFP_TRY_RC(check_mixed(B, o));
+    const int fmt = o.coef_format;
+    EAFP_TRY_RC(ensure_mixed(B, fmt));
+    MixedWork* w = B.mixed;
+    int interval = o.interval > 0 ? o.interval : 32;
+    interval = (interval + 1) & ~1;    // even: the p buffer parity restarts every step
+    EAFP_TRY_RC(mixed_graph(B, fmt, interval));
+    EAFP_TRY(hipEventRecord(w->t0, B.stream));
+    EAFP_TRY_RC(mixed_start(B, o, fmt));
+
+    // One graph launch = one refinement step for every system; poll the
+    // finished count of the step before the one just launched.
+    const long max_launch = (long)o.max_steps + 2;
+    long launched = 0;
+    for (;;) {
+        if (launched < max_launch) {
+            EAFP_TRY(hipGraphLaunch(w->exec, B.stream));
+            const int slot = (int)(launched & 1);
+            EAFP_TRY(hipMemcpyAsync(&w->host_cnt[slot], w->ndone.p, sizeof(int), hipMemcpyDeviceToHost, B.stream));
+            EAFP_TRY(hipEventRecord(w->ev[slot], B.stream));
+            ++launched;
+        }
+        if (launched >= 2) {
+            const int older = (int)(launched & 1);
+            EAFP_TRY(hipEventSynchronize(w->ev[older]));
+            if (w->host_cnt[older] >= B.nsys) break;
+        }
+        if (launched >= max_launch) {
+            EAFP_TRY(hipEventSynchronize(w->ev[(launched - 1) & 1]));
+            break;
+        }
+    }
+    k_mproject<<<B.grid, BLOCK, 0, B.stream>>>(w->args, BLOCK * B.cpt);
+    EAFP_TRY(hipGetLastError());
+    for (int s = 0; s < B.nsys; ++s) EAFP_TRY_RC(download_x(B, s, w->xout.p));
+    EAFP_TRY(hipEventRecord(w->t1, B.stream));
+    std::vector<MxSlot> st(B.nsys);
+    EAFP_TRY(hipMemcpyAsync(st.data(), w->slot1.p, sizeof(MxSlot) * B.nsys, hipMemcpyDeviceToHost, B.stream));
+    EAFP_TRY(hipStreamSynchronize(B.stream));
+    float ms = 0.f;
+    EAFP_TRY(hipEventElapsedTime(&ms, w->t0, w->t1));
+    if (stats) {
+        for (int s = 0; s < B.nsys; ++s) {
+            eafp_stats& out = stats[s];
+            out = eafp_stats{};
+            out.residual = st[s].res_l2;
+            out.n_iter = st[s].total_inner;
+            out.n_refine = st[s].n_refine;
+            out.status = st[s].done ? st[s].status : EAFP_STOP_RUNNING;
+            out.true_rel_l2 = st[s].res_l2;
+            out.true_rel_l1 = st[s].res_l1;
+            out.seconds = ms * 1e-3;
+        }
+    }
+    return EAFP_OK;
+}
+
+// Average durations of the two inner-iteration kernels over `reps`
+// iterations (systems kept running: the tolerance is out of reach), each
+// launch bracketed by HIP events on the library stream.
+int time_mixed_kernels(Batch& B, int fmt, int reps, float* ms_spmv, float* ms_update) {
+    eafp_mixed_opts o{};
+    o.tol = 1e-300;
+    o.interval = 2;
+    o.max_steps = 1 << 20;
+    o.coef_format = fmt;
+    EAFP_TRY_RC(check_mixed(B, o));
+    if (reps < 1) return EAFP_ERR_ARG;
+    EAFP_TRY_RC(ensure_mixed(B, fmt));
+    EAFP_TRY_RC(mixed_start(B, o, fmt));
+    hipEvent_t e[3];
+    for (auto& x : e) EAFP_TRY(hipEventCreate(&x));
+    float tot_s = 0.f, tot_u = 0.f;
+    for (int r = 0; r < reps; ++r) {
+        const int par = r & 1;
+        EAFP_TRY(hipEventRecord(e[0], B.stream));
+        launch_mspmv(B, fmt, par);
+        EAFP_TRY(hipEventRecord(e[1], B.stream));
+        launch_mupdate(B, par);
+        EAFP_TRY(hipEventRecord(e[2], B.stream));
+        EAFP_TRY(hipEventSynchronize(e[2]));
+        float ta = 0.f, tb = 0.f;
+        EAFP_TRY(hipEventElapsedTime(&ta, e[0], e[1]));
+        EAFP_TRY(hipEventElapsedTime(&tb, e[1], e[2]));
+        tot_s += ta;
+        tot_u += tb;
+    }
+    for (auto& x : e) (void)hipEventDestroy(x);
+    if (ms_spmv) *ms_spmv = tot_s / (float)reps;
+    if (ms_update) *ms_update = tot_u / (float)reps;
+    return EAFP_OK;
+}
+
 }  // namespace eafp

@@ -27,7 +27,8 @@ COEF_FP32, COEF_BF16, COEF_FP16 = 0, 1, 2
 EXPORTS = (
     "eafp_cg_solve", "eafp_release", "eafp_batch_create", "eafp_batch_upload",
     "eafp_batch_solve_fp64", "eafp_batch_solve_mixed", "eafp_batch_download",
-    "eafp_batch_info", "eafp_batch_destroy", "eafp_version",
+    "eafp_batch_info", "eafp_batch_set_tiling", "eafp_batch_time_kernels",
+    "eafp_batch_autotune", "eafp_batch_destroy", "eafp_version",
 )
 
 
@@ -60,8 +61,8 @@ class Stats(ctypes.Structure):
 
 
 class MixedOpts(ctypes.Structure):
-    _fields_ = [("tol", ctypes.c_double), ("inner_tol", ctypes.c_double), ("max_inner", ctypes.c_int),
-                ("max_refine", ctypes.c_int), ("coef_format", ctypes.c_int), ("chunk", ctypes.c_int)]
+    _fields_ = [("tol", ctypes.c_double), ("interval", ctypes.c_int), ("max_steps", ctypes.c_int),
+                ("coef_format", ctypes.c_int), ("restart", ctypes.c_int)]
 
 
 _lib = None
@@ -93,6 +94,14 @@ def load():
     L.eafp_batch_download.restype = i
     L.eafp_batch_info.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(ctypes.c_long)]
     L.eafp_batch_info.restype = i
+    fp = ctypes.POINTER(ctypes.c_float)
+    ip = ctypes.POINTER(i)
+    L.eafp_batch_set_tiling.argtypes = [vp, i, i]
+    L.eafp_batch_set_tiling.restype = i
+    L.eafp_batch_time_kernels.argtypes = [vp, i, i, i, fp, fp]
+    L.eafp_batch_time_kernels.restype = i
+    L.eafp_batch_autotune.argtypes = [vp, i, i, i, ip, ip, fp]
+    L.eafp_batch_autotune.restype = i
     L.eafp_batch_destroy.argtypes = [vp]
     L.eafp_batch_destroy.restype = i
     L.eafp_version.argtypes = []

@@ -114,12 +114,33 @@ class Batch:
         check(self._L.eafp_batch_solve_fp64(self._h, int(max_iter), float(tol), st), "eafp_batch_solve_fp64")
         return [s.as_dict() for s in st]
 
-    def solve_mixed(self, tol=1e-8, inner_tol=1e-2, max_inner=20000, max_refine=30,
-                    coef_format=_lib.COEF_FP32, chunk=0):
-        o = MixedOpts(float(tol), float(inner_tol), int(max_inner), int(max_refine), int(coef_format), int(chunk))
+    def solve_mixed(self, tol=1e-8, interval=32, max_steps=2000, coef_format=_lib.COEF_FP32, restart=False):
+        """FP32 inner CG + FP64 refinement every `interval` inner iterations."""
+        o = MixedOpts(float(tol), int(interval), int(max_steps), int(coef_format), int(bool(restart)))
         st = (Stats * len(self))()
         check(self._L.eafp_batch_solve_mixed(self._h, ctypes.byref(o), st), "eafp_batch_solve_mixed")
         return [s.as_dict() for s in st]
+
+    SOLVER_FP64, SOLVER_MIXED = 0, 1
+
+    def set_tiling(self, cells_per_thread, blocks=0):
+        check(self._L.eafp_batch_set_tiling(self._h, int(cells_per_thread), int(blocks)), "eafp_batch_set_tiling")
+
+    def time_kernels(self, solver, coef_format=_lib.COEF_FP32, reps=20):
+        """Average ms of one launch of each iteration kernel (spmv, update)."""
+        a = ctypes.c_float()
+        b = ctypes.c_float()
+        check(self._L.eafp_batch_time_kernels(self._h, int(solver), int(coef_format), int(reps),
+                                              ctypes.byref(a), ctypes.byref(b)), "eafp_batch_time_kernels")
+        return a.value, b.value
+
+    def autotune(self, solver, coef_format=_lib.COEF_FP32, reps=10):
+        cpt = ctypes.c_int()
+        blocks = ctypes.c_int()
+        ms = ctypes.c_float()
+        check(self._L.eafp_batch_autotune(self._h, int(solver), int(coef_format), int(reps), ctypes.byref(cpt),
+                                          ctypes.byref(blocks), ctypes.byref(ms)), "eafp_batch_autotune")
+        return {"cells_per_thread": cpt.value, "blocks": blocks.value, "ms_per_iter": ms.value}
 
     def download(self, s, phi=None):
         nr, nth, nz = self.shapes[s]

@@ -50,7 +50,7 @@ typedef struct eafp_stats {
     double residual;      /* FP64 CG: relative recurrence residual, as cg_3d_mpi's
                              `residual`; mixed: true relative residual ||b-Ax||/||b|| */
     int n_iter;           /* FP64 CG: iterations, as cg_3d_mpi's `n_iter`;
-                             mixed: total inner iterations */
+                             mixed: total inner FP32 iterations */
     int n_refine;         /* mixed: FP64 refinement steps (0 for FP64 CG) */
     int status;           /* EAFP_STOP_* */
     int pad_;
@@ -71,12 +71,14 @@ typedef struct eafp_stats {
 #define EAFP_COEF_FP16 2         /* FP16 with a per-cell FP32 scale */
 
 typedef struct eafp_mixed_opts {
-    double tol;           /* outer target: true relative residual (e.g. 1e-8) */
-    double inner_tol;     /* inner FP32 CG relative residual per refinement step */
-    int max_inner;        /* inner iteration cap per refinement step */
-    int max_refine;       /* refinement step cap */
+    double tol;           /* target: true relative residual ||b-Ax||/||b|| (e.g. 1e-8) */
+    int interval;         /* inner FP32 iterations between FP64 refinement steps
+                             (rounded up to even; 0: library default) */
+    int max_steps;        /* cap on FP64 refinement steps */
     int coef_format;      /* EAFP_COEF_* */
-    int chunk;            /* iterations per captured graph (0: library default) */
+    int restart;          /* 0: the inner CG keeps its search direction across
+                             refinement steps (residual replacement); 1: it restarts
+                             at every step (classical iterative refinement) */
 } eafp_mixed_opts;
 
 /* ---------------------------------------------------------------------------
@@ -123,10 +125,13 @@ int eafp_batch_upload(eafp_batch* b, int s,
 int eafp_batch_solve_fp64(eafp_batch* b, int max_iter, double tol, eafp_stats* stats);
 
 /* Mixed precision: FP32 inner Jacobi-PCG on low-precision symmetric face
- * coefficients, wrapped in FP64 iterative refinement (FP64 true residual,
- * FP64 dot accumulation, FP64 zero-mean projection of every singular
- * Neumann component).  The result is gauge-fixed: zero mean per singular
- * component.  stats: nsys entries (may be NULL). */
+ * coefficients, wrapped in FP64 iterative refinement: every `interval`
+ * inner iterations the FP64 correction is folded into x, the true residual
+ * b - A x is recomputed with the caller's FP64 operator and replaces the
+ * inner residual, projected to zero mean on every singular Neumann
+ * component (FP64); every dot product is accumulated in FP64.  The result
+ * is gauge-fixed: zero mean per singular component.  stats: nsys entries
+ * (may be NULL). */
 int eafp_batch_solve_mixed(eafp_batch* b, const eafp_mixed_opts* opts, eafp_stats* stats);
 
 /* Copies system s's phi (padded Fortran layout: interior from the last
@@ -136,6 +141,26 @@ int eafp_batch_download(eafp_batch* b, int s, double* phi, int to_device);
 
 /* Launch shape in use: blocks per launch and cells per tile (for reports). */
 int eafp_batch_info(const eafp_batch* b, int* grid, int* tile_cells, long* ncells);
+
+/* Launch shape of every kernel of the batch: a tile of 256*cells_per_thread
+ * cells (cells_per_thread in {1, 2, 4, 8}) and `blocks` blocks per launch
+ * (0: one per tile, at most 8 per CU).  No reference counterpart. */
+int eafp_batch_set_tiling(eafp_batch* b, int cells_per_thread, int blocks);
+
+/* Measurement: average duration (ms) of one launch of each of the two
+ * iteration kernels of a solver (solver 0: FP64 CG, k64_spmv/k64_update;
+ * solver 1: mixed, k_mspmv/k_mupdate with coef_format) over `reps`
+ * iterations on the uploaded systems, timed with HIP events on the
+ * library's stream.  Leaves the batch's results undefined (solve again).
+ * No reference counterpart. */
+int eafp_batch_time_kernels(eafp_batch* b, int solver, int coef_format, int reps, float* ms_spmv,
+                            float* ms_update);
+
+/* Autotuner: times every tile shape (cells per thread x blocks per CU) for
+ * the solver's iteration kernels and keeps the fastest; returns the shape
+ * and its iteration time (ms).  No reference counterpart. */
+int eafp_batch_autotune(eafp_batch* b, int solver, int coef_format, int reps, int* cells_per_thread,
+                        int* blocks, float* ms_iter);
 
 int eafp_batch_destroy(eafp_batch* b);
 

@@ -30,10 +30,10 @@ def test_header_symbols_exported():
 
 
 def test_struct_layouts_match_header():
-    # eafp_desc: 7 ints; eafp_stats: double, 4 ints, 3 doubles; eafp_mixed_opts: 2 doubles, 4 ints
+    # eafp_desc: 7 ints; eafp_stats: double, 4 ints, 3 doubles; eafp_mixed_opts: double, 4 ints
     assert ctypes.sizeof(_lib.Desc) == 28
     assert ctypes.sizeof(_lib.Stats) == 48
-    assert ctypes.sizeof(_lib.MixedOpts) == 32
+    assert ctypes.sizeof(_lib.MixedOpts) == 24
 
 
 def _arrays(nr=4, nth=6, nz=5):

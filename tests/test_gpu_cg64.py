@@ -159,3 +159,31 @@ def test_batch_fp64_ragged():
         st2 = b.solve_fp64(max_iter=100000, tol=1e-8)
         assert [x["n_iter"] for x in st2] == [x["n_iter"] for x in st]
         assert [x["residual"] for x in st2] == [x["residual"] for x in st]
+
+
+@pytest.mark.parametrize("cpt", [1, 2, 4, 8])
+def test_tilings_agree(cpt):
+    """Every launch shape the autotuner may pick gives the same answer
+    (only the partial-sum tree changes)."""
+    syss = [systems.make_system(3, s) for s in range(3)]
+    with eafp.Batch.from_systems(syss) as b:
+        b.set_tiling(cpt, 0)
+        st = b.solve_fp64(100000, 1e-8)
+        for s, ps in enumerate(syss):
+            phi = b.download(s)
+            phi_ref, _, nit_ref, _ = oracle.cg(ps)
+            assert st[s]["status"] == "converged"
+            assert abs(st[s]["n_iter"] - nit_ref) <= max(2, nit_ref // 50)
+            assert oracle.gauge_error(oracle.interior(phi), oracle.interior(phi_ref), ps.labels, ps.ncomp) < 1e-6
+
+
+def test_autotune_keeps_results():
+    ps = systems.make_system(2, 0, (20, 36, 28))
+    with eafp.Batch.from_systems([ps]) as b:
+        st0 = b.solve_fp64(100000, 1e-8)[0]
+        best = b.autotune(eafp.Batch.SOLVER_FP64, reps=3)
+        assert best["cells_per_thread"] in (1, 2, 4, 8) and best["ms_per_iter"] > 0
+        info = b.info()
+        assert info["tile_cells"] == 256 * best["cells_per_thread"] and info["grid"] == best["blocks"]
+        st1 = b.solve_fp64(100000, 1e-8)[0]
+        assert st1["status"] == "converged" and abs(st1["n_iter"] - st0["n_iter"]) <= 2
