@@ -88,18 +88,6 @@ void free_cg64(Cg64Work* w) {
 
 int download_x(Batch& B, int s, const double* x_compact);
 
-// Compact offsets of the six stencil neighbours; the theta pair wraps, the
-// r/z pairs are flagged off at the walls (the zeroed halo of the
-// reference's work vectors contributes nothing there).
-#define EAFP_NEIGHBOURS(id, g, o)                                           \
-    const long sr_ = (long)(g).nth * (g).nz;                                \
-    const long oW = (o) - sr_, oE = (o) + sr_;                              \
-    const long oS = (id).j > 0 ? (o) - (g).nz : (o) + (long)((g).nth - 1) * (g).nz; \
-    const long oN = (id).j < (g).nth - 1 ? (o) + (g).nz : (o) - (long)((g).nth - 1) * (g).nz; \
-    const long oB = (o) - 1, oT = (o) + 1;                                  \
-    const bool hW = (id).i > 0, hE = (id).i < (g).nr - 1;                   \
-    const bool hB = (id).k > 0, hT = (id).k < (g).nz - 1;
-
 // ---------------------------------------------------------------------------
 // r = Su - A*phi0, z = r/max(|aP|,SMALL), pv_old = 0, partials r.r, r.z,
 // Su.Su (src/mod_solver_3d.f90:509-530).  Writes the fresh slot 1.
@@ -122,17 +110,18 @@ __global__ __launch_bounds__(BLOCK) void k64_init(Cg64Args a) {
         for (int c = (t - T.tile_begin) * TILE + (int)threadIdx.x; c < c_end; c += BLOCK) {
             const CellIdx id = decompose(c, T);
             const long o = T.off + c;
-            EAFP_NEIGHBOURS(id, T, o)
+            const Nb nb = neighbours(id, T, o);
             const double* x = a.x0;
-            const double xW = hW ? x[oW] : 0.0, xE = hE ? x[oE] : 0.0;
-            const double xB = hB ? x[oB] : 0.0, xT = hT ? x[oT] : 0.0;
+            const double xc = x[o], xW = x[nb.W], xE = x[nb.E], xS = x[nb.S], xN = x[nb.N], xB = x[nb.B],
+                         xT = x[nb.T];
             const double ap = a.aP[o];
-            const double ax = ap * x[o] - a.aW[o] * xW - a.aE[o] * xE - a.aS[o] * x[oS] -
-                              a.aN[o] * x[oN] - a.aB[o] * xB - a.aT[o] * xT;
+            const double ax = ap * xc - a.aW[o] * (nb.hW ? xW : 0.0) - a.aE[o] * (nb.hE ? xE : 0.0) -
+                              a.aS[o] * xS - a.aN[o] * xN - a.aB[o] * (nb.hB ? xB : 0.0) -
+                              a.aT[o] * (nb.hT ? xT : 0.0);
             const double su = a.b[o];
             const double rr = su - (ax - a.fold[o]);
             const double zz = rr / fmax(fabs(ap), SMALL);
-            a.x[o] = x[o];
+            a.x[o] = xc;
             a.r[o] = rr;
             a.z[o] = zz;
             a.p[0][o] = 0.0;
@@ -204,26 +193,60 @@ __global__ __launch_bounds__(BLOCK) void k64_spmv(Cg64Args a, int par) {
         if (t == T.tile_begin && threadIdx.x == 0) a.slot[0][s] = N;
         if (!N.done) {
             const double beta = N.beta;
+            const double* __restrict__ cP = a.aP;
+            const double* __restrict__ cW = a.aW;
+            const double* __restrict__ cE = a.aE;
+            const double* __restrict__ cS = a.aS;
+            const double* __restrict__ cN = a.aN;
+            const double* __restrict__ cB = a.aB;
+            const double* __restrict__ cT = a.aT;
+            double* __restrict__ qo = a.q;
             double acc[1] = {0.0};
             const int c_end = min(T.n, (run_end - T.tile_begin) * TILE);
             for (int c0 = (t - T.tile_begin) * TILE + (int)threadIdx.x; c0 < c_end; c0 += TILE) {
+                // load phase: every operand of the CPT cells of this thread
+                double zv[CPT][7], pv[CPT][7], cf[CPT][7];
+                long oc[CPT];
+                bool ok[CPT], mW[CPT], mE[CPT], mB[CPT], mT[CPT];
 #pragma unroll
                 for (int m = 0; m < CPT; ++m) {
                     const int c = c0 + m * BLOCK;
-                    if (c >= c_end) break;
-                    const CellIdx id = decompose(c, T);
-                    const long o = T.off + c;
-                    EAFP_NEIGHBOURS(id, T, o)
-                    auto pv = [&](long q) { return z[q] + beta * pold[q]; };
-                    const double pc = pv(o);
-                    const double pW = hW ? pv(oW) : 0.0, pE = hE ? pv(oE) : 0.0;
-                    const double pB = hB ? pv(oB) : 0.0, pT = hT ? pv(oT) : 0.0;
-                    const double pS = pv(oS), pN = pv(oN);
-                    const double y = a.aP[o] * pc - a.aW[o] * pW - a.aE[o] * pE - a.aS[o] * pS -
-                                     a.aN[o] * pN - a.aB[o] * pB - a.aT[o] * pT;
-                    pnew[o] = pc;
-                    a.q[o] = y;
-                    acc[0] += pc * y;
+                    ok[m] = c < c_end;
+                    const int cc = ok[m] ? c : c0;
+                    const CellIdx id = decompose(cc, T);
+                    const long o = T.off + cc;
+                    const Nb nb = neighbours(id, T, o);
+                    oc[m] = o;
+                    mW[m] = nb.hW; mE[m] = nb.hE; mB[m] = nb.hB; mT[m] = nb.hT;
+                    const long idx[7] = {o, nb.W, nb.E, nb.S, nb.N, nb.B, nb.T};
+#pragma unroll
+                    for (int q = 0; q < 7; ++q) {
+                        zv[m][q] = z[idx[q]];
+                        pv[m][q] = pold[idx[q]];
+                    }
+                    cf[m][0] = cP[o];
+                    cf[m][1] = cW[o];
+                    cf[m][2] = cE[o];
+                    cf[m][3] = cS[o];
+                    cf[m][4] = cN[o];
+                    cf[m][5] = cB[o];
+                    cf[m][6] = cT[o];
+                }
+                // compute phase
+#pragma unroll
+                for (int m = 0; m < CPT; ++m) {
+                    double p[7];
+#pragma unroll
+                    for (int q = 0; q < 7; ++q) p[q] = zv[m][q] + beta * pv[m][q];
+                    const double pW = mW[m] ? p[1] : 0.0, pE = mE[m] ? p[2] : 0.0;
+                    const double pB = mB[m] ? p[5] : 0.0, pT = mT[m] ? p[6] : 0.0;
+                    const double y = cf[m][0] * p[0] - cf[m][1] * pW - cf[m][2] * pE - cf[m][3] * p[3] -
+                                     cf[m][4] * p[4] - cf[m][5] * pB - cf[m][6] * pT;
+                    if (ok[m]) {
+                        pnew[oc[m]] = p[0];
+                        qo[oc[m]] = y;
+                        acc[0] += p[0] * y;
+                    }
                 }
             }
             block_sum<1>(acc, sh);
@@ -264,21 +287,40 @@ __global__ __launch_bounds__(BLOCK) void k64_update(Cg64Args a, int par) {
         if (t == T.tile_begin && threadIdx.x == 0) a.slot[1][s] = N;
         if (!N.done) {
             const double alpha = N.alpha;
+            double* __restrict__ x = a.x;
+            double* __restrict__ r = a.r;
+            double* __restrict__ zo = a.z;
+            const double* __restrict__ q = a.q;
+            const double* __restrict__ aP = a.aP;
             double acc[3] = {0.0, 0.0, 0.0};
             const int c_end = min(T.n, (run_end - T.tile_begin) * TILE);
             for (int c0 = (t - T.tile_begin) * TILE + (int)threadIdx.x; c0 < c_end; c0 += TILE) {
+                double xv[CPT], pvv[CPT], rv[CPT], qv[CPT], apv[CPT];
+                long oc[CPT];
+                bool ok[CPT];
 #pragma unroll
                 for (int m = 0; m < CPT; ++m) {
                     const int c = c0 + m * BLOCK;
-                    if (c >= c_end) break;
-                    const long o = T.off + c;
-                    a.x[o] = a.x[o] + alpha * pv[o];
-                    const double rr = a.r[o] - alpha * a.q[o];
-                    const double zz = rr / fmax(fabs(a.aP[o]), SMALL);
-                    a.r[o] = rr;
-                    a.z[o] = zz;
-                    acc[0] += rr * rr;
-                    acc[1] += rr * zz;
+                    ok[m] = c < c_end;
+                    const long o = T.off + (ok[m] ? c : c0);
+                    oc[m] = o;
+                    xv[m] = x[o];
+                    pvv[m] = pv[o];
+                    rv[m] = r[o];
+                    qv[m] = q[o];
+                    apv[m] = aP[o];
+                }
+#pragma unroll
+                for (int m = 0; m < CPT; ++m) {
+                    const double rr = rv[m] - alpha * qv[m];
+                    const double zz = rr / fmax(fabs(apv[m]), SMALL);
+                    if (ok[m]) {
+                        x[oc[m]] = xv[m] + alpha * pvv[m];
+                        r[oc[m]] = rr;
+                        zo[oc[m]] = zz;
+                        acc[0] += rr * rr;
+                        acc[1] += rr * zz;
+                    }
                 }
             }
             block_sum<3>(acc, sh);

@@ -64,6 +64,33 @@ struct CellIdx {
     int i, j, k;
 };
 
+// Offsets of the six stencil neighbours of compact cell o = off + c.  The
+// theta pair wraps; an r/z neighbour beyond a wall is replaced by o itself
+// (always a valid address) and flagged off, so every load can be issued
+// unconditionally and the wall applied afterwards by a select: the loads of
+// a cell never wait on a branch.
+struct Nb {
+    long W, E, S, N, B, T;
+    bool hW, hE, hB, hT;
+};
+
+template <typename G>
+__device__ __forceinline__ Nb neighbours(const CellIdx& id, const G& g, long o) {
+    Nb n;
+    const long sr = (long)g.nth * g.nz;
+    n.hW = id.i > 0;
+    n.hE = id.i < g.nr - 1;
+    n.hB = id.k > 0;
+    n.hT = id.k < g.nz - 1;
+    n.W = n.hW ? o - sr : o;
+    n.E = n.hE ? o + sr : o;
+    n.S = id.j > 0 ? o - g.nz : o + (long)(g.nth - 1) * g.nz;
+    n.N = id.j < g.nth - 1 ? o + g.nz : o - (long)(g.nth - 1) * g.nz;
+    n.B = n.hB ? o - 1 : o;
+    n.T = n.hT ? o + 1 : o;
+    return n;
+}
+
 template <typename G>
 __device__ __forceinline__ CellIdx decompose(int local, const G& g) {
     int col = fdiv(local, g.nz, g.inv_nz);     // i*nth + j

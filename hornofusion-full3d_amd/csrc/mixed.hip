@@ -41,15 +41,24 @@
 
 namespace eafp {
 
-struct MxSlot {
-    double bnorm, b1, res_l2, res_l1;
-    double mean_x[MAX_COMP];
+// Per-system state of the mixed solver, double-buffered by kernel role:
+// MxDyn changes every inner iteration (k_mspmv reads slot 1 and writes
+// slot 0, k_mupdate the reverse); MxStat changes once per refinement step
+// (k_res copies slot 1 to slot 0, k_mreplace writes slot 1 from slot 0).
+// Every thread loads the few MxDyn fields it needs and the writer stores
+// its new values straight from registers.
+struct MxDyn {
     double rz, alpha, beta;
-    int n_refine, done, status, xbuf;
     int fresh;          // next k_mspmv starts a new Krylov sequence (beta = 0)
     int broken;         // inner breakdown: idle until the next refinement step
-    int total_inner, pad;
-    double spare;       // size to a multiple of 16 bytes
+    int total_inner;
+    int done;           // mirror of MxStat::done for the inner kernels
+};
+
+struct MxStat {
+    double bnorm, b1, res_l2, res_l1;
+    double mean_x[MAX_COMP];
+    int n_refine, done, status, xbuf;
 };
 
 struct MxConst {
@@ -81,7 +90,8 @@ struct MxArgs {
     double* partR;       // ntiles * NRES (k_res)
     double* partA;       // ntiles        (k_mspmv: p.q)
     double* partB;       // ntiles * 2    (k_mreplace, k_mupdate: r.r, r.z)
-    MxSlot* slot[2];
+    MxDyn* dyn[2];
+    MxStat* stat[2];
     const MxConst* cst;
     int* ndone;
 };
@@ -128,25 +138,6 @@ struct WFmt<EAFP_COEF_FP16> {
 
 size_t wbytes(int fmt) { return fmt == EAFP_COEF_FP32 ? 4 : 2; }
 
-#define EAFP_NB(id, g, o)                                                   \
-    const long sr_ = (long)(g).nth * (g).nz;                                \
-    const long oW = (o) - sr_, oE = (o) + sr_;                              \
-    const long oS = (id).j > 0 ? (o) - (g).nz : (o) + (long)((g).nth - 1) * (g).nz; \
-    const long oN = (id).j < (g).nth - 1 ? (o) + (g).nz : (o) - (long)((g).nth - 1) * (g).nz; \
-    const long oB = (o) - 1, oT = (o) + 1;                                  \
-    const bool hW = (id).i > 0, hE = (id).i < (g).nr - 1;                   \
-    const bool hB = (id).k > 0, hT = (id).k < (g).nz - 1;
-
-// Copy of a slot by one thread in 16-byte words (the struct never needs a
-// private-memory copy).
-__device__ __forceinline__ void copy_slot(MxSlot* dst, const MxSlot* src) {
-    const uint4* s = reinterpret_cast<const uint4*>(src);
-    uint4* d = reinterpret_cast<uint4*>(dst);
-#pragma unroll
-    for (int i = 0; i < (int)(sizeof(MxSlot) / sizeof(uint4)); ++i) d[i] = s[i];
-}
-static_assert(sizeof(MxSlot) % sizeof(uint4) == 0, "slot copy in 16-byte words");
-
 // ---------------------------------------------------------------------------
 // Setup: symmetric face weights, excess, Jacobi inverse.
 // ---------------------------------------------------------------------------
@@ -161,7 +152,9 @@ __global__ __launch_bounds__(BLOCK) void k_compress(MxArgs a, int tile) {
         for (int c = (t - T.tile_begin) * tile + (int)threadIdx.x; c < c_end; c += BLOCK) {
             const CellIdx id = decompose(c, T);
             const long o = T.off + c;
-            EAFP_NB(id, T, o)
+            const Nb nb = neighbours(id, T, o);
+            const bool hW = nb.hW, hE = nb.hE, hB = nb.hB, hT = nb.hT;
+            const long oW = nb.W, oE = nb.E, oS = nb.S, oN = nb.N, oB = nb.B, oT = nb.T;
             const double ap = a.aP[o];
             const double wr = hE ? 0.5 * (a.aE[o] + a.aW[oE]) : 0.0;
             const double wrm = hW ? 0.5 * (a.aE[oW] + a.aW[o]) : 0.0;
@@ -180,9 +173,9 @@ __global__ __launch_bounds__(BLOCK) void k_compress(MxArgs a, int tile) {
                                  ez = W::enc((float)wz, s_o);
             // the rounded weights exactly as the inner operator sees them
             float D = W::dec(er, s_o) + W::dec(et, s_o) + W::dec(ez, s_o);
-            if (hW) D += W::dec(W::enc((float)wrm, sc(oW)), sc(oW));
+            D += hW ? W::dec(W::enc((float)wrm, sc(oW)), sc(oW)) : 0.f;
             D += W::dec(W::enc((float)wtm, sc(oS)), sc(oS));
-            if (hB) D += W::dec(W::enc((float)wzm, sc(oB)), sc(oB));
+            D += hB ? W::dec(W::enc((float)wzm, sc(oB)), sc(oB)) : 0.f;
             const float e32 = (float)e;
             D += e32;
             ((typename W::T*)a.w[0])[o] = er;
@@ -209,10 +202,14 @@ __global__ __launch_bounds__(BLOCK) void k_res(MxArgs a, int tile) {
         const TileDesc T = a.tiles[t];
         const int s = T.sys;
         const int run_end = min(tr.end, T.tile_end);
-        const MxSlot& P = a.slot[1][s];
-        const int done = P.done;
-        const int rnd = P.n_refine;
-        if (t == T.tile_begin && threadIdx.x == 0) copy_slot(&a.slot[0][s], &P);
+        const int done = a.stat[1][s].done;
+        const int rnd = a.stat[1][s].n_refine;
+        if (t == T.tile_begin && threadIdx.x == 0) {
+            const MxStat st = a.stat[1][s];
+            const MxDyn dy = a.dyn[1][s];
+            a.stat[0][s] = st;
+            a.dyn[0][s] = dy;
+        }
         if (!done) {
             const bool first = (rnd == 0);
             const double* __restrict__ xo = first ? a.x0 : a.X[rnd & 1];
@@ -220,26 +217,39 @@ __global__ __launch_bounds__(BLOCK) void k_res(MxArgs a, int tile) {
             double acc[NRES];
 #pragma unroll
             for (int c = 0; c < NRES; ++c) acc[c] = 0.0;
+            const double* __restrict__ aW = a.aW;
+            const double* __restrict__ aE = a.aE;
+            const double* __restrict__ aS = a.aS;
+            const double* __restrict__ aN = a.aN;
+            const double* __restrict__ aB = a.aB;
+            const double* __restrict__ aT = a.aT;
+            const double* __restrict__ aP = a.aP;
+            const double* __restrict__ bb = a.b;
+            const double* __restrict__ fold = a.fold;
+            const unsigned char* __restrict__ comp = a.comp;
+            double* __restrict__ r64 = a.r64;
             const int c_end = min(T.n, (run_end - T.tile_begin) * tile);
             for (int c = (t - T.tile_begin) * tile + (int)threadIdx.x; c < c_end; c += BLOCK) {
                 const CellIdx id = decompose(c, T);
                 const long o = T.off + c;
-                EAFP_NB(id, T, o)
-                auto xv = [&](long q) { return first ? xo[q] : xo[q] + (double)d[q]; };
-                const double xc = xv(o);
-                const double xW = hW ? xv(oW) : 0.0, xE = hE ? xv(oE) : 0.0;
-                const double xB = hB ? xv(oB) : 0.0, xT = hT ? xv(oT) : 0.0;
-                const double ax = a.aP[o] * xc - a.aW[o] * xW - a.aE[o] * xE - a.aS[o] * xv(oS) -
-                                  a.aN[o] * xv(oN) - a.aB[o] * xB - a.aT[o] * xT;
-                const double su = a.b[o];
-                const double res = su - (ax - a.fold[o]);
+                const Nb nb = neighbours(id, T, o);
+                const long idx[7] = {o, nb.W, nb.E, nb.S, nb.N, nb.B, nb.T};
+                double xv[7];
+#pragma unroll
+                for (int q = 0; q < 7; ++q) xv[q] = first ? xo[idx[q]] : xo[idx[q]] + (double)d[idx[q]];
+                const double cP = aP[o], cW = aW[o], cE = aE[o], cS = aS[o], cN = aN[o], cB = aB[o], cT = aT[o];
+                const double su = bb[o], fo = fold[o];
+                const int slot = comp[o];
+                const double xc = xv[0];
+                const double ax = cP * xc - cW * (nb.hW ? xv[1] : 0.0) - cE * (nb.hE ? xv[2] : 0.0) -
+                                  cS * xv[3] - cN * xv[4] - cB * (nb.hB ? xv[5] : 0.0) - cT * (nb.hT ? xv[6] : 0.0);
+                const double res = su - (ax - fo);
                 xn[o] = xc;
-                a.r64[o] = res;
+                r64[o] = res;
                 acc[0] += res * res;
                 acc[1] += fabs(res);
                 acc[2] += su * su;
                 acc[3] += fabs(su);
-                const int slot = a.comp[o];
 #pragma unroll
                 for (int q = 0; q < MAX_COMP; ++q) {
                     acc[4 + q] += slot == q ? res : 0.0;
@@ -266,7 +276,7 @@ __global__ __launch_bounds__(BLOCK) void k_mreplace(MxArgs a, int tile) {
         const TileDesc T = a.tiles[t];
         const int s = T.sys;
         const int run_end = min(tr.end, T.tile_end);
-        const MxSlot& P = a.slot[0][s];
+        const MxStat P = a.stat[0][s];
         bool done = P.done != 0;
         bool zero_p = false;
         double mean_r[MAX_COMP] = {};
@@ -275,60 +285,72 @@ __global__ __launch_bounds__(BLOCK) void k_mreplace(MxArgs a, int tile) {
             double tot[NRES];
             reduce_tiles<NRES>(a.partR, T.tile_begin, T.tile_end, tot, sh);
             const int rnd = P.n_refine;
-            const double bnorm = rnd == 0 ? sqrt(tot[2]) : P.bnorm;
-            const double b1 = rnd == 0 ? tot[3] : P.b1;
+            MxStat N = P;
+            N.bnorm = rnd == 0 ? sqrt(tot[2]) : P.bnorm;
+            N.b1 = rnd == 0 ? tot[3] : P.b1;
             int status = EAFP_STOP_RUNNING;
-            double res_l2 = 0.0, res_l1 = 0.0;
-            if (bnorm < SMALL) {
+            if (N.bnorm < SMALL) {
+                N.res_l2 = 0.0;
+                N.res_l1 = 0.0;
                 status = EAFP_STOP_ZERO_RHS;
             } else {
-                res_l2 = sqrt(tot[0]) / bnorm;
-                res_l1 = b1 > 0.0 ? tot[1] / b1 : tot[1];
-                if (res_l2 < K.tol) status = EAFP_STOP_CONVERGED;
+                N.res_l2 = sqrt(tot[0]) / N.bnorm;
+                N.res_l1 = N.b1 > 0.0 ? tot[1] / N.b1 : tot[1];
+                if (N.res_l2 < K.tol) status = EAFP_STOP_CONVERGED;
                 else if (rnd >= K.max_steps) status = EAFP_STOP_MAX_ITER;
             }
             done = status != EAFP_STOP_RUNNING;
-            zero_p = (rnd == 0) || K.restart || P.broken;
+            const MxDyn D0 = a.dyn[0][s];
+            zero_p = (rnd == 0) || K.restart || D0.broken;
 #pragma unroll
-            for (int q = 0; q < MAX_COMP; ++q) mean_r[q] = q < K.ncomp ? tot[4 + q] * K.inv_cnt[q] : 0.0;
+            for (int q = 0; q < MAX_COMP; ++q) {
+                mean_r[q] = q < K.ncomp ? tot[4 + q] * K.inv_cnt[q] : 0.0;
+                N.mean_x[q] = q < K.ncomp ? tot[4 + MAX_COMP + q] * K.inv_cnt[q] : 0.0;
+            }
+            N.xbuf = (rnd + 1) & 1;
+            MxDyn D = D0;
+            if (done) {
+                N.done = 1;
+                N.status = status;
+                D.done = 1;
+            } else {
+                N.n_refine = rnd + 1;
+                D.fresh = zero_p ? 1 : 0;
+                D.broken = 0;
+            }
             if (t == T.tile_begin && threadIdx.x == 0) {
-                MxSlot* N = &a.slot[1][s];
-                copy_slot(N, &P);
-                N->bnorm = bnorm;
-                N->b1 = b1;
-                N->res_l2 = res_l2;
-                N->res_l1 = res_l1;
-                N->xbuf = (rnd + 1) & 1;
-#pragma unroll
-                for (int q = 0; q < MAX_COMP; ++q)
-                    N->mean_x[q] = q < K.ncomp ? tot[4 + MAX_COMP + q] * K.inv_cnt[q] : 0.0;
-                if (done) {
-                    N->done = 1;
-                    N->status = status;
-                } else {
-                    N->n_refine = rnd + 1;
-                    N->fresh = zero_p ? 1 : 0;
-                    N->broken = 0;
-                }
+                a.stat[1][s] = N;
+                a.dyn[1][s] = D;
             }
         } else if (t == T.tile_begin && threadIdx.x == 0) {
-            copy_slot(&a.slot[1][s], &P);
+            const MxDyn D0 = a.dyn[0][s];
+            a.stat[1][s] = P;
+            a.dyn[1][s] = D0;
         }
         if (!done) {
+            const double* __restrict__ r64 = a.r64;
+            const unsigned char* __restrict__ comp = a.comp;
+            const float* __restrict__ dinv = a.dinv;
+            float* __restrict__ r = a.r;
+            float* __restrict__ z = a.z;
+            float* __restrict__ d = a.d;
+            float* __restrict__ p0 = a.p[0];
             double acc[2] = {0.0, 0.0};
             const int c_end = min(T.n, (run_end - T.tile_begin) * tile);
             for (int c = (t - T.tile_begin) * tile + (int)threadIdx.x; c < c_end; c += BLOCK) {
                 const long o = T.off + c;
-                const int slot = a.comp[o];
+                const int slot = comp[o];
+                const double rv = r64[o];
+                const float iv = dinv[o];
                 double mu = 0.0;
 #pragma unroll
                 for (int q = 0; q < MAX_COMP; ++q) mu = slot == q ? mean_r[q] : mu;
-                const float rr = (float)(a.r64[o] - mu);
-                const float zz = a.dinv[o] * rr;
-                a.r[o] = rr;
-                a.z[o] = zz;
-                a.d[o] = 0.0f;
-                if (zero_p) a.p[0][o] = 0.0f;
+                const float rr = (float)(rv - mu);
+                const float zz = iv * rr;
+                r[o] = rr;
+                z[o] = zz;
+                d[o] = 0.0f;
+                if (zero_p) p0[o] = 0.0f;
                 acc[0] += (double)rr * (double)rr;
                 acc[1] += (double)rr * (double)zz;
             }
@@ -356,53 +378,80 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv(MxArgs a, int par) {
         const TileDesc T = a.tiles[t];
         const int s = T.sys;
         const int run_end = min(tr.end, T.tile_end);
-        const MxSlot& P = a.slot[1][s];
+        const MxDyn P = a.dyn[1][s];
+        MxDyn N = P;
         bool skip = P.done || P.broken;
-        double beta = 0.0, rz = P.rz;
         if (!skip) {
             double tot[2];
             reduce_tiles<2>(a.partB, T.tile_begin, T.tile_end, tot, sh);
             if (!(tot[1] > 0.0)) {
                 skip = true;     // r.z lost positivity: wait for the next refinement step
+                N.broken = 1;
             } else {
-                beta = P.fresh ? 0.0 : tot[1] / P.rz;
-                rz = tot[1];
+                N.beta = P.fresh ? 0.0 : tot[1] / P.rz;
+                N.rz = tot[1];
+                N.fresh = 0;
             }
         }
-        if (t == T.tile_begin && threadIdx.x == 0) {
-            MxSlot* N = &a.slot[0][s];
-            copy_slot(N, &P);
-            if (!P.done && !P.broken) {
-                N->beta = beta;
-                N->rz = rz;
-                N->fresh = 0;
-                N->broken = skip ? 1 : 0;
-            }
-        }
+        if (t == T.tile_begin && threadIdx.x == 0) a.dyn[0][s] = N;
         if (!skip) {
-            const float fb = (float)beta;
+            const float fb = (float)N.beta;
+            const float* __restrict__ ew = a.e;
+            float* __restrict__ qo = a.q;
             double acc[1] = {0.0};
             const int c_end = min(T.n, (run_end - T.tile_begin) * TILE);
             for (int c0 = (t - T.tile_begin) * TILE + (int)threadIdx.x; c0 < c_end; c0 += TILE) {
+                // load phase: every operand of the CPT cells of this thread
+                float zv[CPT][7], pv[CPT][7], wv[CPT][6], ev[CPT];
+                long oc[CPT];
+                bool ok[CPT], mW[CPT], mB[CPT];
 #pragma unroll
                 for (int m = 0; m < CPT; ++m) {
                     const int c = c0 + m * BLOCK;
-                    if (c >= c_end) break;
-                    const CellIdx id = decompose(c, T);
-                    const long o = T.off + c;
-                    EAFP_NB(id, T, o)
-                    auto pv = [&](long q) { return z[q] + fb * pold[q]; };
-                    const float pc = pv(o);
-                    float y = a.e[o] * pc;
-                    y += W::load(a.w[1], a.scale, o) * (pc - pv(oN));
-                    y += W::load(a.w[1], a.scale, oS) * (pc - pv(oS));
-                    if (hE) y += W::load(a.w[0], a.scale, o) * (pc - pv(oE));
-                    if (hW) y += W::load(a.w[0], a.scale, oW) * (pc - pv(oW));
-                    if (hT) y += W::load(a.w[2], a.scale, o) * (pc - pv(oT));
-                    if (hB) y += W::load(a.w[2], a.scale, oB) * (pc - pv(oB));
-                    pnew[o] = pc;
-                    a.q[o] = y;
-                    acc[0] += (double)pc * (double)y;
+                    ok[m] = c < c_end;
+                    const int cc = ok[m] ? c : c0;
+                    const CellIdx id = decompose(cc, T);
+                    const long o = T.off + cc;
+                    const Nb nb = neighbours(id, T, o);
+                    oc[m] = o;
+                    mW[m] = nb.hW;
+                    mB[m] = nb.hB;
+                    const long idx[7] = {o, nb.W, nb.E, nb.S, nb.N, nb.B, nb.T};
+#pragma unroll
+                    for (int q = 0; q < 7; ++q) {
+                        zv[m][q] = z[idx[q]];
+                        pv[m][q] = pold[idx[q]];
+                    }
+                    // own +faces; -faces are the neighbours' +faces.  A wall
+                    // +face weight is stored as 0 and its neighbour index is
+                    // the cell itself, so it adds 0.
+                    wv[m][0] = W::load(a.w[0], a.scale, o);
+                    wv[m][1] = W::load(a.w[0], a.scale, nb.W);
+                    wv[m][2] = W::load(a.w[1], a.scale, o);
+                    wv[m][3] = W::load(a.w[1], a.scale, nb.S);
+                    wv[m][4] = W::load(a.w[2], a.scale, o);
+                    wv[m][5] = W::load(a.w[2], a.scale, nb.B);
+                    ev[m] = ew[o];
+                }
+                // compute phase
+#pragma unroll
+                for (int m = 0; m < CPT; ++m) {
+                    float p[7];
+#pragma unroll
+                    for (int q = 0; q < 7; ++q) p[q] = zv[m][q] + fb * pv[m][q];
+                    const float pc = p[0];
+                    float y = ev[m] * pc;
+                    y += wv[m][0] * (pc - p[2]);
+                    y += (mW[m] ? wv[m][1] : 0.f) * (pc - p[1]);
+                    y += wv[m][2] * (pc - p[4]);
+                    y += wv[m][3] * (pc - p[3]);
+                    y += wv[m][4] * (pc - p[6]);
+                    y += (mB[m] ? wv[m][5] : 0.f) * (pc - p[5]);
+                    if (ok[m]) {
+                        pnew[oc[m]] = pc;
+                        qo[oc[m]] = y;
+                        acc[0] += (double)pc * (double)y;
+                    }
                 }
             }
             block_sum<1>(acc, sh);
@@ -426,41 +475,57 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate(MxArgs a, int par) {
         const TileDesc T = a.tiles[t];
         const int s = T.sys;
         const int run_end = min(tr.end, T.tile_end);
-        const MxSlot& P = a.slot[0][s];
+        const MxDyn P = a.dyn[0][s];
+        MxDyn N = P;
         bool skip = P.done || P.broken;
-        double alpha = 0.0;
         if (!skip) {
             double tot[1];
             reduce_tiles<1>(a.partA, T.tile_begin, T.tile_end, tot, sh);
-            if (!(fabs(tot[0]) > SMALL)) skip = true;
-            else alpha = P.rz / tot[0];
-        }
-        if (t == T.tile_begin && threadIdx.x == 0) {
-            MxSlot* N = &a.slot[1][s];
-            copy_slot(N, &P);
-            if (!P.done && !P.broken) {
-                N->alpha = alpha;
-                N->broken = skip ? 1 : 0;
-                if (!skip) N->total_inner = P.total_inner + 1;
+            if (!(fabs(tot[0]) > SMALL)) {
+                skip = true;
+                N.broken = 1;
+            } else {
+                N.alpha = P.rz / tot[0];
+                N.total_inner = P.total_inner + 1;
             }
         }
+        if (t == T.tile_begin && threadIdx.x == 0) a.dyn[1][s] = N;
         if (!skip) {
-            const float fa = (float)alpha;
+            const float fa = (float)N.alpha;
+            float* __restrict__ d = a.d;
+            float* __restrict__ r = a.r;
+            float* __restrict__ zo = a.z;
+            const float* __restrict__ q = a.q;
+            const float* __restrict__ dinv = a.dinv;
             double acc[2] = {0.0, 0.0};
             const int c_end = min(T.n, (run_end - T.tile_begin) * TILE);
             for (int c0 = (t - T.tile_begin) * TILE + (int)threadIdx.x; c0 < c_end; c0 += TILE) {
+                float dv[CPT], pvv[CPT], rv[CPT], qv[CPT], iv[CPT];
+                long oc[CPT];
+                bool ok[CPT];
 #pragma unroll
                 for (int m = 0; m < CPT; ++m) {
                     const int c = c0 + m * BLOCK;
-                    if (c >= c_end) break;
-                    const long o = T.off + c;
-                    a.d[o] = a.d[o] + fa * pv[o];
-                    const float rr = a.r[o] - fa * a.q[o];
-                    const float zz = a.dinv[o] * rr;
-                    a.r[o] = rr;
-                    a.z[o] = zz;
-                    acc[0] += (double)rr * (double)rr;
-                    acc[1] += (double)rr * (double)zz;
+                    ok[m] = c < c_end;
+                    const long o = T.off + (ok[m] ? c : c0);
+                    oc[m] = o;
+                    dv[m] = d[o];
+                    pvv[m] = pv[o];
+                    rv[m] = r[o];
+                    qv[m] = q[o];
+                    iv[m] = dinv[o];
+                }
+#pragma unroll
+                for (int m = 0; m < CPT; ++m) {
+                    const float rr = rv[m] - fa * qv[m];
+                    const float zz = iv[m] * rr;
+                    if (ok[m]) {
+                        d[oc[m]] = dv[m] + fa * pvv[m];
+                        r[oc[m]] = rr;
+                        zo[oc[m]] = zz;
+                        acc[0] += (double)rr * (double)rr;
+                        acc[1] += (double)rr * (double)zz;
+                    }
                 }
             }
             block_sum<2>(acc, sh);
@@ -474,7 +539,7 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate(MxArgs a, int par) {
 __global__ __launch_bounds__(BLOCK) void k_mcount(MxArgs a) {
     __shared__ double sh[BLOCK / 64];
     double v[1] = {0.0};
-    for (int s = threadIdx.x; s < a.nsys; s += BLOCK) v[0] += a.slot[1][s].done ? 1.0 : 0.0;
+    for (int s = threadIdx.x; s < a.nsys; s += BLOCK) v[0] += a.stat[1][s].done ? 1.0 : 0.0;
     block_sum<1>(v, sh);
     if (threadIdx.x == 0) *a.ndone = (int)v[0];
 }
@@ -487,7 +552,7 @@ __global__ __launch_bounds__(BLOCK) void k_mproject(MxArgs a, int tile) {
     for (int t = tr.t; t < tr.end;) {
         const TileDesc T = a.tiles[t];
         const int run_end = min(tr.end, T.tile_end);
-        const MxSlot& S = a.slot[1][T.sys];
+        const MxStat& S = a.stat[1][T.sys];
         const double* __restrict__ x = a.X[S.xbuf];
         double mean[MAX_COMP];
 #pragma unroll
@@ -510,7 +575,8 @@ struct MixedWork {
     DevBuf<double> X0, X1, r64, xout, partR, partA, partB;
     DevBuf<unsigned char> w[3];
     DevBuf<float> scale, e, dinv, d, r, z, q, p0, p1;
-    DevBuf<MxSlot> slot0, slot1;
+    DevBuf<MxDyn> dyn0, dyn1;
+    DevBuf<MxStat> stat0, stat1;
     DevBuf<MxConst> cst;
     DevBuf<int> ndone;
     int* host_cnt = nullptr;        // pinned, 2 slots
@@ -554,8 +620,10 @@ static int ensure_mixed(Batch& B, int fmt) {
         EAFP_TRY(hipMemset(w->d.p, 0, n * sizeof(float)));
         EAFP_TRY(hipMemset(w->p0.p, 0, n * sizeof(float)));
         EAFP_TRY(hipMemset(w->p1.p, 0, n * sizeof(float)));
-        EAFP_TRY_RC(w->slot0.alloc(B.nsys));
-        EAFP_TRY_RC(w->slot1.alloc(B.nsys));
+        EAFP_TRY_RC(w->dyn0.alloc(B.nsys));
+        EAFP_TRY_RC(w->dyn1.alloc(B.nsys));
+        EAFP_TRY_RC(w->stat0.alloc(B.nsys));
+        EAFP_TRY_RC(w->stat1.alloc(B.nsys));
         EAFP_TRY_RC(w->cst.alloc(B.nsys));
         EAFP_TRY_RC(w->ndone.alloc(1));
         EAFP_TRY(hipHostMalloc((void**)&w->host_cnt, 2 * sizeof(int), hipHostMallocDefault));
@@ -605,8 +673,10 @@ static int ensure_mixed(Batch& B, int fmt) {
     a.partR = w->partR.p;
     a.partA = w->partA.p;
     a.partB = w->partB.p;
-    a.slot[0] = w->slot0.p;
-    a.slot[1] = w->slot1.p;
+    a.dyn[0] = w->dyn0.p;
+    a.dyn[1] = w->dyn1.p;
+    a.stat[0] = w->stat0.p;
+    a.stat[1] = w->stat1.p;
     a.cst = w->cst.p;
     a.ndone = w->ndone.p;
     return EAFP_OK;
@@ -691,9 +761,11 @@ static int mixed_start(Batch& B, const eafp_mixed_opts& o, int fmt) {
         K.ncomp = B.comp_info[s].ncomp;
         for (int q = 0; q < MAX_COMP; ++q) K.inv_cnt[q] = B.comp_info[s].inv_cnt[q];
     }
-    std::vector<MxSlot> st(B.nsys, MxSlot{});
+    std::vector<MxStat> st(B.nsys, MxStat{});
+    std::vector<MxDyn> dy(B.nsys, MxDyn{});
     EAFP_TRY(hipMemcpyAsync(w->cst.p, cst.data(), sizeof(MxConst) * B.nsys, hipMemcpyHostToDevice, B.stream));
-    EAFP_TRY(hipMemcpyAsync(w->slot1.p, st.data(), sizeof(MxSlot) * B.nsys, hipMemcpyHostToDevice, B.stream));
+    EAFP_TRY(hipMemcpyAsync(w->stat1.p, st.data(), sizeof(MxStat) * B.nsys, hipMemcpyHostToDevice, B.stream));
+    EAFP_TRY(hipMemcpyAsync(w->dyn1.p, dy.data(), sizeof(MxDyn) * B.nsys, hipMemcpyHostToDevice, B.stream));
     const int tile = BLOCK * B.cpt;
     if (fmt == EAFP_COEF_BF16) k_compress<EAFP_COEF_BF16><<<B.grid, BLOCK, 0, B.stream>>>(w->args, tile);
     else if (fmt == EAFP_COEF_FP16) k_compress<EAFP_COEF_FP16><<<B.grid, BLOCK, 0, B.stream>>>(w->args, tile);
@@ -750,8 +822,10 @@ int solve_mixed(Batch& B, const eafp_mixed_opts& o, eafp_stats* stats) {
     EAFP_TRY(hipGetLastError());
     for (int s = 0; s < B.nsys; ++s) EAFP_TRY_RC(download_x(B, s, w->xout.p));
     EAFP_TRY(hipEventRecord(w->t1, B.stream));
-    std::vector<MxSlot> st(B.nsys);
-    EAFP_TRY(hipMemcpyAsync(st.data(), w->slot1.p, sizeof(MxSlot) * B.nsys, hipMemcpyDeviceToHost, B.stream));
+    std::vector<MxStat> st(B.nsys);
+    std::vector<MxDyn> dy(B.nsys);
+    EAFP_TRY(hipMemcpyAsync(st.data(), w->stat1.p, sizeof(MxStat) * B.nsys, hipMemcpyDeviceToHost, B.stream));
+    EAFP_TRY(hipMemcpyAsync(dy.data(), w->dyn1.p, sizeof(MxDyn) * B.nsys, hipMemcpyDeviceToHost, B.stream));
     EAFP_TRY(hipStreamSynchronize(B.stream));
     float ms = 0.f;
     EAFP_TRY(hipEventElapsedTime(&ms, w->t0, w->t1));
@@ -760,7 +834,7 @@ int solve_mixed(Batch& B, const eafp_mixed_opts& o, eafp_stats* stats) {
             eafp_stats& out = stats[s];
             out = eafp_stats{};
             out.residual = st[s].res_l2;
-            out.n_iter = st[s].total_inner;
+            out.n_iter = dy[s].total_inner;
             out.n_refine = st[s].n_refine;
             out.status = st[s].done ? st[s].status : EAFP_STOP_RUNNING;
             out.true_rel_l2 = st[s].res_l2;
