@@ -36,6 +36,7 @@ HBM_PEAK_GBS = 8000.0
 BYTES_MSPMV = {"fp32": 32, "bf16": 26, "fp16": 30}   # z, p_old, 3 weights (+scale), e; p, q
 BYTES_MUPDATE = 32                                   # d, r, p, q, 1/D; d, r, z
 BYTES_FP64_ITER = 152                                # k64_spmv 88 + k64_update 64
+CALIB_BYTES = 1 << 30                                # calibration copy for the HBM counters
 
 
 def log(msg):
@@ -77,6 +78,104 @@ def read_traffic(kernel, workload):
         return None
 
 
+def _counter_rows(path):
+    """(kernel name, counter name, value) of every row of the rocprofv3
+    counter-collection CSV files under `path` (a file or a directory)."""
+    import csv
+    import glob
+    files = [path] if os.path.isfile(path) else sorted(
+        glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True))
+    if not files:
+        raise FileNotFoundError(f"no *counter_collection.csv under {path}")
+    rows = []
+    for fn in files:
+        with open(fn, newline="") as f:
+            for r in csv.DictReader(f):
+                rows.append((r["Kernel_Name"], r["Counter_Name"], float(r["Counter_Value"])))
+    return rows
+
+
+def derive_traffic(fetch_path, write_path, workload, ncells, coef):
+    """HBM bytes per launch of k_mspmv and k_mupdate from two rocprofv3
+    --pmc runs of `bench.py --kernels-only` (FETCH_SIZE, then WRITE_SIZE, in
+    KiB per dispatch).  Each counter is scaled by what it reads for the
+    calibration copy of CALIB_BYTES (the runtime's device-to-device copy
+    kernel), so the result is in bytes the HBM actually moved."""
+    per = {}
+    for path, counter in ((fetch_path, "FETCH_SIZE"), (write_path, "WRITE_SIZE")):
+        for kname, cname, val in _counter_rows(path):
+            if cname != counter:
+                continue
+            key = ("copy" if "copyBuffer" in kname else
+                   "k_mspmv" if "k_mspmv" in kname else "k_mupdate" if "k_mupdate" in kname else None)
+            if key:
+                per.setdefault((key, counter), []).append(val * 1024.0)
+    calib = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        v = per.get(("copy", counter))
+        if not v:
+            raise RuntimeError(f"calibration copy not found in the {counter} run")
+        calib[counter] = CALIB_BYTES / max(v)     # the 1 GiB copy is the largest copy dispatch
+    algo = {"k_mspmv": BYTES_MSPMV[coef] * ncells, "k_mupdate": BYTES_MUPDATE * ncells}
+    rec = {"calibration": {"copy_bytes": CALIB_BYTES, "fetch_factor": round(calib["FETCH_SIZE"], 4),
+                           "write_factor": round(calib["WRITE_SIZE"], 4)}}
+    for k in ("k_mspmv", "k_mupdate"):
+        f = per.get((k, "FETCH_SIZE"), [])
+        w = per.get((k, "WRITE_SIZE"), [])
+        if not f or not w:
+            continue
+        fb = float(np.mean(f)) * calib["FETCH_SIZE"]
+        wb = float(np.mean(w)) * calib["WRITE_SIZE"]
+        rec[k] = {"bytes_per_launch": round(fb + wb), "fetch_bytes": round(fb), "write_bytes": round(wb),
+                  "algorithmic_bytes": algo[k], "traffic_over_algorithmic": round((fb + wb) / algo[k], 4),
+                  "dispatches": [len(f), len(w)]}
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as fh:
+            allrec = json.load(fh)
+    except (OSError, ValueError):
+        allrec = {}
+    allrec[workload] = rec
+    with open(path, "w") as fh:
+        json.dump(allrec, fh, indent=1, sort_keys=True)
+        fh.write("\n")
+    return rec
+
+
+def measure_workload(eafp, syss, fmt, coef, interval, label):
+    """Mixed (batched) and plain FP64 GPU solves of one workload to 1e-8:
+    solves/s, iterations, and the iteration kernels' algorithmic GB/s."""
+    ncells = sum(p.ncells for p in syss)
+    b = eafp.Batch.from_systems(syss)
+    rec = {"workload": label, "systems": len(syss), "cells": ncells, "coef_format": coef}
+    b.autotune(eafp.Batch.SOLVER_MIXED, fmt, reps=3)
+    b.solve_mixed(tol=1e-8, interval=interval, coef_format=fmt)
+    t = time.perf_counter()
+    st = b.solve_mixed(tol=1e-8, interval=interval, coef_format=fmt)
+    tm = time.perf_counter() - t
+    ms_sp, ms_up = b.time_kernels(eafp.Batch.SOLVER_MIXED, fmt, reps=10)
+    rec["mixed"] = {"solves_per_s": round(len(syss) / tm, 3), "seconds": round(tm, 4),
+                    "all_converged": all(x["status"] == "converged" for x in st),
+                    "inner_iters_max": max(x["n_iter"] for x in st),
+                    "max_true_rel_residual": max(x["residual"] for x in st),
+                    "launch_shape": b.info(),
+                    "k_mspmv_GBps": round(BYTES_MSPMV[coef] * ncells / (ms_sp * 1e-3) / 1e9, 1),
+                    "k_mupdate_GBps": round(BYTES_MUPDATE * ncells / (ms_up * 1e-3) / 1e9, 1)}
+    b.autotune(eafp.Batch.SOLVER_FP64, reps=3)
+    t = time.perf_counter()
+    st64 = b.solve_fp64(200000, 1e-8)
+    t64 = time.perf_counter() - t
+    ms64 = b.time_kernels(eafp.Batch.SOLVER_FP64, reps=10)
+    rec["fp64"] = {"solves_per_s": round(len(syss) / t64, 3), "seconds": round(t64, 4),
+                   "all_converged": all(x["status"] == "converged" for x in st64),
+                   "n_iter_max": max(x["n_iter"] for x in st64),
+                   "k64_spmv_GBps": round(88 * ncells / (ms64[0] * 1e-3) / 1e9, 1),
+                   "k64_update_GBps": round(64 * ncells / (ms64[1] * 1e-3) / 1e9, 1)}
+    rec["speedup_mixed_vs_fp64"] = round(t64 / tm, 3)
+    b.close()
+    return rec
+
+
 def rel_err(x, xr, ps):
     from eafp.systems import project_zero_mean
     px = project_zero_mean(x, ps.labels, ps.ncomp)
@@ -95,10 +194,15 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-reference", action="store_true", help="skip the FP64 legs and error checks")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the other workloads (256 ragged systems; one 120x240x168 system in BF16)")
     ap.add_argument("--tiling", default=None, help="pin the launch shape 'cells_per_thread,blocks' (no autotune)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the roofline leg: a 1 GiB device copy (counter calibration) and the timed "
                          "k_mspmv/k_mupdate launches, as measured for the bench line")
+    ap.add_argument("--derive-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write profiles/hbm_traffic.json from the rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                         "outputs of two `bench.py --kernels-only` runs with the same options; no GPU use")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
@@ -106,11 +210,17 @@ def main():
               file=sys.stderr)
         return 2
 
+    B = args.systems
+    workload = f"batch{B}_60x120x84_{args.coef}"
+    if args.derive_traffic:
+        rec = derive_traffic(args.derive_traffic[0], args.derive_traffic[1], workload, B * 60 * 120 * 84, args.coef)
+        print(json.dumps({workload: rec}, indent=1), flush=True)
+        return 0
+
     import eafp
     from eafp import systems
     from eafp._lib import COEF_BF16, COEF_FP16, COEF_FP32
     fmt = {"fp32": COEF_FP32, "bf16": COEF_BF16, "fp16": COEF_FP16}[args.coef]
-    B = args.systems
 
     t0 = time.time()
     syss = [systems.make_system(5, s) for s in range(B)]
@@ -126,13 +236,13 @@ def main():
         tune = batch.autotune(eafp.Batch.SOLVER_MIXED, fmt, reps=5)
         log(f"autotune (mixed {args.coef}): {tune}")
     if args.kernels_only:
-        calibration_copy(1 << 30)
+        calibration_copy(CALIB_BYTES)
         ms_sp, ms_up = batch.time_kernels(eafp.Batch.SOLVER_MIXED, fmt, reps=20)
         out = {"kernels_only": True, "launch_shape": batch.info(), "coef_format": args.coef,
                "k_mspmv_avg_us": round(ms_sp * 1e3, 2), "k_mupdate_avg_us": round(ms_up * 1e3, 2),
                "k_mspmv_algorithmic_bytes": BYTES_MSPMV[args.coef] * ncells,
                "k_mupdate_algorithmic_bytes": BYTES_MUPDATE * ncells,
-               "calibration_copy_bytes": 1 << 30}
+               "calibration_copy_bytes": CALIB_BYTES}
         batch.close()
         print(json.dumps(out), flush=True)
         return 0
@@ -161,7 +271,7 @@ def main():
     b_up = BYTES_MUPDATE * ncells
     ach = b_sp / (ms_sp * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": read_traffic("k_mspmv", "batch64_60x120x84"),
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": read_traffic("k_mspmv", workload),
                 "kernel": f"k_mspmv<{args.coef}>", "avg_launch_us": round(ms_sp * 1e3, 2),
                 "algorithmic_bytes_per_launch": b_sp,
                 "partner": {"kernel": "k_mupdate", "avg_launch_us": round(ms_up * 1e3, 2),
@@ -231,22 +341,46 @@ def main():
         out["mixed_batch"]["rel_err_checked_systems"] = nchk
 
         if not args.no_cpu_baseline:
-            # CPU leg: the oracle restatement of cg_3d_mpi, 1 core, on the
-            # production system (the checker, timed as the baseline only)
+            # CPU legs: the oracle restatement of cg_3d_mpi on the production
+            # system, on all host threads (OpenMP loops, parallel dots) and on
+            # one core (the checker, timed as a baseline only)
             sys.path.insert(0, ROOT)
             import oracle
             oracle.lib()
+            oracle.lib_omp()
+            threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+            t = time.perf_counter()
+            phi_p, res_p, nit_p = oracle.cg_omp(ps, max_iter=100000, tol=1e-8)
+            tp = time.perf_counter() - t
             t = time.perf_counter()
             phi_c, res_c, nit_c, _ = oracle.cg(ps, max_iter=100000, tol=1e-8)
             tc = time.perf_counter() - t
             xc = phi_c[2:-2, 2:-2, 2:-2]
-            out["cpu_baseline"] = {"value": round(1.0 / tc, 5), "unit": "solves/s", "cores": 1, "kind": "port",
-                                   "sample": f"one full FP64 Jacobi-PCG solve (oracle restatement of cg_3d_mpi, "
-                                             f"{nit_c} iterations to 1e-8) of the 60x120x84 production system, "
-                                             f"{tc:.1f} s on 1 host core"}
+            out["cpu_baseline"] = {"value": round(1.0 / tp, 5), "unit": "solves/s", "cores": threads, "kind": "port",
+                                   "sample": f"one full FP64 Jacobi-PCG solve (restatement of cg_3d_mpi with OpenMP "
+                                             f"loops and parallel dots, {nit_p} iterations to 1e-8) of the "
+                                             f"60x120x84 production system, {tp:.2f} s on {threads} host threads",
+                                   "one_core": {"value": round(1.0 / tc, 5), "seconds": round(tc, 2),
+                                                "n_iter": nit_c}}
             out["mixed_single"]["rel_err_vs_cpu_oracle"] = rel_err(xmx, xc, ps)
             out["fp64_single"]["rel_err_vs_cpu_oracle"] = rel_err(x64, xc, ps)
     batch.close()
+
+    if not args.no_extras:
+        # the other workloads of the metric's table, mixed against FP64 on the same GPU
+        out["other_workloads"] = []
+        syss3 = [systems.make_system(3, s) for s in range(256)]
+        out["other_workloads"].append(measure_workload(
+            eafp, syss3, fmt, args.coef, args.interval,
+            "256 ragged systems U{18..35}x{36..66}x{24..50}, liquid height and gas density jittered"))
+        del syss3
+        log("workload 256 ragged systems done")
+        ps4 = systems.make_system(4, 0)
+        out["other_workloads"].append(measure_workload(
+            eafp, [ps4], COEF_BF16, "bf16", args.interval,
+            "one 120x240x168 system, scrap band 0.9<z<2.5 m with face coefficients x1e-2(1-alpha_s), BF16 weights"))
+        del ps4
+        log("workload 120x240x168 done")
     print(json.dumps(out), flush=True)
     return 0
 

@@ -44,30 +44,8 @@ bool valid_cpt(int cpt) { return cpt == 1 || cpt == 2 || cpt == 4 || cpt == 8; }
 int Batch::set_tiling(int cells_per_thread, int blocks) {
     if (!valid_cpt(cells_per_thread) || blocks < 0) return EAFP_ERR_ARG;
     const int tile = BLOCK * cells_per_thread;
-    tiles_h.clear();
-    int t = 0;
-    for (int s = 0; s < nsys; ++s) {
-        SysGeom& g = geom[s];
-        const int nt = (g.n + tile - 1) / tile;
-        g.tile_begin = t;
-        g.tile_end = t + nt;
-        for (int u = 0; u < nt; ++u) {
-            TileDesc d{};
-            d.sys = s;
-            d.tile_begin = g.tile_begin;
-            d.tile_end = g.tile_end;
-            d.n = g.n;
-            d.nr = g.nr;
-            d.nth = g.nth;
-            d.nz = g.nz;
-            d.off = g.off;
-            d.inv_nz = g.inv_nz;
-            d.inv_nth = g.inv_nth;
-            tiles_h.push_back(d);
-        }
-        t += nt;
-    }
-    ntiles = t;
+    tiles_h = make_tile_table(geom, tile);
+    ntiles = (int)tiles_h.size();
     cpt = cells_per_thread;
     EAFP_TRY_RC(tiles.alloc(ntiles));
     EAFP_TRY(hipMemcpy(tiles.p, tiles_h.data(), sizeof(TileDesc) * ntiles, hipMemcpyHostToDevice));
@@ -98,20 +76,12 @@ int Batch::init(int n, const eafp_desc* descs) {
     for (int s = 0; s < n; ++s) EAFP_TRY_RC(validate_desc(&descs[s]));
     nsys = n;
     geom.resize(n);
-    long off = 0;
     for (int s = 0; s < n; ++s) {
-        SysGeom& g = geom[s];
-        g.nr = descs[s].nr;
-        g.nth = descs[s].ntheta;
-        g.nz = descs[s].nz;
-        g.n = g.nr * g.nth * g.nz;
-        g.off = off;
-        g.inv_nz = 1.0f / (float)g.nz;
-        g.inv_nth = 1.0f / (float)g.nth;
-        // keep every system's first cell 256-byte aligned for FP64 arrays
-        off += (g.n + 31) / 32 * 32;
+        geom[s].nr = descs[s].nr;
+        geom[s].nth = descs[s].ntheta;
+        geom[s].nz = descs[s].nz;
     }
-    ncells = off;
+    ncells = layout_systems(geom);
     EAFP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     EAFP_TRY_RC(geom_d.alloc(n));
     EAFP_TRY_RC(set_tiling(2, 0));

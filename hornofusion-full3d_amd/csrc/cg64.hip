@@ -96,42 +96,45 @@ template <int CPT>
 __global__ __launch_bounds__(BLOCK) void k64_init(Cg64Args a) {
     constexpr int TILE = BLOCK * CPT;
     __shared__ double sh[3 * (BLOCK / 64)];
-    const TileRange tr(a.ntiles);
-    for (int t = tr.t; t < tr.end;) {
+    const TileWalk w = my_tiles(a.ntiles);
+    for (int t = w.first; t < w.end;) {
         const TileDesc T = a.tiles[t];
-        const int run_end = min(tr.end, T.tile_end);
+        const int nrun = w.run(t, T.tile_end);
         if (t == T.tile_begin && threadIdx.x == 0) {
             CgSlot n{};
             n.bnorm = -1.0;
-            a.slot[1][T.sys] = n;
+            store_state(&a.slot[1][T.sys], n);
         }
         double acc[3] = {0.0, 0.0, 0.0};
-        const int c_end = min(T.n, (run_end - T.tile_begin) * TILE);
-        for (int c = (t - T.tile_begin) * TILE + (int)threadIdx.x; c < c_end; c += BLOCK) {
-            const CellIdx id = decompose(c, T);
-            const long o = T.off + c;
-            const Nb nb = neighbours(id, T, o);
-            const double* x = a.x0;
-            const double xc = x[o], xW = x[nb.W], xE = x[nb.E], xS = x[nb.S], xN = x[nb.N], xB = x[nb.B],
-                         xT = x[nb.T];
-            const double ap = a.aP[o];
-            const double ax = ap * xc - a.aW[o] * (nb.hW ? xW : 0.0) - a.aE[o] * (nb.hE ? xE : 0.0) -
-                              a.aS[o] * xS - a.aN[o] * xN - a.aB[o] * (nb.hB ? xB : 0.0) -
-                              a.aT[o] * (nb.hT ? xT : 0.0);
-            const double su = a.b[o];
-            const double rr = su - (ax - a.fold[o]);
-            const double zz = rr / fmax(fabs(ap), SMALL);
-            a.x[o] = xc;
-            a.r[o] = rr;
-            a.z[o] = zz;
-            a.p[0][o] = 0.0;
-            acc[0] += rr * rr;
-            acc[1] += rr * zz;
-            acc[2] += su * su;
+        for (int u = 0; u < nrun; ++u) {
+            const int base = (t + u * w.stride - T.tile_begin) * TILE;
+            const int c_end = min(T.n, base + TILE);
+            for (int c = base + (int)threadIdx.x; c < c_end; c += BLOCK) {
+                const CellIdx id = decompose(c, T);
+                const long o = T.off + c;
+                const Nb nb = neighbours(id, T, o);
+                const double* x = a.x0;
+                const double xc = x[o], xW = x[nb.W], xE = x[nb.E], xS = x[nb.S], xN = x[nb.N], xB = x[nb.B],
+                             xT = x[nb.T];
+                const double ap = a.aP[o];
+                const double ax = ap * xc - a.aW[o] * (nb.hW ? xW : 0.0) - a.aE[o] * (nb.hE ? xE : 0.0) -
+                                  a.aS[o] * xS - a.aN[o] * xN - a.aB[o] * (nb.hB ? xB : 0.0) -
+                                  a.aT[o] * (nb.hT ? xT : 0.0);
+                const double su = a.b[o];
+                const double rr = su - (ax - a.fold[o]);
+                const double zz = rr / fmax(fabs(ap), SMALL);
+                a.x[o] = xc;
+                a.r[o] = rr;
+                a.z[o] = zz;
+                a.p[0][o] = 0.0;
+                acc[0] += rr * rr;
+                acc[1] += rr * zz;
+                acc[2] += su * su;
+            }
         }
         block_sum<3>(acc, sh);
-        store_run<3>(a.partB, t, run_end, acc);
-        t = run_end;
+        store_run<3>(a.partB, t, nrun, w.stride, acc);
+        t += nrun * w.stride;
     }
 }
 
@@ -147,11 +150,54 @@ __global__ __launch_bounds__(BLOCK) void k64_spmv(Cg64Args a, int par) {
     const double* __restrict__ pold = a.p[par];
     double* __restrict__ pnew = a.p[par ^ 1];
     const double* __restrict__ z = a.z;
-    const TileRange tr(a.ntiles);
-    for (int t = tr.t; t < tr.end;) {
+    const double* __restrict__ cP = a.aP;
+    const double* __restrict__ cW = a.aW;
+    const double* __restrict__ cE = a.aE;
+    const double* __restrict__ cS = a.aS;
+    const double* __restrict__ cN = a.aN;
+    const double* __restrict__ cB = a.aB;
+    const double* __restrict__ cT = a.aT;
+    double* __restrict__ qo = a.q;
+    const TileWalk w = my_tiles(a.ntiles);
+    for (int t = w.first; t < w.end;) {
         const TileDesc T = a.tiles[t];
         const int s = T.sys;
-        const int run_end = min(tr.end, T.tile_end);
+        const int nrun = w.run(t, T.tile_end);
+        double zv[CPT][7], pv[CPT][7], cf[CPT][7];
+        long oc[CPT];
+        bool ok[CPT], mW[CPT], mE[CPT], mB[CPT], mT[CPT];
+        auto load = [&](int tile) {
+            const int c0 = (tile - T.tile_begin) * TILE + (int)threadIdx.x;
+#pragma unroll
+            for (int m = 0; m < CPT; ++m) {
+                const int c = c0 + m * BLOCK;
+                ok[m] = c < T.n;
+                const int cc = clamp_cell(c, T.n);
+                const CellIdx id = decompose(cc, T);
+                const long o = T.off + cc;
+                const Nb nb = neighbours(id, T, o);
+                oc[m] = o;
+                mW[m] = nb.hW;
+                mE[m] = nb.hE;
+                mB[m] = nb.hB;
+                mT[m] = nb.hT;
+                const long idx[7] = {o, nb.W, nb.E, nb.S, nb.N, nb.B, nb.T};
+#pragma unroll
+                for (int q = 0; q < 7; ++q) {
+                    zv[m][q] = z[idx[q]];
+                    pv[m][q] = pold[idx[q]];
+                }
+                cf[m][0] = cP[o];
+                cf[m][1] = cW[o];
+                cf[m][2] = cE[o];
+                cf[m][3] = cS[o];
+                cf[m][4] = cN[o];
+                cf[m][5] = cB[o];
+                cf[m][6] = cT[o];
+            }
+        };
+        // the first batch of cell loads is in flight while the partials are reduced
+        load(t);
         const CgSlot P = a.slot[1][s];
         CgSlot N = P;
         if (!P.done) {
@@ -190,69 +236,32 @@ __global__ __launch_bounds__(BLOCK) void k64_spmv(Cg64Args a, int par) {
             }
             N.it = P.it + 1;
         }
-        if (t == T.tile_begin && threadIdx.x == 0) a.slot[0][s] = N;
+        if (t == T.tile_begin && threadIdx.x == 0) store_state(&a.slot[0][s], N);
         if (!N.done) {
             const double beta = N.beta;
-            const double* __restrict__ cP = a.aP;
-            const double* __restrict__ cW = a.aW;
-            const double* __restrict__ cE = a.aE;
-            const double* __restrict__ cS = a.aS;
-            const double* __restrict__ cN = a.aN;
-            const double* __restrict__ cB = a.aB;
-            const double* __restrict__ cT = a.aT;
-            double* __restrict__ qo = a.q;
             double acc[1] = {0.0};
-            const int c_end = min(T.n, (run_end - T.tile_begin) * TILE);
-            for (int c0 = (t - T.tile_begin) * TILE + (int)threadIdx.x; c0 < c_end; c0 += TILE) {
-                // load phase: every operand of the CPT cells of this thread
-                double zv[CPT][7], pv[CPT][7], cf[CPT][7];
-                long oc[CPT];
-                bool ok[CPT], mW[CPT], mE[CPT], mB[CPT], mT[CPT];
+            for (int u = 0; u < nrun; ++u) {
+                if (u) load(t + u * w.stride);
 #pragma unroll
                 for (int m = 0; m < CPT; ++m) {
-                    const int c = c0 + m * BLOCK;
-                    ok[m] = c < c_end;
-                    const int cc = ok[m] ? c : c0;
-                    const CellIdx id = decompose(cc, T);
-                    const long o = T.off + cc;
-                    const Nb nb = neighbours(id, T, o);
-                    oc[m] = o;
-                    mW[m] = nb.hW; mE[m] = nb.hE; mB[m] = nb.hB; mT[m] = nb.hT;
-                    const long idx[7] = {o, nb.W, nb.E, nb.S, nb.N, nb.B, nb.T};
+                    double pp[7];
 #pragma unroll
-                    for (int q = 0; q < 7; ++q) {
-                        zv[m][q] = z[idx[q]];
-                        pv[m][q] = pold[idx[q]];
-                    }
-                    cf[m][0] = cP[o];
-                    cf[m][1] = cW[o];
-                    cf[m][2] = cE[o];
-                    cf[m][3] = cS[o];
-                    cf[m][4] = cN[o];
-                    cf[m][5] = cB[o];
-                    cf[m][6] = cT[o];
-                }
-                // compute phase
-#pragma unroll
-                for (int m = 0; m < CPT; ++m) {
-                    double p[7];
-#pragma unroll
-                    for (int q = 0; q < 7; ++q) p[q] = zv[m][q] + beta * pv[m][q];
-                    const double pW = mW[m] ? p[1] : 0.0, pE = mE[m] ? p[2] : 0.0;
-                    const double pB = mB[m] ? p[5] : 0.0, pT = mT[m] ? p[6] : 0.0;
-                    const double y = cf[m][0] * p[0] - cf[m][1] * pW - cf[m][2] * pE - cf[m][3] * p[3] -
-                                     cf[m][4] * p[4] - cf[m][5] * pB - cf[m][6] * pT;
+                    for (int q = 0; q < 7; ++q) pp[q] = zv[m][q] + beta * pv[m][q];
+                    const double pW = mW[m] ? pp[1] : 0.0, pE = mE[m] ? pp[2] : 0.0;
+                    const double pB = mB[m] ? pp[5] : 0.0, pT = mT[m] ? pp[6] : 0.0;
+                    const double y = cf[m][0] * pp[0] - cf[m][1] * pW - cf[m][2] * pE - cf[m][3] * pp[3] -
+                                     cf[m][4] * pp[4] - cf[m][5] * pB - cf[m][6] * pT;
                     if (ok[m]) {
-                        pnew[oc[m]] = p[0];
+                        pnew[oc[m]] = pp[0];
                         qo[oc[m]] = y;
-                        acc[0] += p[0] * y;
+                        acc[0] += pp[0] * y;
                     }
                 }
             }
             block_sum<1>(acc, sh);
-            store_run<1>(a.partA, t, run_end, acc);
+            store_run<1>(a.partA, t, nrun, w.stride, acc);
         }
-        t = run_end;
+        t += nrun * w.stride;
     }
 }
 
@@ -266,11 +275,36 @@ __global__ __launch_bounds__(BLOCK) void k64_update(Cg64Args a, int par) {
     constexpr int TILE = BLOCK * CPT;
     __shared__ double sh[3 * (BLOCK / 64)];
     const double* __restrict__ pv = a.p[par ^ 1];
-    const TileRange tr(a.ntiles);
-    for (int t = tr.t; t < tr.end;) {
+    double* __restrict__ x = a.x;
+    double* __restrict__ r = a.r;
+    double* __restrict__ zo = a.z;
+    const double* __restrict__ q = a.q;
+    const double* __restrict__ aP = a.aP;
+    const TileWalk w = my_tiles(a.ntiles);
+    for (int t = w.first; t < w.end;) {
         const TileDesc T = a.tiles[t];
         const int s = T.sys;
-        const int run_end = min(tr.end, T.tile_end);
+        const int nrun = w.run(t, T.tile_end);
+        double xv[CPT], pvv[CPT], rv[CPT], qv[CPT], apv[CPT];
+        long oc[CPT];
+        bool ok[CPT];
+        auto load = [&](int tile) {
+            const int c0 = (tile - T.tile_begin) * TILE + (int)threadIdx.x;
+#pragma unroll
+            for (int m = 0; m < CPT; ++m) {
+                const int c = c0 + m * BLOCK;
+                ok[m] = c < T.n;
+                const long o = T.off + clamp_cell(c, T.n);
+                oc[m] = o;
+                xv[m] = x[o];
+                pvv[m] = pv[o];
+                rv[m] = r[o];
+                qv[m] = q[o];
+                apv[m] = aP[o];
+            }
+        };
+        // the first batch of cell loads is in flight while the partials are reduced
+        load(t);
         const CgSlot P = a.slot[0][s];
         CgSlot N = P;
         if (!P.done) {
@@ -284,32 +318,12 @@ __global__ __launch_bounds__(BLOCK) void k64_update(Cg64Args a, int par) {
                 N.alpha = P.rz / pq;
             }
         }
-        if (t == T.tile_begin && threadIdx.x == 0) a.slot[1][s] = N;
+        if (t == T.tile_begin && threadIdx.x == 0) store_state(&a.slot[1][s], N);
         if (!N.done) {
             const double alpha = N.alpha;
-            double* __restrict__ x = a.x;
-            double* __restrict__ r = a.r;
-            double* __restrict__ zo = a.z;
-            const double* __restrict__ q = a.q;
-            const double* __restrict__ aP = a.aP;
             double acc[3] = {0.0, 0.0, 0.0};
-            const int c_end = min(T.n, (run_end - T.tile_begin) * TILE);
-            for (int c0 = (t - T.tile_begin) * TILE + (int)threadIdx.x; c0 < c_end; c0 += TILE) {
-                double xv[CPT], pvv[CPT], rv[CPT], qv[CPT], apv[CPT];
-                long oc[CPT];
-                bool ok[CPT];
-#pragma unroll
-                for (int m = 0; m < CPT; ++m) {
-                    const int c = c0 + m * BLOCK;
-                    ok[m] = c < c_end;
-                    const long o = T.off + (ok[m] ? c : c0);
-                    oc[m] = o;
-                    xv[m] = x[o];
-                    pvv[m] = pv[o];
-                    rv[m] = r[o];
-                    qv[m] = q[o];
-                    apv[m] = aP[o];
-                }
+            for (int u = 0; u < nrun; ++u) {
+                if (u) load(t + u * w.stride);
 #pragma unroll
                 for (int m = 0; m < CPT; ++m) {
                     const double rr = rv[m] - alpha * qv[m];
@@ -324,9 +338,9 @@ __global__ __launch_bounds__(BLOCK) void k64_update(Cg64Args a, int par) {
                 }
             }
             block_sum<3>(acc, sh);
-            store_run<3>(a.partB, t, run_end, acc);
+            store_run<3>(a.partB, t, nrun, w.stride, acc);
         }
-        t = run_end;
+        t += nrun * w.stride;
     }
 }
 

@@ -9,16 +9,19 @@
 //
 // Work decomposition: each system is cut into tiles of BLOCK*CPT
 // consecutive cells (CPT = cells per thread, chosen by the autotuner).  All
-// tiles of all systems form one table; every block takes a contiguous
-// range of it, with consecutive ranges on the same XCD so stencil
-// neighbours hit the same L2.  A block accumulates each run of tiles of one
-// system in registers and writes one partial sum at the run's first tile
-// (zeros at the others); the next kernel reduces a system's partials in
-// tile order at its start, so the reduction is deterministic and nothing
-// crosses between blocks inside a kernel.
+// tiles of all systems form one table, cut into one contiguous chunk per
+// XCD; the blocks of an XCD sweep its chunk together, block by block and
+// tile by tile (TileWalk), so the r-neighbour planes a stencil reads were
+// brought into that XCD's L2 by the other blocks of the same sweep.  A
+// block accumulates the tiles it takes of one system (a run) in registers
+// and writes one partial sum at the run's first tile (zeros at its others);
+// the next kernel reduces a system's partials in tile order at its start,
+// so the reduction is deterministic and nothing crosses between blocks
+// inside a kernel.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 
 namespace eafp {
 
@@ -50,9 +53,52 @@ struct SysGeom {
     float inv_nth;
 };
 
+// Lays out the systems' cells in one arena: fills nr/nth/nz/n/off and the
+// division reciprocals of every entry of g (whose nr, nth, nz are set) and
+// returns the arena length.  Every system starts at a 32-cell boundary, so
+// its FP64 arrays start 256-byte aligned.
+inline long layout_systems(std::vector<SysGeom>& g) {
+    long off = 0;
+    for (SysGeom& s : g) {
+        s.n = s.nr * s.nth * s.nz;
+        s.off = off;
+        s.inv_nz = 1.0f / (float)s.nz;
+        s.inv_nth = 1.0f / (float)s.nth;
+        off += (s.n + 31) / 32 * 32;
+    }
+    return off;
+}
+
+// The tile table for tiles of `tile` cells: each system's cells cut into
+// consecutive tiles, systems in order.  Sets every system's tile range.
+inline std::vector<TileDesc> make_tile_table(std::vector<SysGeom>& g, int tile) {
+    std::vector<TileDesc> tiles;
+    int t = 0;
+    for (int s = 0; s < (int)g.size(); ++s) {
+        SysGeom& G = g[s];
+        const int nt = (G.n + tile - 1) / tile;
+        G.tile_begin = t;
+        G.tile_end = t + nt;
+        TileDesc d{};
+        d.sys = s;
+        d.tile_begin = G.tile_begin;
+        d.tile_end = G.tile_end;
+        d.n = G.n;
+        d.nr = G.nr;
+        d.nth = G.nth;
+        d.nz = G.nz;
+        d.off = G.off;
+        d.inv_nz = G.inv_nz;
+        d.inv_nth = G.inv_nth;
+        tiles.insert(tiles.end(), nt, d);
+        t += nt;
+    }
+    return tiles;
+}
+
 // Quotient of u by d for u < 2^24 using a float reciprocal and a one-step
 // integer correction (exact for every u < 2^24, d >= 1).
-__device__ __forceinline__ int fdiv(int u, int d, float inv_d) {
+__host__ __device__ __forceinline__ int fdiv(int u, int d, float inv_d) {
     int q = (int)((float)u * inv_d);
     int r = u - q * d;
     if (r >= d) q += 1;
@@ -75,7 +121,7 @@ struct Nb {
 };
 
 template <typename G>
-__device__ __forceinline__ Nb neighbours(const CellIdx& id, const G& g, long o) {
+__host__ __device__ __forceinline__ Nb neighbours(const CellIdx& id, const G& g, long o) {
     Nb n;
     const long sr = (long)g.nth * g.nz;
     n.hW = id.i > 0;
@@ -92,7 +138,7 @@ __device__ __forceinline__ Nb neighbours(const CellIdx& id, const G& g, long o) 
 }
 
 template <typename G>
-__device__ __forceinline__ CellIdx decompose(int local, const G& g) {
+__host__ __device__ __forceinline__ CellIdx decompose(int local, const G& g) {
     int col = fdiv(local, g.nz, g.inv_nz);     // i*nth + j
     int k = local - col * g.nz;
     int i = fdiv(col, g.nth, g.inv_nth);
@@ -151,17 +197,18 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* sh) {
     __syncthreads();
 }
 
-// Writes the block's run sum v (NV columns) at tile t0 and zeros at the
-// other tiles of the run [t0, t1).
+// Writes the block's run sum v (NV columns) at the run's first tile t0 and
+// zeros at its other n-1 tiles t0 + u*stride.
 template <int NV>
-__device__ __forceinline__ void store_run(double* partial, int t0, int t1, const double (&v)[NV]) {
+__device__ __forceinline__ void store_run(double* partial, int t0, int n, int stride, const double (&v)[NV]) {
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int c = 0; c < NV; ++c) partial[(long)t0 * NV + c] = v[c];
     }
-    for (int u = t0 + 1 + (int)threadIdx.x; u < t1; u += BLOCK) {
+    for (int u = 1 + (int)threadIdx.x; u < n; u += BLOCK) {
+        const long t = t0 + (long)u * stride;
 #pragma unroll
-        for (int c = 0; c < NV; ++c) partial[(long)u * NV + c] = 0.0;
+        for (int c = 0; c < NV; ++c) partial[t * NV + c] = 0.0;
     }
 }
 
@@ -179,19 +226,53 @@ __device__ __forceinline__ void reduce_tiles(const double* partial, int t0, int 
     block_sum<NV>(out, sh);
 }
 
-// Contiguous tile range of this block.  Logical block lb = x*(G/8) + b/8
-// for the block b on XCD x = b % 8 (blocks are dealt to XCDs round robin),
-// so consecutive tiles -- stencil neighbours -- run on the same XCD and
-// share its L2; a block's tiles touch few systems.
-struct TileRange {
-    int t, end;
-    __device__ TileRange(int ntiles) {
-        const int G = gridDim.x;
-        const int lb = (blockIdx.x % NXCD) * (G / NXCD) + blockIdx.x / NXCD;
-        t = (int)(((long)ntiles * lb) / G);
-        end = (int)(((long)ntiles * (lb + 1)) / G);
+// The tiles of block b of a grid of G blocks (G a multiple of NXCD).  The
+// hardware deals blocks to the XCDs round robin, block b to XCD x = b % 8;
+// the table is cut into 8 contiguous chunks, chunk x for XCD x, and inside
+// it the block takes tiles chunk_begin + b/8 + u*(G/8), u = 0, 1, ...  At
+// any moment the blocks of one XCD thus work on one contiguous window of
+// the table: a stencil's r-neighbour plane (nth*nz cells away) lies in the
+// same window or the one just before it, already in that XCD's L2.  When G
+// is at least the tile count every block takes at most one tile.
+struct TileWalk {
+    int first;    // the block's first tile (>= end: no tiles)
+    int end;      // end of the XCD's chunk
+    int stride;   // G / 8
+    __host__ __device__ TileWalk(int ntiles, int G, int b) {
+        const int x = b % NXCD;
+        const int begin = (int)(((long)ntiles * x) / NXCD);
+        end = (int)(((long)ntiles * (x + 1)) / NXCD);
+        stride = G / NXCD;
+        first = begin + b / NXCD;
+    }
+    // Tiles of the run that starts at the block's tile t and ends with the
+    // system's last tile or the chunk: t, t + stride, ... < min(end, tile_end).
+    __host__ __device__ int run(int t, int tile_end) const {
+        const int e = tile_end < end ? tile_end : end;
+        return (e - t + stride - 1) / stride;
     }
 };
+
+__device__ __forceinline__ TileWalk my_tiles(int ntiles) { return TileWalk(ntiles, (int)gridDim.x, (int)blockIdx.x); }
+
+// Compact cell index of a tile's slot: the cell itself when it exists, else
+// the system's last cell, so that a load for a slot past the end of a
+// system's last (partial) tile stays inside the system.
+__host__ __device__ __forceinline__ int clamp_cell(int c, int n) { return c < n ? c : n - 1; }
+
+// Writes a small per-system state record with 8-byte nontemporal stores.
+// The record is written by one thread of one block per kernel and read by
+// every block of the next kernel; measured on MI355X, an ordinary store of
+// it held the writing block back by ~13 us (a 604,800-cell update kernel
+// went from 9 us to 22 us), a nontemporal store costs nothing measurable.
+template <typename R>
+__device__ __forceinline__ void store_state(R* dst, const R& v) {
+    static_assert(sizeof(R) % 8 == 0, "state records are whole 8-byte words");
+    const unsigned long long* s = reinterpret_cast<const unsigned long long*>(&v);
+    unsigned long long* d = reinterpret_cast<unsigned long long*>(dst);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(R) / 8); ++i) __builtin_nontemporal_store(s[i], d + i);
+}
 
 // Per-system scalar state lives in two slots.  Kernels come in two roles
 // that alternate in every launch sequence: role 0 reads slot 1 and writes

@@ -144,48 +144,51 @@ size_t wbytes(int fmt) { return fmt == EAFP_COEF_FP32 ? 4 : 2; }
 template <int F>
 __global__ __launch_bounds__(BLOCK) void k_compress(MxArgs a, int tile) {
     using W = WFmt<F>;
-    const TileRange tr(a.ntiles);
-    for (int t = tr.t; t < tr.end;) {
+    const TileWalk w = my_tiles(a.ntiles);
+    for (int t = w.first; t < w.end;) {
         const TileDesc T = a.tiles[t];
-        const int run_end = min(tr.end, T.tile_end);
-        const int c_end = min(T.n, (run_end - T.tile_begin) * tile);
-        for (int c = (t - T.tile_begin) * tile + (int)threadIdx.x; c < c_end; c += BLOCK) {
-            const CellIdx id = decompose(c, T);
-            const long o = T.off + c;
-            const Nb nb = neighbours(id, T, o);
-            const bool hW = nb.hW, hE = nb.hE, hB = nb.hB, hT = nb.hT;
-            const long oW = nb.W, oE = nb.E, oS = nb.S, oN = nb.N, oB = nb.B, oT = nb.T;
-            const double ap = a.aP[o];
-            const double wr = hE ? 0.5 * (a.aE[o] + a.aW[oE]) : 0.0;
-            const double wrm = hW ? 0.5 * (a.aE[oW] + a.aW[o]) : 0.0;
-            const double wt = 0.5 * (a.aN[o] + a.aS[oN]);
-            const double wtm = 0.5 * (a.aN[oS] + a.aS[o]);
-            const double wz = hT ? 0.5 * (a.aT[o] + a.aB[oT]) : 0.0;
-            const double wzm = hB ? 0.5 * (a.aT[oB] + a.aB[o]) : 0.0;
-            const double e = ap - (wr + wrm + wt + wtm + wz + wzm);
-            // scale of a cell's own weights (FP16 only): its diagonal
-            auto sc = [&](long q) {
-                const float v = (float)a.aP[q];
-                return v > 0.f ? v : 1.f;
-            };
-            const float s_o = sc(o);
-            const typename W::T er = W::enc((float)wr, s_o), et = W::enc((float)wt, s_o),
-                                 ez = W::enc((float)wz, s_o);
-            // the rounded weights exactly as the inner operator sees them
-            float D = W::dec(er, s_o) + W::dec(et, s_o) + W::dec(ez, s_o);
-            D += hW ? W::dec(W::enc((float)wrm, sc(oW)), sc(oW)) : 0.f;
-            D += W::dec(W::enc((float)wtm, sc(oS)), sc(oS));
-            D += hB ? W::dec(W::enc((float)wzm, sc(oB)), sc(oB)) : 0.f;
-            const float e32 = (float)e;
-            D += e32;
-            ((typename W::T*)a.w[0])[o] = er;
-            ((typename W::T*)a.w[1])[o] = et;
-            ((typename W::T*)a.w[2])[o] = ez;
-            if (F == EAFP_COEF_FP16) a.scale[o] = s_o;
-            a.e[o] = e32;
-            a.dinv[o] = D > 0.f ? 1.0f / D : 0.0f;
+        const int nrun = w.run(t, T.tile_end);
+        for (int u = 0; u < nrun; ++u) {
+            const int base = (t + u * w.stride - T.tile_begin) * tile;
+            const int c_end = min(T.n, base + tile);
+            for (int c = base + (int)threadIdx.x; c < c_end; c += BLOCK) {
+                const CellIdx id = decompose(c, T);
+                const long o = T.off + c;
+                const Nb nb = neighbours(id, T, o);
+                const bool hW = nb.hW, hE = nb.hE, hB = nb.hB, hT = nb.hT;
+                const long oW = nb.W, oE = nb.E, oS = nb.S, oN = nb.N, oB = nb.B, oT = nb.T;
+                const double ap = a.aP[o];
+                const double wr = hE ? 0.5 * (a.aE[o] + a.aW[oE]) : 0.0;
+                const double wrm = hW ? 0.5 * (a.aE[oW] + a.aW[o]) : 0.0;
+                const double wt = 0.5 * (a.aN[o] + a.aS[oN]);
+                const double wtm = 0.5 * (a.aN[oS] + a.aS[o]);
+                const double wz = hT ? 0.5 * (a.aT[o] + a.aB[oT]) : 0.0;
+                const double wzm = hB ? 0.5 * (a.aT[oB] + a.aB[o]) : 0.0;
+                const double e = ap - (wr + wrm + wt + wtm + wz + wzm);
+                // scale of a cell's own weights (FP16 only): its diagonal
+                auto sc = [&](long q) {
+                    const float v = (float)a.aP[q];
+                    return v > 0.f ? v : 1.f;
+                };
+                const float s_o = sc(o);
+                const typename W::T er = W::enc((float)wr, s_o), et = W::enc((float)wt, s_o),
+                                     ez = W::enc((float)wz, s_o);
+                // the rounded weights exactly as the inner operator sees them
+                float D = W::dec(er, s_o) + W::dec(et, s_o) + W::dec(ez, s_o);
+                D += hW ? W::dec(W::enc((float)wrm, sc(oW)), sc(oW)) : 0.f;
+                D += W::dec(W::enc((float)wtm, sc(oS)), sc(oS));
+                D += hB ? W::dec(W::enc((float)wzm, sc(oB)), sc(oB)) : 0.f;
+                const float e32 = (float)e;
+                D += e32;
+                ((typename W::T*)a.w[0])[o] = er;
+                ((typename W::T*)a.w[1])[o] = et;
+                ((typename W::T*)a.w[2])[o] = ez;
+                if (F == EAFP_COEF_FP16) a.scale[o] = s_o;
+                a.e[o] = e32;
+                a.dinv[o] = D > 0.f ? 1.0f / D : 0.0f;
+            }
         }
-        t = run_end;
+        t += nrun * w.stride;
     }
 }
 
@@ -196,19 +199,19 @@ __global__ __launch_bounds__(BLOCK) void k_compress(MxArgs a, int tile) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_res(MxArgs a, int tile) {
     __shared__ double sh[NRES * (BLOCK / 64)];
-    const TileRange tr(a.ntiles);
+    const TileWalk w = my_tiles(a.ntiles);
     const float* __restrict__ d = a.d;
-    for (int t = tr.t; t < tr.end;) {
+    for (int t = w.first; t < w.end;) {
         const TileDesc T = a.tiles[t];
         const int s = T.sys;
-        const int run_end = min(tr.end, T.tile_end);
+        const int nrun = w.run(t, T.tile_end);
         const int done = a.stat[1][s].done;
         const int rnd = a.stat[1][s].n_refine;
         if (t == T.tile_begin && threadIdx.x == 0) {
             const MxStat st = a.stat[1][s];
             const MxDyn dy = a.dyn[1][s];
-            a.stat[0][s] = st;
-            a.dyn[0][s] = dy;
+            store_state(&a.stat[0][s], st);
+            store_state(&a.dyn[0][s], dy);
         }
         if (!done) {
             const bool first = (rnd == 0);
@@ -228,38 +231,41 @@ __global__ __launch_bounds__(BLOCK) void k_res(MxArgs a, int tile) {
             const double* __restrict__ fold = a.fold;
             const unsigned char* __restrict__ comp = a.comp;
             double* __restrict__ r64 = a.r64;
-            const int c_end = min(T.n, (run_end - T.tile_begin) * tile);
-            for (int c = (t - T.tile_begin) * tile + (int)threadIdx.x; c < c_end; c += BLOCK) {
-                const CellIdx id = decompose(c, T);
-                const long o = T.off + c;
-                const Nb nb = neighbours(id, T, o);
-                const long idx[7] = {o, nb.W, nb.E, nb.S, nb.N, nb.B, nb.T};
-                double xv[7];
+            for (int u = 0; u < nrun; ++u) {
+                const int base = (t + u * w.stride - T.tile_begin) * tile;
+                const int c_end = min(T.n, base + tile);
+                for (int c = base + (int)threadIdx.x; c < c_end; c += BLOCK) {
+                    const CellIdx id = decompose(c, T);
+                    const long o = T.off + c;
+                    const Nb nb = neighbours(id, T, o);
+                    const long idx[7] = {o, nb.W, nb.E, nb.S, nb.N, nb.B, nb.T};
+                    double xv[7];
 #pragma unroll
-                for (int q = 0; q < 7; ++q) xv[q] = first ? xo[idx[q]] : xo[idx[q]] + (double)d[idx[q]];
-                const double cP = aP[o], cW = aW[o], cE = aE[o], cS = aS[o], cN = aN[o], cB = aB[o], cT = aT[o];
-                const double su = bb[o], fo = fold[o];
-                const int slot = comp[o];
-                const double xc = xv[0];
-                const double ax = cP * xc - cW * (nb.hW ? xv[1] : 0.0) - cE * (nb.hE ? xv[2] : 0.0) -
-                                  cS * xv[3] - cN * xv[4] - cB * (nb.hB ? xv[5] : 0.0) - cT * (nb.hT ? xv[6] : 0.0);
-                const double res = su - (ax - fo);
-                xn[o] = xc;
-                r64[o] = res;
-                acc[0] += res * res;
-                acc[1] += fabs(res);
-                acc[2] += su * su;
-                acc[3] += fabs(su);
+                    for (int q = 0; q < 7; ++q) xv[q] = first ? xo[idx[q]] : xo[idx[q]] + (double)d[idx[q]];
+                    const double cP = aP[o], cW = aW[o], cE = aE[o], cS = aS[o], cN = aN[o], cB = aB[o], cT = aT[o];
+                    const double su = bb[o], fo = fold[o];
+                    const int slot = comp[o];
+                    const double xc = xv[0];
+                    const double ax = cP * xc - cW * (nb.hW ? xv[1] : 0.0) - cE * (nb.hE ? xv[2] : 0.0) -
+                                      cS * xv[3] - cN * xv[4] - cB * (nb.hB ? xv[5] : 0.0) - cT * (nb.hT ? xv[6] : 0.0);
+                    const double res = su - (ax - fo);
+                    xn[o] = xc;
+                    r64[o] = res;
+                    acc[0] += res * res;
+                    acc[1] += fabs(res);
+                    acc[2] += su * su;
+                    acc[3] += fabs(su);
 #pragma unroll
-                for (int q = 0; q < MAX_COMP; ++q) {
-                    acc[4 + q] += slot == q ? res : 0.0;
-                    acc[4 + MAX_COMP + q] += slot == q ? xc : 0.0;
+                    for (int q = 0; q < MAX_COMP; ++q) {
+                        acc[4 + q] += slot == q ? res : 0.0;
+                        acc[4 + MAX_COMP + q] += slot == q ? xc : 0.0;
+                    }
                 }
             }
             block_sum<NRES>(acc, sh);
-            store_run<NRES>(a.partR, t, run_end, acc);
+            store_run<NRES>(a.partR, t, nrun, w.stride, acc);
         }
-        t = run_end;
+        t += nrun * w.stride;
     }
 }
 
@@ -271,11 +277,11 @@ __global__ __launch_bounds__(BLOCK) void k_res(MxArgs a, int tile) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_mreplace(MxArgs a, int tile) {
     __shared__ double sh[NRES * (BLOCK / 64)];
-    const TileRange tr(a.ntiles);
-    for (int t = tr.t; t < tr.end;) {
+    const TileWalk w = my_tiles(a.ntiles);
+    for (int t = w.first; t < w.end;) {
         const TileDesc T = a.tiles[t];
         const int s = T.sys;
-        const int run_end = min(tr.end, T.tile_end);
+        const int nrun = w.run(t, T.tile_end);
         const MxStat P = a.stat[0][s];
         bool done = P.done != 0;
         bool zero_p = false;
@@ -319,13 +325,13 @@ __global__ __launch_bounds__(BLOCK) void k_mreplace(MxArgs a, int tile) {
                 D.broken = 0;
             }
             if (t == T.tile_begin && threadIdx.x == 0) {
-                a.stat[1][s] = N;
-                a.dyn[1][s] = D;
+                store_state(&a.stat[1][s], N);
+                store_state(&a.dyn[1][s], D);
             }
         } else if (t == T.tile_begin && threadIdx.x == 0) {
             const MxDyn D0 = a.dyn[0][s];
-            a.stat[1][s] = P;
-            a.dyn[1][s] = D0;
+            store_state(&a.stat[1][s], P);
+            store_state(&a.dyn[1][s], D0);
         }
         if (!done) {
             const double* __restrict__ r64 = a.r64;
@@ -336,28 +342,31 @@ __global__ __launch_bounds__(BLOCK) void k_mreplace(MxArgs a, int tile) {
             float* __restrict__ d = a.d;
             float* __restrict__ p0 = a.p[0];
             double acc[2] = {0.0, 0.0};
-            const int c_end = min(T.n, (run_end - T.tile_begin) * tile);
-            for (int c = (t - T.tile_begin) * tile + (int)threadIdx.x; c < c_end; c += BLOCK) {
-                const long o = T.off + c;
-                const int slot = comp[o];
-                const double rv = r64[o];
-                const float iv = dinv[o];
-                double mu = 0.0;
+            for (int u = 0; u < nrun; ++u) {
+                const int base = (t + u * w.stride - T.tile_begin) * tile;
+                const int c_end = min(T.n, base + tile);
+                for (int c = base + (int)threadIdx.x; c < c_end; c += BLOCK) {
+                    const long o = T.off + c;
+                    const int slot = comp[o];
+                    const double rv = r64[o];
+                    const float iv = dinv[o];
+                    double mu = 0.0;
 #pragma unroll
-                for (int q = 0; q < MAX_COMP; ++q) mu = slot == q ? mean_r[q] : mu;
-                const float rr = (float)(rv - mu);
-                const float zz = iv * rr;
-                r[o] = rr;
-                z[o] = zz;
-                d[o] = 0.0f;
-                if (zero_p) p0[o] = 0.0f;
-                acc[0] += (double)rr * (double)rr;
-                acc[1] += (double)rr * (double)zz;
+                    for (int q = 0; q < MAX_COMP; ++q) mu = slot == q ? mean_r[q] : mu;
+                    const float rr = (float)(rv - mu);
+                    const float zz = iv * rr;
+                    r[o] = rr;
+                    z[o] = zz;
+                    d[o] = 0.0f;
+                    if (zero_p) p0[o] = 0.0f;
+                    acc[0] += (double)rr * (double)rr;
+                    acc[1] += (double)rr * (double)zz;
+                }
             }
             block_sum<2>(acc, sh);
-            store_run<2>(a.partB, t, run_end, acc);
+            store_run<2>(a.partB, t, nrun, w.stride, acc);
         }
-        t = run_end;
+        t += nrun * w.stride;
     }
 }
 
@@ -373,11 +382,49 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv(MxArgs a, int par) {
     const float* __restrict__ pold = a.p[par];
     float* __restrict__ pnew = a.p[par ^ 1];
     const float* __restrict__ z = a.z;
-    const TileRange tr(a.ntiles);
-    for (int t = tr.t; t < tr.end;) {
+    const float* __restrict__ ew = a.e;
+    float* __restrict__ qo = a.q;
+    const TileWalk w = my_tiles(a.ntiles);
+    for (int t = w.first; t < w.end;) {
         const TileDesc T = a.tiles[t];
         const int s = T.sys;
-        const int run_end = min(tr.end, T.tile_end);
+        const int nrun = w.run(t, T.tile_end);
+        float zv[CPT][7], pv[CPT][7], wv[CPT][6], ev[CPT];
+        long oc[CPT];
+        bool ok[CPT], mW[CPT], mB[CPT];
+        auto load = [&](int tile) {
+            const int c0 = (tile - T.tile_begin) * TILE + (int)threadIdx.x;
+#pragma unroll
+            for (int m = 0; m < CPT; ++m) {
+                const int c = c0 + m * BLOCK;
+                ok[m] = c < T.n;
+                const int cc = clamp_cell(c, T.n);
+                const CellIdx id = decompose(cc, T);
+                const long o = T.off + cc;
+                const Nb nb = neighbours(id, T, o);
+                oc[m] = o;
+                mW[m] = nb.hW;
+                mB[m] = nb.hB;
+                const long idx[7] = {o, nb.W, nb.E, nb.S, nb.N, nb.B, nb.T};
+#pragma unroll
+                for (int q = 0; q < 7; ++q) {
+                    zv[m][q] = z[idx[q]];
+                    pv[m][q] = pold[idx[q]];
+                }
+                // own +faces; -faces are the neighbours' +faces.  A wall
+                // +face weight is stored as 0 and its neighbour index is the
+                // cell itself, so it adds 0.
+                wv[m][0] = W::load(a.w[0], a.scale, o);
+                wv[m][1] = W::load(a.w[0], a.scale, nb.W);
+                wv[m][2] = W::load(a.w[1], a.scale, o);
+                wv[m][3] = W::load(a.w[1], a.scale, nb.S);
+                wv[m][4] = W::load(a.w[2], a.scale, o);
+                wv[m][5] = W::load(a.w[2], a.scale, nb.B);
+                ev[m] = ew[o];
+            }
+        };
+        // the first batch of cell loads is in flight while the partials are reduced
+        load(t);
         const MxDyn P = a.dyn[1][s];
         MxDyn N = P;
         bool skip = P.done || P.broken;
@@ -393,47 +440,12 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv(MxArgs a, int par) {
                 N.fresh = 0;
             }
         }
-        if (t == T.tile_begin && threadIdx.x == 0) a.dyn[0][s] = N;
+        if (t == T.tile_begin && threadIdx.x == 0) store_state(&a.dyn[0][s], N);
         if (!skip) {
             const float fb = (float)N.beta;
-            const float* __restrict__ ew = a.e;
-            float* __restrict__ qo = a.q;
             double acc[1] = {0.0};
-            const int c_end = min(T.n, (run_end - T.tile_begin) * TILE);
-            for (int c0 = (t - T.tile_begin) * TILE + (int)threadIdx.x; c0 < c_end; c0 += TILE) {
-                // load phase: every operand of the CPT cells of this thread
-                float zv[CPT][7], pv[CPT][7], wv[CPT][6], ev[CPT];
-                long oc[CPT];
-                bool ok[CPT], mW[CPT], mB[CPT];
-#pragma unroll
-                for (int m = 0; m < CPT; ++m) {
-                    const int c = c0 + m * BLOCK;
-                    ok[m] = c < c_end;
-                    const int cc = ok[m] ? c : c0;
-                    const CellIdx id = decompose(cc, T);
-                    const long o = T.off + cc;
-                    const Nb nb = neighbours(id, T, o);
-                    oc[m] = o;
-                    mW[m] = nb.hW;
-                    mB[m] = nb.hB;
-                    const long idx[7] = {o, nb.W, nb.E, nb.S, nb.N, nb.B, nb.T};
-#pragma unroll
-                    for (int q = 0; q < 7; ++q) {
-                        zv[m][q] = z[idx[q]];
-                        pv[m][q] = pold[idx[q]];
-                    }
-                    // own +faces; -faces are the neighbours' +faces.  A wall
-                    // +face weight is stored as 0 and its neighbour index is
-                    // the cell itself, so it adds 0.
-                    wv[m][0] = W::load(a.w[0], a.scale, o);
-                    wv[m][1] = W::load(a.w[0], a.scale, nb.W);
-                    wv[m][2] = W::load(a.w[1], a.scale, o);
-                    wv[m][3] = W::load(a.w[1], a.scale, nb.S);
-                    wv[m][4] = W::load(a.w[2], a.scale, o);
-                    wv[m][5] = W::load(a.w[2], a.scale, nb.B);
-                    ev[m] = ew[o];
-                }
-                // compute phase
+            for (int u = 0; u < nrun; ++u) {
+                if (u) load(t + u * w.stride);
 #pragma unroll
                 for (int m = 0; m < CPT; ++m) {
                     float p[7];
@@ -455,9 +467,9 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv(MxArgs a, int par) {
                 }
             }
             block_sum<1>(acc, sh);
-            store_run<1>(a.partA, t, run_end, acc);
+            store_run<1>(a.partA, t, nrun, w.stride, acc);
         }
-        t = run_end;
+        t += nrun * w.stride;
     }
 }
 
@@ -470,11 +482,36 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate(MxArgs a, int par) {
     constexpr int TILE = BLOCK * CPT;
     __shared__ double sh[2 * (BLOCK / 64)];
     const float* __restrict__ pv = a.p[par ^ 1];
-    const TileRange tr(a.ntiles);
-    for (int t = tr.t; t < tr.end;) {
+    float* __restrict__ d = a.d;
+    float* __restrict__ r = a.r;
+    float* __restrict__ zo = a.z;
+    const float* __restrict__ q = a.q;
+    const float* __restrict__ dinv = a.dinv;
+    const TileWalk w = my_tiles(a.ntiles);
+    for (int t = w.first; t < w.end;) {
         const TileDesc T = a.tiles[t];
         const int s = T.sys;
-        const int run_end = min(tr.end, T.tile_end);
+        const int nrun = w.run(t, T.tile_end);
+        float dv[CPT], pvv[CPT], rv[CPT], qv[CPT], iv[CPT];
+        long oc[CPT];
+        bool ok[CPT];
+        auto load = [&](int tile) {
+            const int c0 = (tile - T.tile_begin) * TILE + (int)threadIdx.x;
+#pragma unroll
+            for (int m = 0; m < CPT; ++m) {
+                const int c = c0 + m * BLOCK;
+                ok[m] = c < T.n;
+                const long o = T.off + clamp_cell(c, T.n);
+                oc[m] = o;
+                dv[m] = d[o];
+                pvv[m] = pv[o];
+                rv[m] = r[o];
+                qv[m] = q[o];
+                iv[m] = dinv[o];
+            }
+        };
+        // the first batch of cell loads is in flight while the partials are reduced
+        load(t);
         const MxDyn P = a.dyn[0][s];
         MxDyn N = P;
         bool skip = P.done || P.broken;
@@ -489,32 +526,12 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate(MxArgs a, int par) {
                 N.total_inner = P.total_inner + 1;
             }
         }
-        if (t == T.tile_begin && threadIdx.x == 0) a.dyn[1][s] = N;
+        if (t == T.tile_begin && threadIdx.x == 0) store_state(&a.dyn[1][s], N);
         if (!skip) {
             const float fa = (float)N.alpha;
-            float* __restrict__ d = a.d;
-            float* __restrict__ r = a.r;
-            float* __restrict__ zo = a.z;
-            const float* __restrict__ q = a.q;
-            const float* __restrict__ dinv = a.dinv;
             double acc[2] = {0.0, 0.0};
-            const int c_end = min(T.n, (run_end - T.tile_begin) * TILE);
-            for (int c0 = (t - T.tile_begin) * TILE + (int)threadIdx.x; c0 < c_end; c0 += TILE) {
-                float dv[CPT], pvv[CPT], rv[CPT], qv[CPT], iv[CPT];
-                long oc[CPT];
-                bool ok[CPT];
-#pragma unroll
-                for (int m = 0; m < CPT; ++m) {
-                    const int c = c0 + m * BLOCK;
-                    ok[m] = c < c_end;
-                    const long o = T.off + (ok[m] ? c : c0);
-                    oc[m] = o;
-                    dv[m] = d[o];
-                    pvv[m] = pv[o];
-                    rv[m] = r[o];
-                    qv[m] = q[o];
-                    iv[m] = dinv[o];
-                }
+            for (int u = 0; u < nrun; ++u) {
+                if (u) load(t + u * w.stride);
 #pragma unroll
                 for (int m = 0; m < CPT; ++m) {
                     const float rr = rv[m] - fa * qv[m];
@@ -529,9 +546,9 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate(MxArgs a, int par) {
                 }
             }
             block_sum<2>(acc, sh);
-            store_run<2>(a.partB, t, run_end, acc);
+            store_run<2>(a.partB, t, nrun, w.stride, acc);
         }
-        t = run_end;
+        t += nrun * w.stride;
     }
 }
 
@@ -548,25 +565,28 @@ __global__ __launch_bounds__(BLOCK) void k_mcount(MxArgs a) {
 // Result: x - mean(x) on every singular component (the gauge fix).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_mproject(MxArgs a, int tile) {
-    const TileRange tr(a.ntiles);
-    for (int t = tr.t; t < tr.end;) {
+    const TileWalk w = my_tiles(a.ntiles);
+    for (int t = w.first; t < w.end;) {
         const TileDesc T = a.tiles[t];
-        const int run_end = min(tr.end, T.tile_end);
+        const int nrun = w.run(t, T.tile_end);
         const MxStat& S = a.stat[1][T.sys];
         const double* __restrict__ x = a.X[S.xbuf];
         double mean[MAX_COMP];
 #pragma unroll
         for (int q = 0; q < MAX_COMP; ++q) mean[q] = S.mean_x[q];
-        const int c_end = min(T.n, (run_end - T.tile_begin) * tile);
-        for (int c = (t - T.tile_begin) * tile + (int)threadIdx.x; c < c_end; c += BLOCK) {
-            const long o = T.off + c;
-            const int slot = a.comp[o];
-            double mu = 0.0;
+        for (int u = 0; u < nrun; ++u) {
+            const int base = (t + u * w.stride - T.tile_begin) * tile;
+            const int c_end = min(T.n, base + tile);
+            for (int c = base + (int)threadIdx.x; c < c_end; c += BLOCK) {
+                const long o = T.off + c;
+                const int slot = a.comp[o];
+                double mu = 0.0;
 #pragma unroll
-            for (int q = 0; q < MAX_COMP; ++q) mu = slot == q ? mean[q] : mu;
-            a.xout[o] = x[o] - mu;
+                for (int q = 0; q < MAX_COMP; ++q) mu = slot == q ? mean[q] : mu;
+                a.xout[o] = x[o] - mu;
+            }
         }
-        t = run_end;
+        t += nrun * w.stride;
     }
 }
 

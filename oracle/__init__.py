@@ -33,14 +33,16 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_OMP_PATH = os.path.join(_HERE, "liboracle_omp.so")
 _lib = None
+_omp = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
 
 def build(force: bool = False) -> str:
-    """Compile ``liboracle.so`` with the committed Makefile."""
-    if force or not os.path.exists(_LIB_PATH):
+    """Compile ``liboracle.so`` and ``liboracle_omp.so`` with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH) or not os.path.exists(_OMP_PATH):
         subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True,
                        stdout=subprocess.DEVNULL)
     return _LIB_PATH
@@ -66,6 +68,21 @@ def lib():
         L.oracle_true_residual.restype = None
         _lib = L
     return _lib
+
+
+def lib_omp():
+    """``liboracle_omp.so``: the same CG with OpenMP loops (``cg_omp.c``)."""
+    global _omp
+    if _omp is None:
+        if not os.path.exists(_OMP_PATH):
+            build()
+        L = ctypes.CDLL(_OMP_PATH)
+        i = ctypes.c_int
+        L.oracle_cg_omp.argtypes = [i, i, i] + [_dp] * 8 + [_dp, i, ctypes.c_double, _dp,
+                                                             ctypes.POINTER(ctypes.c_int)]
+        L.oracle_cg_omp.restype = ctypes.c_int
+        _omp = L
+    return _omp
 
 
 def _p(a: np.ndarray):
@@ -95,6 +112,20 @@ def cg(ps, phi=None, max_iter=100000, tol=1e-8, history=False):
     if history:
         hist = hist[: nit.value + 1]
     return phi, res.value, nit.value, hist
+
+
+def cg_omp(ps, phi=None, max_iter=100000, tol=1e-8):
+    """The multi-core CPU baseline: ``cg`` with OpenMP loops (threads from
+    ``OMP_NUM_THREADS``).  Returns ``(phi, residual, n_iter)``."""
+    nr, nth, nz = ps.shape
+    phi = np.zeros((nr + 4, nth + 4, nz + 4), order="F") if phi is None else np.array(phi, order="F", dtype=np.float64)
+    res = ctypes.c_double()
+    nit = ctypes.c_int()
+    rc = lib_omp().oracle_cg_omp(nr, nth, nz, *[_p(ps.padded(n)) for n in COEF], _p(ps.padded("Su")),
+                                 _p(phi), int(max_iter), float(tol), ctypes.byref(res), ctypes.byref(nit))
+    if rc != 0:
+        raise MemoryError("oracle_cg_omp could not allocate its work arrays")
+    return phi, res.value, nit.value
 
 
 def matvec(ps, x):

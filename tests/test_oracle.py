@@ -220,3 +220,20 @@ def test_planted_mistake_is_caught():
     xs = []
     sla.cg(A, b, rtol=1e-8, atol=0.0, M=sp.diags(d), maxiter=5, callback=lambda xk: xs.append(xk.copy()))
     assert np.linalg.norm(x - xs[4]) > 1e-3 * np.linalg.norm(xs[4])
+
+
+@pytest.mark.parametrize("config,shape", [(1, None), (2, (20, 36, 28)), (5, (24, 48, 30))])
+def test_openmp_baseline_matches_sequential_oracle(config, shape):
+    """The multi-core baseline (cg_omp.c) is the same CG with parallel dot
+    reductions: same iteration count within one, same solution, halos the
+    periodic copies."""
+    ps = systems.make_system(config, 0, shape)
+    phi_s, res_s, nit_s, _ = oracle.cg(ps, max_iter=20000, tol=1e-8)
+    phi_p, res_p, nit_p = oracle.cg_omp(ps, max_iter=20000, tol=1e-8)
+    assert abs(nit_p - nit_s) <= 1
+    assert res_p < 1e-8 and res_s < 1e-8
+    err = oracle.gauge_error(oracle.interior(phi_p), oracle.interior(phi_s), ps.labels, ps.ncomp)
+    assert err < 1e-6
+    nth = ps.shape[1]
+    np.testing.assert_array_equal(phi_p[:, 1, :], phi_p[:, nth + 1, :])   # j=0 <- j=nth
+    np.testing.assert_array_equal(phi_p[:, nth + 3, :], phi_p[:, 3, :])   # j=nth+2 <- j=2
