@@ -265,13 +265,19 @@ __host__ __device__ __forceinline__ int clamp_cell(int c, int n) { return c < n 
 // every block of the next kernel; measured on MI355X, an ordinary store of
 // it held the writing block back by ~13 us (a 604,800-cell update kernel
 // went from 9 us to 22 us), a nontemporal store costs nothing measurable.
+// The words are taken out of v with memcpy: reading the record through an
+// integer pointer would break strict aliasing, and the compiler then may
+// (and did) store words loaded before v's double fields were updated.
 template <typename R>
 __device__ __forceinline__ void store_state(R* dst, const R& v) {
     static_assert(sizeof(R) % 8 == 0, "state records are whole 8-byte words");
-    const unsigned long long* s = reinterpret_cast<const unsigned long long*>(&v);
     unsigned long long* d = reinterpret_cast<unsigned long long*>(dst);
 #pragma unroll
-    for (int i = 0; i < (int)(sizeof(R) / 8); ++i) __builtin_nontemporal_store(s[i], d + i);
+    for (int i = 0; i < (int)(sizeof(R) / 8); ++i) {
+        unsigned long long w;
+        __builtin_memcpy(&w, reinterpret_cast<const char*>(&v) + 8 * i, 8);
+        __builtin_nontemporal_store(w, d + i);
+    }
 }
 
 // Per-system scalar state lives in two slots.  Kernels come in two roles
