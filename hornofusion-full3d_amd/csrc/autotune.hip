@@ -3,8 +3,10 @@
 // The two inner-iteration kernels of a solver are timed (HIP events on the
 // library stream, averaged over `reps` iterations) for every candidate
 // shape: cells per thread in {1, 2, 4, 8} (the tile is 256 * cpt cells)
-// and blocks per CU in {4, 8, 16} (capped by the tile count).  The fastest
-// shape is kept in the batch; solvers pick it up at their next solve.
+// and blocks per CU in {4, 8, 16} (capped by the tile count) or one block
+// per tile.  For the mixed solver the column-marching stencil is then timed
+// at 1 .. 16 r-segments.  The fastest shape is kept in the batch; solvers
+// pick it up at their next solve.
 #include <cfloat>
 
 #include "batch.hpp"
@@ -23,13 +25,15 @@ int time_kernels(Batch& B, int solver, int fmt, int reps, float* ms_spmv, float*
 int autotune(Batch& B, int solver, int fmt, int reps, int* best_cpt, int* best_blocks, float* best_ms) {
     const int cus = device_cus();
     const int cpts[] = {1, 2, 4, 8};
-    const int per_cu[] = {4, 8, 16};
+    const int per_cu[] = {4, 8, 16, 0};
     float best = FLT_MAX;
     int bc = B.cpt, bb = B.grid;
+    EAFP_TRY_RC(B.set_columns(0));
     for (int cpt : cpts) {
         for (int k : per_cu) {
             EAFP_TRY_RC(B.set_tiling(cpt, 0));
-            const int blocks = std::min(B.ntiles, k * cus);
+            // k = 0: one block per tile (the hardware dispatcher walks the table)
+            const int blocks = k == 0 ? B.ntiles : std::min(B.ntiles, k * cus);
             EAFP_TRY_RC(B.set_tiling(cpt, blocks));
             float a = 0.f, b = 0.f;
             // one untimed pass settles caches and code, then the timed reps
@@ -42,7 +46,23 @@ int autotune(Batch& B, int solver, int fmt, int reps, int* best_cpt, int* best_b
             }
         }
     }
+    EAFP_TRY_RC(B.set_columns(0));
     EAFP_TRY_RC(B.set_tiling(bc, bb));
+    if (solver == 1) {
+        // the column-marching stencil, where every system qualifies
+        int bs = 0;
+        for (int k : {1, 2, 4, 8, 16}) {
+            if (B.set_columns(k) != EAFP_OK) continue;
+            float a = 0.f, b = 0.f;
+            EAFP_TRY_RC(time_kernels(B, solver, fmt, 2, &a, &b));
+            EAFP_TRY_RC(time_kernels(B, solver, fmt, reps, &a, &b));
+            if (a + b < best) {
+                best = a + b;
+                bs = k;
+            }
+        }
+        EAFP_TRY_RC(B.set_columns(bs));
+    }
     if (best_cpt) *best_cpt = bc;
     if (best_blocks) *best_blocks = bb;
     if (best_ms) *best_ms = best;

@@ -57,6 +57,30 @@ int Batch::set_tiling(int cells_per_thread, int blocks) {
         grid = choose_grid(ntiles);
     }
     ++tiling_gen;
+    // the column table refers to the tile ranges: rebuild it (or drop it
+    // when the new tiles leave a system fewer slots than segments)
+    if (col_split > 0 && set_columns(col_split) != EAFP_OK) col_split = 0;
+    return EAFP_OK;
+}
+
+// Column-marching stencil for the mixed solver: r_splits > 0 cuts every
+// column into that many r-segments; 0 returns to the tile kernels.
+// EAFP_ERR_UNSUPPORTED when a system does not qualify (nz not a multiple
+// of 4 or above COL_MAX_NZ, or fewer tiles than segments).
+int Batch::set_columns(int r_splits) {
+    if (r_splits < 0) return EAFP_ERR_ARG;
+    if (r_splits == 0) {
+        col_split = 0;
+        ++tiling_gen;
+        return EAFP_OK;
+    }
+    std::vector<ColDesc> t;
+    if (!make_col_table(geom, r_splits, t)) return EAFP_ERR_UNSUPPORTED;
+    cols_h = t;
+    EAFP_TRY_RC(cols.alloc(cols_h.size()));
+    EAFP_TRY(hipMemcpy(cols.p, cols_h.data(), sizeof(ColDesc) * cols_h.size(), hipMemcpyHostToDevice));
+    col_split = r_splits;
+    ++tiling_gen;
     return EAFP_OK;
 }
 
@@ -82,6 +106,8 @@ int Batch::init(int n, const eafp_desc* descs) {
         geom[s].nz = descs[s].nz;
     }
     ncells = layout_systems(geom);
+    quad = true;
+    for (const SysGeom& g : geom) quad = quad && (g.nz % 4 == 0);
     EAFP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     EAFP_TRY_RC(geom_d.alloc(n));
     EAFP_TRY_RC(set_tiling(2, 0));

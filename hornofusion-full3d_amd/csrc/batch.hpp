@@ -77,7 +77,14 @@ struct Batch {
     long ncells = 0;
     int ntiles = 0;
     int cpt = 2;                         // cells per thread; tile = BLOCK*cpt cells
-    int tiling_gen = 0;                  // bumped whenever tiles or grid change
+    bool quad = false;                   // every system's nz is a multiple of 4: the
+                                         // iteration kernels may take 4 cells of a z-row
+                                         // per thread (cpt 4 and 8)
+    int tiling_gen = 0;                  // bumped whenever tiles, grid or columns change
+    int col_split = 0;                   // > 0: the mixed stencil marches columns, each cut
+                                         // into this many r-segments (0: tile kernels)
+    std::vector<ColDesc> cols_h;
+    DevBuf<ColDesc> cols;
     std::vector<TileDesc> tiles_h;
     DevBuf<TileDesc> tiles;
     DevBuf<double> coef[NCOEF];          // FP64 operand arenas, compact
@@ -97,6 +104,7 @@ struct Batch {
     ~Batch();
     int init(int n, const eafp_desc* descs);
     int set_tiling(int cells_per_thread, int blocks);
+    int set_columns(int r_splits);
     int upload(int s, const double* const host[8], const double* phi0, int arrays_on_device);
     int download(int s, double* phi_host, int to_device);
     long padded_len(int s) const {

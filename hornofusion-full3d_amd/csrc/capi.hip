@@ -112,6 +112,17 @@ int eafp_batch_set_tiling(eafp_batch* b, int cells_per_thread, int blocks) {
     return b->B.set_tiling(cells_per_thread, blocks);
 }
 
+int eafp_batch_set_columns(eafp_batch* b, int r_splits) {
+    if (!b) return EAFP_ERR_ARG;
+    return b->B.set_columns(r_splits);
+}
+
+int eafp_batch_get_columns(const eafp_batch* b, int* r_splits) {
+    if (!b || !r_splits) return EAFP_ERR_ARG;
+    *r_splits = b->B.col_split;
+    return EAFP_OK;
+}
+
 int eafp_batch_time_kernels(eafp_batch* b, int solver, int coef_format, int reps, float* ms_spmv,
                             float* ms_update) {
     if (!b || reps < 1) return EAFP_ERR_ARG;

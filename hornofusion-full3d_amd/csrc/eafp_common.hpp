@@ -96,6 +96,66 @@ inline std::vector<TileDesc> make_tile_table(std::vector<SysGeom>& g, int tile) 
     return tiles;
 }
 
+// ---------------------------------------------------------------------------
+// Column segments, the work unit of the column-marching stencil kernels
+// (systems whose nz is a multiple of 4 and at most COL_MAX_NZ).  A column
+// is a slab of nj theta-rows of one system (nj*nz <= 4*BLOCK cells, one quad
+// of 4 z-cells per thread), marched through the r-planes i0 .. i1-1 of its
+// segment.  The segments of a system are numbered 0 .. nseg-1; segment u
+// writes its partial sum at tile slot tile_begin + u and zeros at the slots
+// tile_begin + u + v*nseg (v >= 1) of the system, so the system's partials
+// still cover its whole tile range exactly once.
+// ---------------------------------------------------------------------------
+constexpr int COL_MAX_NZ = 256;
+constexpr int COL_CELLS = 4 * BLOCK;   // cells of one plane of a column, at most
+
+struct ColDesc {
+    int tile;      // the system's first tile (its TileDesc carries the geometry); -1: empty
+    int j0, nj;    // theta rows j0 .. j0+nj-1
+    int i0, i1;    // r planes of the segment
+    int seg;       // index among the system's segments (0 writes the system's state)
+    int nseg;      // segments of the system
+    int pad;
+};
+
+// Rows per column for a given nz (every thread holds one quad of a row).
+__host__ __device__ __forceinline__ int col_rows(int nz) { return COL_CELLS / nz; }
+
+// Builds the column table: every system cut into ceil(nth/rows) columns and
+// each column into `splits` r-segments (fewer when nr is smaller).  Padded
+// with empty entries to a multiple of NXCD.  Returns false (and no table)
+// when a system does not qualify: nz not a multiple of 4 or above
+// COL_MAX_NZ, or fewer tiles than segments (the partial slots).
+inline bool make_col_table(const std::vector<SysGeom>& g, int splits, std::vector<ColDesc>& out) {
+    out.clear();
+    for (const SysGeom& G : g) {
+        if (G.nz % 4 != 0 || G.nz > COL_MAX_NZ) return false;
+        const int rows = col_rows(G.nz);
+        const int ncol = (G.nth + rows - 1) / rows;
+        const int nsp = splits < G.nr ? splits : G.nr;
+        const int nseg = ncol * nsp;
+        if (G.tile_end - G.tile_begin < nseg) return false;
+        for (int c = 0; c < ncol; ++c)
+            for (int sp = 0; sp < nsp; ++sp) {
+                ColDesc d{};
+                d.tile = G.tile_begin;
+                d.j0 = c * rows;
+                d.nj = (G.nth - d.j0) < rows ? (G.nth - d.j0) : rows;
+                d.i0 = (int)(((long)G.nr * sp) / nsp);
+                d.i1 = (int)(((long)G.nr * (sp + 1)) / nsp);
+                d.seg = c * nsp + sp;
+                d.nseg = nseg;
+                out.push_back(d);
+            }
+    }
+    while (out.size() % NXCD) {
+        ColDesc e{};
+        e.tile = -1;
+        out.push_back(e);
+    }
+    return true;
+}
+
 // Quotient of u by d for u < 2^24 using a float reciprocal and a one-step
 // integer correction (exact for every u < 2^24, d >= 1).
 __host__ __device__ __forceinline__ int fdiv(int u, int d, float inv_d) {
@@ -135,6 +195,70 @@ __host__ __device__ __forceinline__ Nb neighbours(const CellIdx& id, const G& g,
     n.B = n.hB ? o - 1 : o;
     n.T = n.hT ? o + 1 : o;
     return n;
+}
+
+// A quad: 4 consecutive cells (k0 .. k0+3) of one z-row, for systems whose
+// nz is a multiple of 4 (k0 is then one too).  Its W/E/S/N neighbours are
+// the quads at the same k0 in the adjacent rows -- 16-byte aligned like the
+// quad itself, since every system starts at a 32-cell boundary -- B is the
+// cell below the first and T the cell above the last.  As in neighbours(),
+// a wall neighbour is replaced by the quad's own first cell and flagged.
+struct Nb4 {
+    long W, E, S, N, B, T;
+    bool hW, hE, hB, hT;
+};
+
+template <typename G>
+__host__ __device__ __forceinline__ Nb4 neighbours4(const CellIdx& id, const G& g, long o) {
+    Nb4 n;
+    const long sr = (long)g.nth * g.nz;
+    n.hW = id.i > 0;
+    n.hE = id.i < g.nr - 1;
+    n.hB = id.k > 0;
+    n.hT = id.k + 4 < g.nz;
+    n.W = n.hW ? o - sr : o;
+    n.E = n.hE ? o + sr : o;
+    n.S = id.j > 0 ? o - g.nz : o + (long)(g.nth - 1) * g.nz;
+    n.N = id.j < g.nth - 1 ? o + g.nz : o - (long)(g.nth - 1) * g.nz;
+    n.B = n.hB ? o - 1 : o;
+    n.T = n.hT ? o + 4 : o;
+    return n;
+}
+
+// Four floats moved as one 16-byte access (p 16-byte aligned).
+struct F4 {
+    float v[4];
+};
+
+__device__ __forceinline__ F4 ld4(const float* p) {
+    F4 r;
+    __builtin_memcpy(&r, __builtin_assume_aligned(p, 16), 16);
+    return r;
+}
+
+__device__ __forceinline__ void st4(float* p, const F4& r) {
+    __builtin_memcpy(__builtin_assume_aligned(p, 16), &r, 16);
+}
+
+// The same with the nontemporal hint, for streams that the kernel touches
+// once: measured on MI355X, the update kernel's five loads and two of its
+// stores went from 5.4 to 5.8 TB/s with it.  A kernel that uses these on
+// an array accesses that array through no other type.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ F4 ld4nt(const float* p) {
+    const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
+    F4 r;
+    r.v[0] = v.x;
+    r.v[1] = v.y;
+    r.v[2] = v.z;
+    r.v[3] = v.w;
+    return r;
+}
+
+__device__ __forceinline__ void st4nt(float* p, const F4& r) {
+    const f4v v = {r.v[0], r.v[1], r.v[2], r.v[3]};
+    __builtin_nontemporal_store(v, reinterpret_cast<f4v*>(p));
 }
 
 template <typename G>

@@ -73,6 +73,7 @@ struct MxArgs {
     const TileDesc* tiles;
     int ntiles;
     int nsys;
+    const ColDesc* cols;   // column segments (column-marching stencil)
     // FP64 operand
     const double *aW, *aE, *aS, *aN, *aB, *aT, *aP, *b, *fold, *x0;
     const unsigned char* comp;
@@ -112,6 +113,7 @@ template <>
 struct WFmt<EAFP_COEF_FP32> {
     using T = float;
     __device__ static float load(const void* w, const float*, long o) { return ((const float*)w)[o]; }
+    __device__ static F4 load4(const void* w, const float*, long o) { return ld4((const float*)w + o); }
     __device__ static T enc(float v, float) { return v; }
     __device__ static float dec(T h, float) { return h; }
 };
@@ -122,6 +124,14 @@ struct WFmt<EAFP_COEF_BF16> {
     __device__ static float load(const void* w, const float*, long o) {
         return __uint_as_float((unsigned)((const unsigned short*)w)[o] << 16);
     }
+    __device__ static F4 load4(const void* w, const float*, long o) {
+        unsigned short h[4];
+        __builtin_memcpy(h, __builtin_assume_aligned((const unsigned short*)w + o, 8), 8);
+        F4 r;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r.v[q] = __uint_as_float((unsigned)h[q] << 16);
+        return r;
+    }
     __device__ static T enc(float v, float) { return f32_to_bf16_rne(v); }
     __device__ static float dec(T h, float) { return __uint_as_float((unsigned)h << 16); }
 };
@@ -131,6 +141,14 @@ struct WFmt<EAFP_COEF_FP16> {
     using T = _Float16;
     __device__ static float load(const void* w, const float* sc, long o) {
         return (float)((const _Float16*)w)[o] * sc[o];
+    }
+    __device__ static F4 load4(const void* w, const float* sc, long o) {
+        _Float16 h[4];
+        __builtin_memcpy(h, __builtin_assume_aligned((const _Float16*)w + o, 8), 8);
+        F4 r = ld4(sc + o);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r.v[q] *= (float)h[q];
+        return r;
     }
     __device__ static T enc(float v, float s) { return (_Float16)(v / s); }
     __device__ static float dec(T h, float s) { return (float)h * s; }
@@ -552,6 +570,386 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate(MxArgs a, int par) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Quad forms of the two inner-iteration kernels, for batches in which every
+// nz is a multiple of 4.  A thread takes Q quads -- 4 consecutive cells of
+// one z-row -- per tile, so every operand of a quad arrives in one 16-byte
+// access (8 bytes for 2-byte weights) instead of four, and p = z + beta*p
+// is formed once per loaded value.  The arithmetic per cell is that of
+// k_mspmv / k_mupdate, term for term; only the order of the partial sums
+// differs.  The tile table is the same (a tile is BLOCK*4*Q cells).  The
+// update streams its operands with nontemporal loads and d, r with
+// nontemporal stores; z, which the next stencil reads, is stored plainly.
+// ---------------------------------------------------------------------------
+template <int F, int Q>
+__global__ __launch_bounds__(BLOCK) void k_mspmv4(MxArgs a, int par) {
+    using W = WFmt<F>;
+    constexpr int TILE = BLOCK * 4 * Q;
+    __shared__ double sh[2 * (BLOCK / 64)];
+    const float* __restrict__ pold = a.p[par];
+    float* __restrict__ pnew = a.p[par ^ 1];
+    const float* __restrict__ z = a.z;
+    const float* __restrict__ ew = a.e;
+    float* __restrict__ qo = a.q;
+    const TileWalk w = my_tiles(a.ntiles);
+    for (int t = w.first; t < w.end;) {
+        const TileDesc T = a.tiles[t];
+        const int s = T.sys;
+        const int nrun = w.run(t, T.tile_end);
+        // per quad: z and p_old at the quad and its W/E/S/N quads, at the
+        // cells below and above, and the weights of its faces
+        F4 zq[Q][5], pq[Q][5], wq[Q][5], ev[Q];
+        float zb[Q][2], pb[Q][2], wB[Q];
+        long oc[Q];
+        bool ok[Q], mW[Q], mB[Q];
+        auto load = [&](int tile) {
+            const int c0 = (tile - T.tile_begin) * TILE + 4 * (int)threadIdx.x;
+#pragma unroll
+            for (int m = 0; m < Q; ++m) {
+                const int c = c0 + 4 * m * BLOCK;
+                ok[m] = c < T.n;
+                const int cc = ok[m] ? c : T.n - 4;
+                const CellIdx id = decompose(cc, T);
+                const long o = T.off + cc;
+                const Nb4 nb = neighbours4(id, T, o);
+                oc[m] = o;
+                mW[m] = nb.hW;
+                mB[m] = nb.hB;
+                const long idx[5] = {o, nb.W, nb.E, nb.S, nb.N};
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    zq[m][v] = ld4(z + idx[v]);
+                    pq[m][v] = ld4(pold + idx[v]);
+                }
+                zb[m][0] = z[nb.B];
+                pb[m][0] = pold[nb.B];
+                zb[m][1] = z[nb.T];
+                pb[m][1] = pold[nb.T];
+                wq[m][0] = W::load4(a.w[0], a.scale, o);      // +r faces
+                wq[m][1] = W::load4(a.w[0], a.scale, nb.W);   // -r faces (the W quad's +r)
+                wq[m][2] = W::load4(a.w[1], a.scale, o);      // +theta faces
+                wq[m][3] = W::load4(a.w[1], a.scale, nb.S);   // -theta faces
+                wq[m][4] = W::load4(a.w[2], a.scale, o);      // +z faces
+                wB[m] = W::load(a.w[2], a.scale, nb.B);       // -z face of the first cell
+                ev[m] = ld4(ew + o);
+            }
+        };
+        load(t);
+        const MxDyn P = a.dyn[1][s];
+        MxDyn N = P;
+        bool skip = P.done || P.broken;
+        if (!skip) {
+            double tot[2];
+            reduce_tiles<2>(a.partB, T.tile_begin, T.tile_end, tot, sh);
+            if (!(tot[1] > 0.0)) {
+                skip = true;
+                N.broken = 1;
+            } else {
+                N.beta = P.fresh ? 0.0 : tot[1] / P.rz;
+                N.rz = tot[1];
+                N.fresh = 0;
+            }
+        }
+        if (t == T.tile_begin && threadIdx.x == 0) store_state(&a.dyn[0][s], N);
+        if (!skip) {
+            const float fb = (float)N.beta;
+            double acc[1] = {0.0};
+            for (int u = 0; u < nrun; ++u) {
+                if (u) load(t + u * w.stride);
+#pragma unroll
+                for (int m = 0; m < Q; ++m) {
+                    float p[5][4];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) p[v][q] = zq[m][v].v[q] + fb * pq[m][v].v[q];
+                    const float pB0 = zb[m][0] + fb * pb[m][0];
+                    const float pT3 = zb[m][1] + fb * pb[m][1];
+                    F4 y;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float pc = p[0][q];
+                        const float pT = q == 3 ? pT3 : p[0][q + 1];
+                        const float pB = q == 0 ? pB0 : p[0][q - 1];
+                        const float wb = q == 0 ? (mB[m] ? wB[m] : 0.f) : wq[m][4].v[q - 1];
+                        float yy = ev[m].v[q] * pc;
+                        yy += wq[m][0].v[q] * (pc - p[2][q]);
+                        yy += (mW[m] ? wq[m][1].v[q] : 0.f) * (pc - p[1][q]);
+                        yy += wq[m][2].v[q] * (pc - p[4][q]);
+                        yy += wq[m][3].v[q] * (pc - p[3][q]);
+                        yy += wq[m][4].v[q] * (pc - pT);
+                        yy += wb * (pc - pB);
+                        y.v[q] = yy;
+                    }
+                    if (ok[m]) {
+                        F4 pn;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) pn.v[q] = p[0][q];
+                        st4(pnew + oc[m], pn);
+                        st4(qo + oc[m], y);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) acc[0] += (double)p[0][q] * (double)y.v[q];
+                    }
+                }
+            }
+            block_sum<1>(acc, sh);
+            store_run<1>(a.partA, t, nrun, w.stride, acc);
+        }
+        t += nrun * w.stride;
+    }
+}
+
+template <int Q>
+__global__ __launch_bounds__(BLOCK) void k_mupdate4(MxArgs a, int par) {
+    constexpr int TILE = BLOCK * 4 * Q;
+    __shared__ double sh[2 * (BLOCK / 64)];
+    const float* __restrict__ pv = a.p[par ^ 1];
+    float* __restrict__ d = a.d;
+    float* __restrict__ r = a.r;
+    float* __restrict__ zo = a.z;
+    const float* __restrict__ q = a.q;
+    const float* __restrict__ dinv = a.dinv;
+    const TileWalk w = my_tiles(a.ntiles);
+    for (int t = w.first; t < w.end;) {
+        const TileDesc T = a.tiles[t];
+        const int s = T.sys;
+        const int nrun = w.run(t, T.tile_end);
+        F4 dv[Q], pvv[Q], rv[Q], qv[Q], iv[Q];
+        long oc[Q];
+        bool ok[Q];
+        auto load = [&](int tile) {
+            const int c0 = (tile - T.tile_begin) * TILE + 4 * (int)threadIdx.x;
+#pragma unroll
+            for (int m = 0; m < Q; ++m) {
+                const int c = c0 + 4 * m * BLOCK;
+                ok[m] = c < T.n;
+                const long o = T.off + (ok[m] ? c : T.n - 4);
+                oc[m] = o;
+                dv[m] = ld4nt(d + o);
+                pvv[m] = ld4nt(pv + o);
+                rv[m] = ld4nt(r + o);
+                qv[m] = ld4nt(q + o);
+                iv[m] = ld4nt(dinv + o);
+            }
+        };
+        load(t);
+        const MxDyn P = a.dyn[0][s];
+        MxDyn N = P;
+        bool skip = P.done || P.broken;
+        if (!skip) {
+            double tot[1];
+            reduce_tiles<1>(a.partA, T.tile_begin, T.tile_end, tot, sh);
+            if (!(fabs(tot[0]) > SMALL)) {
+                skip = true;
+                N.broken = 1;
+            } else {
+                N.alpha = P.rz / tot[0];
+                N.total_inner = P.total_inner + 1;
+            }
+        }
+        if (t == T.tile_begin && threadIdx.x == 0) store_state(&a.dyn[1][s], N);
+        if (!skip) {
+            const float fa = (float)N.alpha;
+            double acc[2] = {0.0, 0.0};
+            for (int u = 0; u < nrun; ++u) {
+                if (u) load(t + u * w.stride);
+#pragma unroll
+                for (int m = 0; m < Q; ++m) {
+                    F4 dn, rn, zn;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const float rr = rv[m].v[k] - fa * qv[m].v[k];
+                        rn.v[k] = rr;
+                        zn.v[k] = iv[m].v[k] * rr;
+                        dn.v[k] = dv[m].v[k] + fa * pvv[m].v[k];
+                    }
+                    if (ok[m]) {
+                        st4nt(d + oc[m], dn);
+                        st4nt(r + oc[m], rn);
+                        st4(zo + oc[m], zn);
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            acc[0] += (double)rn.v[k] * (double)rn.v[k];
+                            acc[1] += (double)rn.v[k] * (double)zn.v[k];
+                        }
+                    }
+                }
+            }
+            block_sum<2>(acc, sh);
+            store_run<2>(a.partB, t, nrun, w.stride, acc);
+        }
+        t += nrun * w.stride;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column-marching form of k_mspmv (same state logic, same arithmetic per
+// cell).  A block takes one column segment (ColDesc): nj theta-rows of one
+// system over the r-planes i0 .. i1-1; each thread holds one quad (4
+// z-cells of a row).  p = z + beta*p_old of the planes i-1, i and i+1 stays
+// in registers, so the r-neighbours are loaded once, as the plane itself;
+// p and the +theta weight of plane i go to a double-buffered LDS slab (with
+// the two theta halo rows, loaded by the first 2*nz/4 threads), from which
+// the theta and z neighbours are read: one barrier per plane.  Per cell it
+// reads z, p_old, the 3 weights and e once, plus the halo rows (z and p_old
+// of 2 rows, w1 of one, per nj rows), and writes p and q.
+// ---------------------------------------------------------------------------
+template <int F>
+__global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
+    using W = WFmt<F>;
+    constexpr int SP = COL_CELLS + 2 * COL_MAX_NZ;   // p slab: rows -1 .. nj
+    constexpr int SW = COL_CELLS + COL_MAX_NZ;       // w1 slab: rows -1 .. nj-1
+    __shared__ double sh[2 * (BLOCK / 64)];
+    __shared__ float Sp[2][SP];
+    __shared__ float Sw1[2][SW];
+    __shared__ float Sw2[2][BLOCK];
+    const int G = gridDim.x;
+    const ColDesc C = a.cols[(blockIdx.x % NXCD) * (G / NXCD) + blockIdx.x / NXCD];
+    if (C.tile < 0) return;
+    const TileDesc T = a.tiles[C.tile];
+    const int s = T.sys;
+    const float* __restrict__ pold = a.p[par];
+    float* __restrict__ pnew = a.p[par ^ 1];
+    const float* __restrict__ z = a.z;
+    const float* __restrict__ ew = a.e;
+    float* __restrict__ qo = a.q;
+    const int nz = T.nz, nth = T.nth, nr = T.nr;
+    const int Qr = nz >> 2;                       // quads per row
+    const int tid = threadIdx.x;
+    const int row = tid / Qr, kq = tid - row * Qr;
+    const bool act = row < C.nj;
+    // a lane past the slab shadows row 0: valid addresses, results unused
+    const long qoff = (long)(C.j0 + (act ? row : 0)) * nz + 4 * (act ? kq : 0);
+    const bool hal = tid < 2 * Qr;                // halo lanes: row -1, then row nj
+    const int hrow = tid < Qr ? -1 : C.nj;
+    const int hk = hal ? 4 * (tid < Qr ? tid : tid - Qr) : 0;
+    const long hoff = (long)(((C.j0 + hrow) % nth + nth) % nth) * nz + hk;
+    const long sr = (long)nth * nz;
+    auto base = [&](int i) { return T.off + (long)(i < nr ? i : nr - 1) * sr; };
+    const int i0 = C.i0, i1 = C.i1;
+
+    // planes i0-1 (W), i0 and i0+1 (E) of z and p_old, the weights of i0,
+    // the halos of i0: in flight while the partials are reduced
+    F4 zW = ld4(z + base(i0 > 0 ? i0 - 1 : i0) + qoff), oW = ld4(pold + base(i0 > 0 ? i0 - 1 : i0) + qoff);
+    F4 w0W = W::load4(a.w[0], a.scale, base(i0 > 0 ? i0 - 1 : i0) + qoff);
+    F4 zC = ld4(z + base(i0) + qoff), oC = ld4(pold + base(i0) + qoff);
+    F4 zE = ld4(z + base(i0 + 1) + qoff), oE = ld4(pold + base(i0 + 1) + qoff);
+    F4 w0 = W::load4(a.w[0], a.scale, base(i0) + qoff);
+    F4 w1 = W::load4(a.w[1], a.scale, base(i0) + qoff);
+    F4 w2 = W::load4(a.w[2], a.scale, base(i0) + qoff);
+    F4 ev = ld4(ew + base(i0) + qoff);
+    F4 zH = ld4(z + base(i0) + hoff), oH = ld4(pold + base(i0) + hoff);
+    F4 w1H = W::load4(a.w[1], a.scale, base(i0) + hoff);
+
+    const MxDyn P = a.dyn[1][s];
+    MxDyn N = P;
+    bool skip = P.done || P.broken;
+    if (!skip) {
+        double tot[2];
+        reduce_tiles<2>(a.partB, T.tile_begin, T.tile_end, tot, sh);
+        if (!(tot[1] > 0.0)) {
+            skip = true;
+            N.broken = 1;
+        } else {
+            N.beta = P.fresh ? 0.0 : tot[1] / P.rz;
+            N.rz = tot[1];
+            N.fresh = 0;
+        }
+    }
+    if (C.seg == 0 && tid == 0) store_state(&a.dyn[0][s], N);
+    if (skip) return;
+    const float fb = (float)N.beta;
+    auto pv = [&](const F4& zz, const F4& oo) {
+        F4 r;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r.v[q] = zz.v[q] + fb * oo.v[q];
+        return r;
+    };
+    F4 pW = pv(zW, oW), pC = pv(zC, oC), pE = pv(zE, oE), pH = pv(zH, oH);
+    double acc[1] = {0.0};
+    for (int i = i0; i < i1; ++i) {
+        const int b = i & 1;
+        // loads for the next plane: z/p_old of i+2 (its E), weights and halos of i+1
+        const bool more = i + 1 < i1;
+        F4 zN2, oN2, w0n, w1n, w2n, evn, zHn, oHn, w1Hn;
+        if (more) {
+            zN2 = ld4(z + base(i + 2) + qoff);
+            oN2 = ld4(pold + base(i + 2) + qoff);
+            w0n = W::load4(a.w[0], a.scale, base(i + 1) + qoff);
+            w1n = W::load4(a.w[1], a.scale, base(i + 1) + qoff);
+            w2n = W::load4(a.w[2], a.scale, base(i + 1) + qoff);
+            evn = ld4(ew + base(i + 1) + qoff);
+            zHn = ld4(z + base(i + 1) + hoff);
+            oHn = ld4(pold + base(i + 1) + hoff);
+            w1Hn = W::load4(a.w[1], a.scale, base(i + 1) + hoff);
+        }
+        // plane i into the slab: p (rows -1 .. nj), w1 (rows -1 .. nj-1), w2 of each quad's last cell
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                Sp[b][(row + 1) * nz + 4 * kq + q] = pC.v[q];
+                Sw1[b][(row + 1) * nz + 4 * kq + q] = w1.v[q];
+            }
+        }
+        if (hal) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Sp[b][(hrow + 1) * nz + hk + q] = pH.v[q];
+            if (hrow < 0) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) Sw1[b][hk + q] = w1H.v[q];
+            }
+        }
+        Sw2[b][tid] = w2.v[3];
+        __syncthreads();
+        if (act) {
+            const float* sp = &Sp[b][(row + 1) * nz + 4 * kq];
+            const float* sw = &Sw1[b][row * nz + 4 * kq];
+            const bool hW = i > 0;
+            const float pB0 = kq > 0 ? sp[-1] : pC.v[0];
+            const float pT3 = kq < Qr - 1 ? sp[4] : pC.v[3];
+            const float wB0 = kq > 0 ? Sw2[b][tid - 1] : 0.f;
+            F4 y;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float pc = pC.v[q];
+                const float pS = sp[q - nz], pN = sp[q + nz];
+                const float pT = q == 3 ? pT3 : pC.v[q + 1];
+                const float pB = q == 0 ? pB0 : pC.v[q - 1];
+                const float wb = q == 0 ? wB0 : w2.v[q - 1];
+                float yy = ev.v[q] * pc;
+                yy += w0.v[q] * (pc - pE.v[q]);
+                yy += (hW ? w0W.v[q] : 0.f) * (pc - pW.v[q]);
+                yy += w1.v[q] * (pc - pN);
+                yy += sw[q] * (pc - pS);
+                yy += w2.v[q] * (pc - pT);
+                yy += wb * (pc - pB);
+                y.v[q] = yy;
+            }
+            const long o = base(i) + qoff;
+            st4(pnew + o, pC);
+            st4(qo + o, y);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[0] += (double)pC.v[q] * (double)y.v[q];
+        }
+        if (more) {
+            pW = pC;
+            pC = pE;
+            pE = pv(zN2, oN2);
+            w0W = w0;
+            w0 = w0n;
+            w1 = w1n;
+            w2 = w2n;
+            ev = evn;
+            pH = pv(zHn, oHn);
+            w1H = w1Hn;
+        }
+    }
+    block_sum<1>(acc, sh);
+    if (tid == 0) a.partA[(long)T.tile_begin + C.seg] = acc[0];
+    for (long t = (long)T.tile_begin + C.seg + (long)(1 + tid) * C.nseg; t < T.tile_end; t += (long)BLOCK * C.nseg)
+        a.partA[t] = 0.0;
+}
+
 // Finished systems in slot 1 (one block).
 __global__ __launch_bounds__(BLOCK) void k_mcount(MxArgs a) {
     __shared__ double sh[BLOCK / 64];
@@ -665,6 +1063,7 @@ static int ensure_mixed(Batch& B, int fmt) {
     a.tiles = B.tiles.p;
     a.ntiles = B.ntiles;
     a.nsys = B.nsys;
+    a.cols = B.cols.p;
     a.aW = B.coef[C_AW].p;
     a.aE = B.coef[C_AE].p;
     a.aS = B.coef[C_AS].p;
@@ -713,6 +1112,15 @@ static void launch_mupdate_t(Batch& B, int par) {
 
 template <int F>
 static void launch_mspmv_f(Batch& B, int par) {
+    if (B.col_split > 0) {
+        k_mspmv_col<F><<<(int)B.cols_h.size(), BLOCK, 0, B.stream>>>(B.mixed->args, par);
+        return;
+    }
+    if (B.quad && B.cpt >= 4) {
+        if (B.cpt == 8) k_mspmv4<F, 2><<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, par);
+        else k_mspmv4<F, 1><<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, par);
+        return;
+    }
     switch (B.cpt) {
         case 1: launch_mspmv_t<F, 1>(B, par); break;
         case 4: launch_mspmv_t<F, 4>(B, par); break;
@@ -728,6 +1136,11 @@ static void launch_mspmv(Batch& B, int fmt, int par) {
 }
 
 static void launch_mupdate(Batch& B, int par) {
+    if (B.quad && B.cpt >= 4) {
+        if (B.cpt == 8) k_mupdate4<2><<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, par);
+        else k_mupdate4<1><<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, par);
+        return;
+    }
     switch (B.cpt) {
         case 1: launch_mupdate_t<1>(B, par); break;
         case 4: launch_mupdate_t<4>(B, par); break;

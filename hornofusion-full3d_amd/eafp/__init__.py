@@ -7,6 +7,7 @@ the reference interface (``solver.cg_3d_mpi``), device-resident batches
 the build step (``build``).
 """
 from . import systems  # noqa: F401
+from ._lib import EafpError  # noqa: F401
 from .solver import Batch, Mesh, cg_3d_mpi, release  # noqa: F401
 
-__all__ = ["Batch", "Mesh", "cg_3d_mpi", "release", "systems"]
+__all__ = ["Batch", "EafpError", "Mesh", "cg_3d_mpi", "release", "systems"]

@@ -27,8 +27,8 @@ COEF_FP32, COEF_BF16, COEF_FP16 = 0, 1, 2
 EXPORTS = (
     "eafp_cg_solve", "eafp_release", "eafp_batch_create", "eafp_batch_upload",
     "eafp_batch_solve_fp64", "eafp_batch_solve_mixed", "eafp_batch_download",
-    "eafp_batch_info", "eafp_batch_set_tiling", "eafp_batch_time_kernels",
-    "eafp_batch_autotune", "eafp_batch_destroy", "eafp_version",
+    "eafp_batch_info", "eafp_batch_set_tiling", "eafp_batch_set_columns", "eafp_batch_get_columns",
+    "eafp_batch_time_kernels", "eafp_batch_autotune", "eafp_batch_destroy", "eafp_version",
 )
 
 
@@ -98,6 +98,10 @@ def load():
     ip = ctypes.POINTER(i)
     L.eafp_batch_set_tiling.argtypes = [vp, i, i]
     L.eafp_batch_set_tiling.restype = i
+    L.eafp_batch_set_columns.argtypes = [vp, i]
+    L.eafp_batch_set_columns.restype = i
+    L.eafp_batch_get_columns.argtypes = [vp, ip]
+    L.eafp_batch_get_columns.restype = i
     L.eafp_batch_time_kernels.argtypes = [vp, i, i, i, fp, fp]
     L.eafp_batch_time_kernels.restype = i
     L.eafp_batch_autotune.argtypes = [vp, i, i, i, ip, ip, fp]

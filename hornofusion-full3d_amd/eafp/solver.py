@@ -126,6 +126,10 @@ class Batch:
     def set_tiling(self, cells_per_thread, blocks=0):
         check(self._L.eafp_batch_set_tiling(self._h, int(cells_per_thread), int(blocks)), "eafp_batch_set_tiling")
 
+    def set_columns(self, r_splits):
+        """Mixed stencil as a column march with r_splits r-segments per column (0: tile kernels)."""
+        check(self._L.eafp_batch_set_columns(self._h, int(r_splits)), "eafp_batch_set_columns")
+
     def time_kernels(self, solver, coef_format=_lib.COEF_FP32, reps=20):
         """Average ms of one launch of each iteration kernel (spmv, update)."""
         a = ctypes.c_float()
@@ -155,7 +159,9 @@ class Batch:
         t = ctypes.c_int()
         n = ctypes.c_long()
         check(self._L.eafp_batch_info(self._h, ctypes.byref(g), ctypes.byref(t), ctypes.byref(n)), "eafp_batch_info")
-        return {"grid": g.value, "tile_cells": t.value, "ncells": n.value}
+        c = ctypes.c_int()
+        check(self._L.eafp_batch_get_columns(self._h, ctypes.byref(c)), "eafp_batch_get_columns")
+        return {"grid": g.value, "tile_cells": t.value, "ncells": n.value, "column_splits": c.value}
 
     def close(self):
         if getattr(self, "_h", None):

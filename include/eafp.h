@@ -147,6 +147,18 @@ int eafp_batch_info(const eafp_batch* b, int* grid, int* tile_cells, long* ncell
  * (0: one per tile, at most 8 per CU).  No reference counterpart. */
 int eafp_batch_set_tiling(eafp_batch* b, int cells_per_thread, int blocks);
 
+/* The mixed solver's stencil as a column march: r_splits > 0 cuts every
+ * system into columns of theta-rows (about 1024 cells per r-plane) marched
+ * through the r-planes, each column in r_splits segments; 0 returns to the
+ * tile kernels.  EAFP_ERR_UNSUPPORTED when a system's nz is not a multiple
+ * of 4 or exceeds 256, or a system has fewer tiles than segments.  Kept
+ * across eafp_batch_set_tiling while the new tiles allow it.  No reference
+ * counterpart. */
+int eafp_batch_set_columns(eafp_batch* b, int r_splits);
+
+/* Column setting in use (0: tile kernels). */
+int eafp_batch_get_columns(const eafp_batch* b, int* r_splits);
+
 /* Measurement: average duration (ms) of one launch of each of the two
  * iteration kernels of a solver (solver 0: FP64 CG, k64_spmv/k64_update;
  * solver 1: mixed, k_mspmv/k_mupdate with coef_format) over `reps`
@@ -157,8 +169,11 @@ int eafp_batch_time_kernels(eafp_batch* b, int solver, int coef_format, int reps
                             float* ms_update);
 
 /* Autotuner: times every tile shape (cells per thread x blocks per CU) for
- * the solver's iteration kernels and keeps the fastest; returns the shape
- * and its iteration time (ms).  No reference counterpart. */
+ * the solver's iteration kernels and keeps the fastest; for the mixed
+ * solver it then times the column-marching stencil at 1, 2, 4, 8 and 16
+ * r-segments and keeps it if it is faster (eafp_batch_get_columns tells).
+ * Returns the tile shape and the iteration time (ms).  No reference
+ * counterpart. */
 int eafp_batch_autotune(eafp_batch* b, int solver, int coef_format, int reps, int* cells_per_thread,
                         int* blocks, float* ms_iter);
 

@@ -65,10 +65,15 @@ def test_mixed_single(config, shape, fmt):
     assert not x[~ps.active].any()
 
 
-def test_mixed_production_size():
+@pytest.mark.parametrize("cpt,cols", [(2, 0), (8, 0), (4, 2)])
+def test_mixed_production_size(cpt, cols):
+    """60x120x84 (nz a multiple of 4): cpt 2 runs the per-cell kernels, cpt 8
+    the quad kernels, columns the column-marching stencil."""
     ps = systems.make_system(2)
-    print(f"case mixed config=2 shape={ps.shape}", flush=True)
+    print(f"case mixed config=2 shape={ps.shape} cpt={cpt} columns={cols}", flush=True)
     with eafp.Batch.from_systems([ps]) as b:
+        b.set_tiling(cpt, 0)
+        b.set_columns(cols)
         st = b.solve_mixed(tol=1e-8, coef_format=COEF_BF16)[0]
         phi = b.download(0)
     tr = host_true_residual(ps, phi)
@@ -142,3 +147,92 @@ def test_mixed_tilings_and_restart(cpt, restart):
             assert host_true_residual(ps, phi) < 1.2e-8
             x_ref, _ = oracle_solution(ps, ("b3", s))
             assert oracle.gauge_error(oracle.interior(phi), x_ref, ps.labels, ps.ncomp) < 1e-5
+
+
+@pytest.mark.parametrize("fmt", list(FORMATS))
+@pytest.mark.parametrize("cpt", [4, 8])
+def test_mixed_quad_kernels(cpt, fmt):
+    """Batches whose every nz is a multiple of 4 run the quad kernels at cpt 4
+    and 8 (4 cells of a z-row per thread): same bar as the per-cell kernels,
+    over ragged shapes with partial last tiles, nz = 4 (one quad per row, both
+    z walls in one quad) and a single theta cell (the wrap is the cell row
+    itself)."""
+    syss = [systems.make_system(2, 0, (20, 36, 28)), systems.make_system(1, 0),
+            systems.make_system(5, 1, (24, 48, 32)), systems.make_system(2, 1, (6, 10, 4)),
+            systems.make_system(1, 5, (5, 1, 12))]
+    with eafp.Batch.from_systems(syss) as b:
+        b.set_tiling(cpt, 0)
+        st = b.solve_mixed(tol=1e-8, coef_format=FORMATS[fmt])
+        for s, ps in enumerate(syss):
+            phi = b.download(s)
+            tr = host_true_residual(ps, phi)
+            x_ref, _ = oracle_solution(ps, ("quad", s))
+            err = oracle.gauge_error(oracle.interior(phi), x_ref, ps.labels, ps.ncomp)
+            print(f"case quad cpt={cpt} fmt={fmt} system={s} shape={ps.shape}: {st[s]['status']} "
+                  f"inner={st[s]['n_iter']} res={tr:.2e} err={err:.2e}", flush=True)
+            assert st[s]["status"] == "converged"
+            assert tr < 1.2e-8
+            assert err < 1e-5
+            x = oracle.interior(phi)
+            if ps.ncomp:
+                assert abs(x[ps.active].mean()) < 1e-12 * np.abs(x).max() + 1e-300
+
+
+@pytest.mark.parametrize("fmt", list(FORMATS))
+@pytest.mark.parametrize("splits", [1, 3])
+def test_mixed_column_kernel(splits, fmt):
+    """The column-marching stencil (theta-row slabs marched through r, one
+    quad per thread, LDS theta/z neighbours): same bar as the tile kernels.
+    Shapes: a slab narrower than nth with a partial last column, a single
+    column whose halo rows wrap onto itself (nth*nz <= 1024), nz = 32 and 28,
+    r-segments that split the planes unevenly."""
+    syss = [systems.make_system(2, 0, (20, 36, 28)), systems.make_system(1, 0),
+            systems.make_system(5, 1, (24, 48, 32)), systems.make_system(2, 3, (7, 50, 20))]
+    with eafp.Batch.from_systems(syss) as b:
+        b.set_tiling(1, 0)
+        b.set_columns(splits)
+        assert b.info()["column_splits"] == splits
+        st = b.solve_mixed(tol=1e-8, coef_format=FORMATS[fmt])
+        for s, ps in enumerate(syss):
+            phi = b.download(s)
+            tr = host_true_residual(ps, phi)
+            x_ref, _ = oracle_solution(ps, ("col", s))
+            err = oracle.gauge_error(oracle.interior(phi), x_ref, ps.labels, ps.ncomp)
+            print(f"case columns splits={splits} fmt={fmt} system={s} shape={ps.shape}: {st[s]['status']} "
+                  f"inner={st[s]['n_iter']} res={tr:.2e} err={err:.2e}", flush=True)
+            assert st[s]["status"] == "converged"
+            assert tr < 1.2e-8
+            assert err < 1e-5
+            x = oracle.interior(phi)
+            if ps.ncomp:
+                assert abs(x[ps.active].mean()) < 1e-12 * np.abs(x).max() + 1e-300
+
+
+def test_mixed_column_kernel_ragged_with_failures():
+    """Column path with a zero-RHS and a NaN system in the batch: both end as
+    in the tile path, the others converge."""
+    syss = [systems.make_system(2, 0, (20, 36, 28)), systems.make_system(1, 1)]
+    z = systems.make_system(1, 2)
+    z.Su[...] = 0.0
+    bad = systems.make_system(1, 3)
+    bad.Su[6, 7, 8] = np.nan
+    syss += [z, bad]
+    with eafp.Batch.from_systems(syss) as b:
+        b.set_columns(1)
+        st = b.solve_mixed(tol=1e-8, max_steps=100, coef_format=COEF_FP32)
+        print(f"case columns ragged: {[x['status'] for x in st]}", flush=True)
+        assert st[2]["status"] == "zero_rhs"
+        assert st[3]["status"] == "max_iter" and math.isnan(st[3]["residual"])
+        for s in (0, 1):
+            assert st[s]["status"] == "converged"
+            assert host_true_residual(syss[s], b.download(s)) < 1.2e-8
+
+
+def test_mixed_columns_refused_when_ineligible():
+    """nz not a multiple of 4: the column setting is refused and the tile
+    kernels stay in use."""
+    ps = systems.make_system(2, 0, (10, 12, 30))
+    with eafp.Batch.from_systems([ps]) as b:
+        with pytest.raises(eafp.EafpError):
+            b.set_columns(1)
+        assert b.info()["column_splits"] == 0

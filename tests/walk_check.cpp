@@ -13,6 +13,15 @@
 //     stencil (theta wrap, r/z walls flagged),
 //   - every tile's partial slot is written exactly once, by its system,
 //   - a run never crosses a system or a chunk.
+// For batches whose every nz is a multiple of 4 it also replays the quad
+// kernels (a thread takes 4 consecutive cells of a z-row): every 16-byte
+// vector access must start 16-byte aligned and lie inside its system, and
+// every cell must be covered exactly once.
+// It replays the column-marching stencil the same way (column table, quad
+// and halo lanes, planes i0-1 .. i1 plus the prefetch of i+1 and i+2, the
+// LDS slab indices, the partial slots) and checks that every global access
+// stays inside its system, every LDS index inside its array, every cell is
+// computed exactly once and every partial slot written exactly once.
 // Prints "ok <cases>" and exits 0, or the first violation and exits 1.
 #include <cstdio>
 #include <cstdlib>
@@ -101,6 +110,125 @@ static void check_shape(std::vector<SysGeom> g, int cpt, int G) {
     ++g_cases;
 }
 
+static void check_quads(std::vector<SysGeom> g, int cpt, int G) {
+    const long ncells = layout_systems(g);
+    const int TILE = BLOCK * cpt;
+    const int Q = cpt / 4;
+    std::vector<TileDesc> tiles = make_tile_table(g, TILE);
+    const int ntiles = (int)tiles.size();
+    std::vector<unsigned char> hit(ncells, 0);
+    for (int b = 0; b < G; ++b) {
+        const TileWalk w(ntiles, G, b);
+        for (int t = w.first; t < w.end;) {
+            const TileDesc T = tiles[t];
+            CHECK(T.nz % 4 == 0 && T.n % 4 == 0 && T.off % 4 == 0, "quad geometry");
+            const int nrun = w.run(t, T.tile_end);
+            for (int u = 0; u < nrun; ++u) {
+                const int tile = t + u * w.stride;
+                for (int tid = 0; tid < BLOCK; ++tid) {
+                    const int c0 = (tile - T.tile_begin) * TILE + 4 * tid;
+                    for (int m = 0; m < Q; ++m) {
+                        const int c = c0 + 4 * m * BLOCK;
+                        const bool ok = c < T.n;
+                        const int cc = ok ? c : T.n - 4;
+                        const CellIdx id = decompose(cc, T);
+                        CHECK(id.k % 4 == 0 && id.k + 3 < T.nz, "quad crosses a row at %d", cc);
+                        const long o = T.off + cc;
+                        const Nb4 nb = neighbours4(id, T, o);
+                        const long lo = T.off, hi = T.off + T.n;
+                        const long vec[5] = {o, nb.W, nb.E, nb.S, nb.N};
+                        for (long v : vec)
+                            CHECK(v % 4 == 0 && v >= lo && v + 4 <= hi, "vector %ld outside [%ld,%ld)", v, lo, hi);
+                        CHECK(nb.B >= lo && nb.B < hi && nb.T >= lo && nb.T < hi, "B/T of quad %d", cc);
+                        const long sr = (long)T.nth * T.nz;
+                        CHECK(nb.W == (id.i > 0 ? o - sr : o) && nb.E == (id.i < T.nr - 1 ? o + sr : o), "W/E");
+                        CHECK(nb.B == (id.k > 0 ? o - 1 : o) && nb.T == (id.k + 4 < T.nz ? o + 4 : o), "B/T");
+                        if (ok)
+                            for (int q = 0; q < 4; ++q) {
+                                CHECK(hit[o + q] == 0, "cell %ld computed twice", o + q);
+                                hit[o + q] = 1;
+                            }
+                    }
+                }
+            }
+            t += nrun * w.stride;
+        }
+    }
+    for (const SysGeom& S : g)
+        for (long c = 0; c < S.n; ++c) CHECK(hit[S.off + c] == 1, "quad cell %ld of system at %ld missed", c, S.off);
+    ++g_cases;
+}
+
+static void check_columns(std::vector<SysGeom> g, int cpt, int splits) {
+    const long ncells = layout_systems(g);
+    std::vector<TileDesc> tiles = make_tile_table(g, BLOCK * cpt);
+    std::vector<ColDesc> cols;
+    if (!make_col_table(g, splits, cols)) return;      // not eligible: the library refuses it too
+    CHECK(cols.size() % NXCD == 0, "column table not padded");
+    const int ntiles = (int)tiles.size();
+    std::vector<unsigned char> hit(ncells, 0), part(ntiles, 0);
+    const int SP = COL_CELLS + 2 * COL_MAX_NZ, SW = COL_CELLS + COL_MAX_NZ;
+    const int G = (int)cols.size();
+    std::vector<unsigned char> seen(G, 0);
+    for (int bidx = 0; bidx < G; ++bidx) {
+        const int lb = (bidx % NXCD) * (G / NXCD) + bidx / NXCD;
+        CHECK(lb >= 0 && lb < G && !seen[lb], "column mapping");
+        seen[lb] = 1;
+        const ColDesc C = cols[lb];
+        if (C.tile < 0) continue;
+        const TileDesc T = tiles[C.tile];
+        const long lo = T.off, hi = T.off + T.n, sr = (long)T.nth * T.nz;
+        const int Qr = T.nz / 4;
+        CHECK(C.nj >= 1 && C.nj * T.nz <= COL_CELLS && C.j0 + C.nj <= T.nth && C.i0 < C.i1 && C.i1 <= T.nr,
+              "column shape");
+        auto base = [&](int i) { return T.off + (long)(i < T.nr ? i : T.nr - 1) * sr; };
+        for (int tid = 0; tid < BLOCK; ++tid) {
+            const int row = tid / Qr, kq = tid - row * Qr;
+            const bool act = row < C.nj;
+            const long qoff = (long)(C.j0 + (act ? row : 0)) * T.nz + 4 * (act ? kq : 0);
+            const bool hal = tid < 2 * Qr;
+            const int hrow = tid < Qr ? -1 : C.nj;
+            const int hk = hal ? 4 * (tid < Qr ? tid : tid - Qr) : 0;
+            const long hoff = (long)(((C.j0 + hrow) % T.nth + T.nth) % T.nth) * T.nz + hk;
+            auto vec = [&](long a) { CHECK(a % 4 == 0 && a >= lo && a + 4 <= hi, "column vector %ld", a); };
+            vec(base(C.i0 > 0 ? C.i0 - 1 : C.i0) + qoff);
+            vec(base(C.i0) + qoff);
+            vec(base(C.i0 + 1) + qoff);
+            vec(base(C.i0) + hoff);
+            for (int i = C.i0; i < C.i1; ++i) {
+                if (i + 1 < C.i1) {
+                    vec(base(i + 2) + qoff);
+                    vec(base(i + 1) + qoff);
+                    vec(base(i + 1) + hoff);
+                }
+                if (act) {
+                    for (int q = 0; q < 4; ++q) {
+                        const int sidx = (row + 1) * T.nz + 4 * kq + q;
+                        CHECK(sidx >= 0 && sidx + T.nz < SP && sidx - T.nz >= 0, "Sp index");
+                        CHECK(row * T.nz + 4 * kq + q < SW && (row + 1) * T.nz + 4 * kq + q < SW, "Sw1 index");
+                    }
+                    const long o = base(i) + qoff;
+                    vec(o);
+                    for (int q = 0; q < 4; ++q) {
+                        CHECK(hit[o + q] == 0, "column cell %ld computed twice", o + q);
+                        hit[o + q] = 1;
+                    }
+                }
+                if (hal) CHECK((hrow + 1) * T.nz + hk + 3 < SP, "halo Sp index");
+            }
+        }
+        CHECK(T.tile_begin + C.seg < T.tile_end, "segment slot");
+        for (long t = T.tile_begin + C.seg; t < T.tile_end; t += C.nseg) {
+            CHECK(part[t] == 0, "column partial slot %ld written twice", t);
+            part[t] = 1;
+        }
+    }
+    for (const SysGeom& S : g)
+        for (long c = 0; c < S.n; ++c) CHECK(hit[S.off + c] == 1, "column cell %ld of system at %ld missed", c, S.off);
+    for (int t = 0; t < ntiles; ++t) CHECK(part[t] == 1, "column partial slot %d never written", t);
+    ++g_cases;
+}
+
 static SysGeom sys(int nr, int nth, int nz) {
     SysGeom s{};
     s.nr = nr;
@@ -117,6 +245,8 @@ int main() {
         {sys(20, 36, 28), sys(1, 1, 1), sys(24, 48, 30), sys(7, 3, 2), sys(9, 64, 5)},   // ragged batch
         {sys(60, 120, 84), sys(17, 40, 33), sys(60, 120, 84)},        // production systems with one between
         {sys(2, 1, 300), sys(1, 257, 1), sys(5, 2, 1)},               // degenerate axes
+        {sys(120, 240, 168)},                                         // the large single system
+        {sys(60, 120, 84), sys(3, 1, 4), sys(1, 7, 8), sys(20, 36, 28)},   // nz multiples of 4, ragged
     };
     const int cpts[] = {1, 2, 4, 8};
     for (auto& bt : batches) {
@@ -129,10 +259,14 @@ int main() {
             // CUs, and small grids
             const int rounded = (ntiles + NXCD - 1) / NXCD * NXCD;
             const int grids[] = {8, 16, 24, 136, rounded, rounded + 8, 4 * 256, 8 * 256, 16 * 256};
+            bool quad = true;
+            for (const SysGeom& s : bt) quad = quad && s.nz % 4 == 0;
             for (int G : grids) {
                 if (G > 16 * 256 && G != rounded) continue;
                 check_shape(bt, cpt, G);
+                if (quad && cpt >= 4) check_quads(bt, cpt, G);
             }
+            for (int splits : {1, 2, 3, 4, 8}) check_columns(bt, cpt, splits);
         }
     }
     std::printf("ok %ld\n", g_cases);
