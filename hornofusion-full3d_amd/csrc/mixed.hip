@@ -5,8 +5,9 @@
 // (+r, +theta, +z) get one symmetric weight w = (a_own + a_neighbour)/2,
 // rounded to the storage format (FP32, BF16, or FP16 scaled by the owner's
 // diagonal aP); the diagonal excess e = aP - sum(interior links) (compliance,
-// acoustic, Dirichlet and wall terms; identity rows give e = aP) is kept in
-// FP32, and the Jacobi inverse 1/(sum of the six rounded weights + e).
+// acoustic, Dirichlet and wall terms; identity rows give e = aP) is stored
+// in the same format, and the Jacobi inverse 1/(sum of the six rounded
+// weights + rounded e) in FP32.
 //
 // Inner operator, flux form:  (A_lo p)_c = sum_f w_f (p_c - p_nb) + e_c p_c.
 // It is an exact weighted graph Laplacian plus a diagonal: rounding the
@@ -28,9 +29,9 @@
 //   k_mspmv   beta, p = z + beta*p (recomputed at neighbours), q = A_lo p, p.q
 //   k_mupdate alpha, d += alpha p, r -= alpha q, z = r/D, r.r and r.z
 // Algorithmic bytes per cell and inner iteration: k_mspmv reads z, p_old,
-// 3 weights (+1 FP16 scale), e; writes p, q: 8+8+3w(+4)+4 B; k_mupdate
-// reads d, r, p, q, 1/D, writes d, r, z: 32 B.  FP32 weights: 64 B,
-// BF16: 58 B, FP16+scale: 62 B.
+// 3 weights and e (+1 FP32 scale for FP16); writes p, q: 8 + 4w (+4) + 8 B;
+// k_mupdate reads d, r, p, q, 1/D, writes d, r, z: 32 B.  FP32 weights:
+// 64 B, BF16: 56 B, FP16 + scale: 60 B.
 //
 // One captured graph runs a whole refinement step for every system:
 // interval x (k_mspmv, k_mupdate), k_res, k_mreplace, k_mcount.
@@ -83,7 +84,7 @@ struct MxArgs {
     // compressed operand
     void* w[3];          // +r, +theta, +z face weights (format dependent)
     float* scale;        // FP16 only: per-cell scale (the owner's aP)
-    float* e;            // diagonal excess
+    void* e;             // diagonal excess, in the weight format (FP16: scaled like the weights)
     float* dinv;         // Jacobi inverse of the rounded operator
     // inner FP32 vectors
     float *d, *r, *z, *q;
@@ -196,13 +197,13 @@ __global__ __launch_bounds__(BLOCK) void k_compress(MxArgs a, int tile) {
                 D += hW ? W::dec(W::enc((float)wrm, sc(oW)), sc(oW)) : 0.f;
                 D += W::dec(W::enc((float)wtm, sc(oS)), sc(oS));
                 D += hB ? W::dec(W::enc((float)wzm, sc(oB)), sc(oB)) : 0.f;
-                const float e32 = (float)e;
-                D += e32;
+                const typename W::T ee = W::enc((float)e, s_o);
+                D += W::dec(ee, s_o);
                 ((typename W::T*)a.w[0])[o] = er;
                 ((typename W::T*)a.w[1])[o] = et;
                 ((typename W::T*)a.w[2])[o] = ez;
                 if (F == EAFP_COEF_FP16) a.scale[o] = s_o;
-                a.e[o] = e32;
+                ((typename W::T*)a.e)[o] = ee;
                 a.dinv[o] = D > 0.f ? 1.0f / D : 0.0f;
             }
         }
@@ -400,7 +401,6 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv(MxArgs a, int par) {
     const float* __restrict__ pold = a.p[par];
     float* __restrict__ pnew = a.p[par ^ 1];
     const float* __restrict__ z = a.z;
-    const float* __restrict__ ew = a.e;
     float* __restrict__ qo = a.q;
     const TileWalk w = my_tiles(a.ntiles);
     for (int t = w.first; t < w.end;) {
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv(MxArgs a, int par) {
                 wv[m][3] = W::load(a.w[1], a.scale, nb.S);
                 wv[m][4] = W::load(a.w[2], a.scale, o);
                 wv[m][5] = W::load(a.w[2], a.scale, nb.B);
-                ev[m] = ew[o];
+                ev[m] = W::load(a.e, a.scale, o);
             }
         };
         // the first batch of cell loads is in flight while the partials are reduced
@@ -589,7 +589,6 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv4(MxArgs a, int par) {
     const float* __restrict__ pold = a.p[par];
     float* __restrict__ pnew = a.p[par ^ 1];
     const float* __restrict__ z = a.z;
-    const float* __restrict__ ew = a.e;
     float* __restrict__ qo = a.q;
     const TileWalk w = my_tiles(a.ntiles);
     for (int t = w.first; t < w.end;) {
@@ -631,7 +630,7 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv4(MxArgs a, int par) {
                 wq[m][3] = W::load4(a.w[1], a.scale, nb.S);   // -theta faces
                 wq[m][4] = W::load4(a.w[2], a.scale, o);      // +z faces
                 wB[m] = W::load(a.w[2], a.scale, nb.B);       // -z face of the first cell
-                ev[m] = ld4(ew + o);
+                ev[m] = W::load4(a.e, a.scale, o);
             }
         };
         load(t);
@@ -811,7 +810,6 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
     const float* __restrict__ pold = a.p[par];
     float* __restrict__ pnew = a.p[par ^ 1];
     const float* __restrict__ z = a.z;
-    const float* __restrict__ ew = a.e;
     float* __restrict__ qo = a.q;
     const int nz = T.nz, nth = T.nth, nr = T.nr;
     const int Qr = nz >> 2;                       // quads per row
@@ -837,7 +835,7 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
     F4 w0 = W::load4(a.w[0], a.scale, base(i0) + qoff);
     F4 w1 = W::load4(a.w[1], a.scale, base(i0) + qoff);
     F4 w2 = W::load4(a.w[2], a.scale, base(i0) + qoff);
-    F4 ev = ld4(ew + base(i0) + qoff);
+    F4 ev = W::load4(a.e, a.scale, base(i0) + qoff);
     F4 zH = ld4(z + base(i0) + hoff), oH = ld4(pold + base(i0) + hoff);
     F4 w1H = W::load4(a.w[1], a.scale, base(i0) + hoff);
 
@@ -878,7 +876,7 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
             w0n = W::load4(a.w[0], a.scale, base(i + 1) + qoff);
             w1n = W::load4(a.w[1], a.scale, base(i + 1) + qoff);
             w2n = W::load4(a.w[2], a.scale, base(i + 1) + qoff);
-            evn = ld4(ew + base(i + 1) + qoff);
+            evn = W::load4(a.e, a.scale, base(i + 1) + qoff);
             zHn = ld4(z + base(i + 1) + hoff);
             oHn = ld4(pold + base(i + 1) + hoff);
             w1Hn = W::load4(a.w[1], a.scale, base(i + 1) + hoff);
@@ -992,7 +990,8 @@ struct MixedWork {
     int fmt = -1;
     DevBuf<double> X0, X1, r64, xout, partR, partA, partB;
     DevBuf<unsigned char> w[3];
-    DevBuf<float> scale, e, dinv, d, r, z, q, p0, p1;
+    DevBuf<unsigned char> e;
+    DevBuf<float> scale, dinv, d, r, z, q, p0, p1;
     DevBuf<MxDyn> dyn0, dyn1;
     DevBuf<MxStat> stat0, stat1;
     DevBuf<MxConst> cst;
@@ -1027,7 +1026,6 @@ static int ensure_mixed(Batch& B, int fmt) {
         EAFP_TRY_RC(w->X1.alloc(n));
         EAFP_TRY_RC(w->r64.alloc(n));
         EAFP_TRY_RC(w->xout.alloc(n));
-        EAFP_TRY_RC(w->e.alloc(n));
         EAFP_TRY_RC(w->dinv.alloc(n));
         EAFP_TRY_RC(w->d.alloc(n));
         EAFP_TRY_RC(w->r.alloc(n));
@@ -1053,6 +1051,7 @@ static int ensure_mixed(Batch& B, int fmt) {
     if (w->fmt != fmt) {
         const size_t n = B.ncells;
         for (auto& wb : w->w) EAFP_TRY_RC(wb.alloc(n * wbytes(fmt)));
+        EAFP_TRY_RC(w->e.alloc(n * wbytes(fmt)));
         if (fmt == EAFP_COEF_FP16) EAFP_TRY_RC(w->scale.alloc(n));
         w->fmt = fmt;
     }
