@@ -65,8 +65,8 @@ int Batch::set_tiling(int cells_per_thread, int blocks) {
 
 // Column-marching stencil for the mixed solver: r_splits > 0 cuts every
 // column into that many r-segments; 0 returns to the tile kernels.
-// EAFP_ERR_UNSUPPORTED when a system does not qualify (nz not a multiple
-// of 4 or above COL_MAX_NZ, or fewer tiles than segments).
+// EAFP_ERR_UNSUPPORTED when a system does not qualify (nz above
+// COL_MAX_NZ, or fewer tiles than segments).
 int Batch::set_columns(int r_splits) {
     if (r_splits < 0) return EAFP_ERR_ARG;
     if (r_splits == 0) {

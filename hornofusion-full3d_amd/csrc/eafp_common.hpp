@@ -98,10 +98,10 @@ inline std::vector<TileDesc> make_tile_table(std::vector<SysGeom>& g, int tile) 
 
 // ---------------------------------------------------------------------------
 // Column segments, the work unit of the column-marching stencil kernels
-// (systems whose nz is a multiple of 4 and at most COL_MAX_NZ).  A column
-// is a slab of nj theta-rows of one system (nj*nz <= 4*BLOCK cells, one quad
-// of 4 z-cells per thread), marched through the r-planes i0 .. i1-1 of its
-// segment.  The segments of a system are numbered 0 .. nseg-1; segment u
+// (systems with nz at most COL_MAX_NZ).  A column is a slab of nj
+// theta-rows of one system (nj*nz <= 4*BLOCK cells: four cells per thread,
+// one quad of a row when nz is a multiple of 4), marched through the
+// r-planes i0 .. i1-1 of its segment.  The segments of a system are numbered 0 .. nseg-1; segment u
 // writes its partial sum at tile slot tile_begin + u and zeros at the slots
 // tile_begin + u + v*nseg (v >= 1) of the system, so the system's partials
 // still cover its whole tile range exactly once.
@@ -124,12 +124,12 @@ __host__ __device__ __forceinline__ int col_rows(int nz) { return COL_CELLS / nz
 // Builds the column table: every system cut into ceil(nth/rows) columns and
 // each column into `splits` r-segments (fewer when nr is smaller).  Padded
 // with empty entries to a multiple of NXCD.  Returns false (and no table)
-// when a system does not qualify: nz not a multiple of 4 or above
-// COL_MAX_NZ, or fewer tiles than segments (the partial slots).
+// when a system does not qualify: nz above COL_MAX_NZ, or fewer tiles than
+// segments (the partial slots).
 inline bool make_col_table(const std::vector<SysGeom>& g, int splits, std::vector<ColDesc>& out) {
     out.clear();
     for (const SysGeom& G : g) {
-        if (G.nz % 4 != 0 || G.nz > COL_MAX_NZ) return false;
+        if (G.nz > COL_MAX_NZ) return false;
         const int rows = col_rows(G.nz);
         const int ncol = (G.nth + rows - 1) / rows;
         const int nsp = splits < G.nr ? splits : G.nr;

@@ -34,7 +34,7 @@
 // 64 B, BF16: 56 B, FP16 + scale: 60 B.
 //
 // One captured graph runs a whole refinement step for every system:
-// interval x (k_mspmv, k_mupdate), k_res, k_mreplace, k_mcount.
+// interval x (k_mspmv, k_mupdate), k_res, k_rsum, k_mreplace, k_mcount.
 #include <cmath>
 #include <vector>
 
@@ -90,6 +90,8 @@ struct MxArgs {
     float *d, *r, *z, *q;
     float* p[2];
     double* partR;       // ntiles * NRES (k_res)
+    double* sumR;        // nsys * NRES: partR reduced per system (k_rsum)
+    const SysGeom* geom; // per-system geometry and tile ranges
     double* partA;       // ntiles        (k_mspmv: p.q)
     double* partB;       // ntiles * 2    (k_mreplace, k_mupdate: r.r, r.z)
     MxDyn* dyn[2];
@@ -261,8 +263,12 @@ __global__ __launch_bounds__(BLOCK) void k_res(MxArgs a, int tile) {
                     double xv[7];
 #pragma unroll
                     for (int q = 0; q < 7; ++q) xv[q] = first ? xo[idx[q]] : xo[idx[q]] + (double)d[idx[q]];
-                    const double cP = aP[o], cW = aW[o], cE = aE[o], cS = aS[o], cN = aN[o], cB = aB[o], cT = aT[o];
-                    const double su = bb[o], fo = fold[o];
+                    // the FP64 operand is read once per refinement: nontemporal
+                    const double cP = __builtin_nontemporal_load(aP + o), cW = __builtin_nontemporal_load(aW + o),
+                                 cE = __builtin_nontemporal_load(aE + o), cS = __builtin_nontemporal_load(aS + o),
+                                 cN = __builtin_nontemporal_load(aN + o), cB = __builtin_nontemporal_load(aB + o),
+                                 cT = __builtin_nontemporal_load(aT + o);
+                    const double su = __builtin_nontemporal_load(bb + o), fo = __builtin_nontemporal_load(fold + o);
                     const int slot = comp[o];
                     const double xc = xv[0];
                     const double ax = cP * xc - cW * (nb.hW ? xv[1] : 0.0) - cE * (nb.hE ? xv[2] : 0.0) -
@@ -288,6 +294,22 @@ __global__ __launch_bounds__(BLOCK) void k_res(MxArgs a, int tile) {
     }
 }
 
+// Refinement, between the parts: one block per system reduces the k_res
+// partials of its tiles (in tile order, as every other reduction) into the
+// system's NRES sums, which every block of k_mreplace then reads, instead
+// of each block reducing the NRES columns of all the system's tiles again.
+__global__ __launch_bounds__(BLOCK) void k_rsum(MxArgs a) {
+    __shared__ double sh[NRES * (BLOCK / 64)];
+    const int s = blockIdx.x;
+    const SysGeom g = a.geom[s];
+    double tot[NRES];
+    reduce_tiles<NRES>(a.partR, g.tile_begin, g.tile_end, tot, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < NRES; ++c) a.sumR[(long)s * NRES + c] = tot[c];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Refinement, part 2 (reads slot 0, writes slot 1): true residual, stop
 // tests, component means; then the inner residual is replaced by
@@ -308,7 +330,8 @@ __global__ __launch_bounds__(BLOCK) void k_mreplace(MxArgs a, int tile) {
         if (!done) {
             const MxConst& K = a.cst[s];
             double tot[NRES];
-            reduce_tiles<NRES>(a.partR, T.tile_begin, T.tile_end, tot, sh);
+#pragma unroll
+            for (int c = 0; c < NRES; ++c) tot[c] = a.sumR[(long)s * NRES + c];
             const int rnd = P.n_refine;
             MxStat N = P;
             N.bnorm = rnd == 0 ? sqrt(tot[2]) : P.bnorm;
@@ -367,16 +390,16 @@ __global__ __launch_bounds__(BLOCK) void k_mreplace(MxArgs a, int tile) {
                 for (int c = base + (int)threadIdx.x; c < c_end; c += BLOCK) {
                     const long o = T.off + c;
                     const int slot = comp[o];
-                    const double rv = r64[o];
-                    const float iv = dinv[o];
+                    const double rv = __builtin_nontemporal_load(r64 + o);
+                    const float iv = __builtin_nontemporal_load(dinv + o);
                     double mu = 0.0;
 #pragma unroll
                     for (int q = 0; q < MAX_COMP; ++q) mu = slot == q ? mean_r[q] : mu;
                     const float rr = (float)(rv - mu);
                     const float zz = iv * rr;
-                    r[o] = rr;
+                    __builtin_nontemporal_store(rr, r + o);
                     z[o] = zz;
-                    d[o] = 0.0f;
+                    __builtin_nontemporal_store(0.0f, d + o);
                     if (zero_p) p0[o] = 0.0f;
                     acc[0] += (double)rr * (double)rr;
                     acc[1] += (double)rr * (double)zz;
@@ -521,11 +544,11 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate(MxArgs a, int par) {
                 ok[m] = c < T.n;
                 const long o = T.off + clamp_cell(c, T.n);
                 oc[m] = o;
-                dv[m] = d[o];
-                pvv[m] = pv[o];
-                rv[m] = r[o];
-                qv[m] = q[o];
-                iv[m] = dinv[o];
+                dv[m] = __builtin_nontemporal_load(d + o);
+                pvv[m] = __builtin_nontemporal_load(pv + o);
+                rv[m] = __builtin_nontemporal_load(r + o);
+                qv[m] = __builtin_nontemporal_load(q + o);
+                iv[m] = __builtin_nontemporal_load(dinv + o);
             }
         };
         // the first batch of cell loads is in flight while the partials are reduced
@@ -555,8 +578,8 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate(MxArgs a, int par) {
                     const float rr = rv[m] - fa * qv[m];
                     const float zz = iv[m] * rr;
                     if (ok[m]) {
-                        d[oc[m]] = dv[m] + fa * pvv[m];
-                        r[oc[m]] = rr;
+                        __builtin_nontemporal_store(dv[m] + fa * pvv[m], d + oc[m]);
+                        __builtin_nontemporal_store(rr, r + oc[m]);
                         zo[oc[m]] = zz;
                         acc[0] += (double)rr * (double)rr;
                         acc[1] += (double)rr * (double)zz;
@@ -948,6 +971,204 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
         a.partA[t] = 0.0;
 }
 
+// ---------------------------------------------------------------------------
+// Column-marching stencil for any nz (at most COL_MAX_NZ): as k_mspmv_col,
+// but a thread takes the slab cells tid + m*BLOCK (m < 4) of each plane with
+// 4-byte accesses, and the z neighbours of every cell come from the LDS
+// slab too (w+z of the whole slab is staged for the -z faces).
+// ---------------------------------------------------------------------------
+template <int F>
+__global__ __launch_bounds__(BLOCK) void k_mspmv_colg(MxArgs a, int par) {
+    using W = WFmt<F>;
+    constexpr int M = COL_CELLS / BLOCK;             // slab cells per thread
+    constexpr int SP = COL_CELLS + 2 * COL_MAX_NZ;   // p slab: rows -1 .. nj
+    constexpr int SW = COL_CELLS + COL_MAX_NZ;       // w+theta slab: rows -1 .. nj-1
+    __shared__ double sh[2 * (BLOCK / 64)];
+    __shared__ float Sp[2][SP];
+    __shared__ float Sw1[2][SW];
+    __shared__ float Sw2[2][COL_CELLS];
+    const int G = gridDim.x;
+    const ColDesc C = a.cols[(blockIdx.x % NXCD) * (G / NXCD) + blockIdx.x / NXCD];
+    if (C.tile < 0) return;
+    const TileDesc T = a.tiles[C.tile];
+    const int s = T.sys;
+    const float* __restrict__ pold = a.p[par];
+    float* __restrict__ pnew = a.p[par ^ 1];
+    const float* __restrict__ z = a.z;
+    float* __restrict__ qo = a.q;
+    const int nz = T.nz, nth = T.nth, nr = T.nr;
+    const int tid = threadIdx.x;
+    const int ncell = C.nj * nz;
+    int row[M], kk[M];
+    long coff[M];
+    bool act[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int c = tid + m * BLOCK;
+        act[m] = c < ncell;
+        const int cc = act[m] ? c : 0;               // lanes past the slab shadow cell 0
+        row[m] = fdiv(cc, nz, T.inv_nz);
+        kk[m] = cc - row[m] * nz;
+        coff[m] = (long)(C.j0 + row[m]) * nz + kk[m];
+    }
+    // halo cells h = tid + m*BLOCK < 2*nz: row -1 (h < nz), then row nj
+    constexpr int MH = 2 * COL_MAX_NZ / BLOCK;
+    bool hal[MH];
+    int hslot[MH];
+    long hoff[MH];
+#pragma unroll
+    for (int m = 0; m < MH; ++m) {
+        const int h = tid + m * BLOCK;
+        hal[m] = h < 2 * nz;
+        const int hc = hal[m] ? h : 0;
+        const int hrow = hc < nz ? -1 : C.nj;
+        const int hk = hc < nz ? hc : hc - nz;
+        hslot[m] = (hrow + 1) * nz + hk;
+        hoff[m] = (long)(((C.j0 + hrow) % nth + nth) % nth) * nz + hk;
+    }
+    const long sr = (long)nth * nz;
+    auto base = [&](int i) { return T.off + (long)(i < nr ? i : nr - 1) * sr; };
+    const int i0 = C.i0, i1 = C.i1;
+    const long bW = base(i0 > 0 ? i0 - 1 : i0), b0 = base(i0), b1 = base(i0 + 1);
+    float zW[M], oW[M], w0W[M], zC[M], oC[M], zE[M], oE[M], w0[M], w1[M], w2[M], ev[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        zW[m] = z[bW + coff[m]];
+        oW[m] = pold[bW + coff[m]];
+        w0W[m] = W::load(a.w[0], a.scale, bW + coff[m]);
+        zC[m] = z[b0 + coff[m]];
+        oC[m] = pold[b0 + coff[m]];
+        zE[m] = z[b1 + coff[m]];
+        oE[m] = pold[b1 + coff[m]];
+        w0[m] = W::load(a.w[0], a.scale, b0 + coff[m]);
+        w1[m] = W::load(a.w[1], a.scale, b0 + coff[m]);
+        w2[m] = W::load(a.w[2], a.scale, b0 + coff[m]);
+        ev[m] = W::load(a.e, a.scale, b0 + coff[m]);
+    }
+    float zH[MH], oH[MH], w1H[MH];
+#pragma unroll
+    for (int m = 0; m < MH; ++m) {
+        zH[m] = z[b0 + hoff[m]];
+        oH[m] = pold[b0 + hoff[m]];
+        w1H[m] = W::load(a.w[1], a.scale, b0 + hoff[m]);
+    }
+
+    const MxDyn P = a.dyn[1][s];
+    MxDyn N = P;
+    bool skip = P.done || P.broken;
+    if (!skip) {
+        double tot[2];
+        reduce_tiles<2>(a.partB, T.tile_begin, T.tile_end, tot, sh);
+        if (!(tot[1] > 0.0)) {
+            skip = true;
+            N.broken = 1;
+        } else {
+            N.beta = P.fresh ? 0.0 : tot[1] / P.rz;
+            N.rz = tot[1];
+            N.fresh = 0;
+        }
+    }
+    if (C.seg == 0 && tid == 0) store_state(&a.dyn[0][s], N);
+    if (skip) return;
+    const float fb = (float)N.beta;
+    float pW[M], pC[M], pE[M], pH[MH];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        pW[m] = zW[m] + fb * oW[m];
+        pC[m] = zC[m] + fb * oC[m];
+        pE[m] = zE[m] + fb * oE[m];
+    }
+#pragma unroll
+    for (int m = 0; m < MH; ++m) pH[m] = zH[m] + fb * oH[m];
+    double acc[1] = {0.0};
+    for (int i = i0; i < i1; ++i) {
+        const int b = i & 1;
+        const bool more = i + 1 < i1;
+        float zN2[M], oN2[M], w0n[M], w1n[M], w2n[M], evn[M], zHn[MH], oHn[MH], w1Hn[MH];
+        if (more) {
+            const long b2 = base(i + 2), bn = base(i + 1);
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                zN2[m] = z[b2 + coff[m]];
+                oN2[m] = pold[b2 + coff[m]];
+                w0n[m] = W::load(a.w[0], a.scale, bn + coff[m]);
+                w1n[m] = W::load(a.w[1], a.scale, bn + coff[m]);
+                w2n[m] = W::load(a.w[2], a.scale, bn + coff[m]);
+                evn[m] = W::load(a.e, a.scale, bn + coff[m]);
+            }
+#pragma unroll
+            for (int m = 0; m < MH; ++m) {
+                zHn[m] = z[bn + hoff[m]];
+                oHn[m] = pold[bn + hoff[m]];
+                w1Hn[m] = W::load(a.w[1], a.scale, bn + hoff[m]);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            if (act[m]) {
+                Sp[b][(row[m] + 1) * nz + kk[m]] = pC[m];
+                Sw1[b][(row[m] + 1) * nz + kk[m]] = w1[m];
+                Sw2[b][row[m] * nz + kk[m]] = w2[m];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < MH; ++m) {
+            if (hal[m]) {
+                Sp[b][hslot[m]] = pH[m];
+                if (hslot[m] < nz) Sw1[b][hslot[m]] = w1H[m];
+            }
+        }
+        __syncthreads();
+        const bool hW = i > 0;
+        const long bi = base(i);
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            if (act[m]) {
+                const int sl = (row[m] + 1) * nz + kk[m];
+                const float pc = pC[m];
+                const bool hB = kk[m] > 0, hT = kk[m] < nz - 1;
+                const float pS = Sp[b][sl - nz], pN = Sp[b][sl + nz];
+                const float pB = hB ? Sp[b][sl - 1] : pc;
+                const float pT = hT ? Sp[b][sl + 1] : pc;
+                const float wS = Sw1[b][sl - nz];
+                const float wB = hB ? Sw2[b][row[m] * nz + kk[m] - 1] : 0.f;
+                float y = ev[m] * pc;
+                y += w0[m] * (pc - pE[m]);
+                y += (hW ? w0W[m] : 0.f) * (pc - pW[m]);
+                y += w1[m] * (pc - pN);
+                y += wS * (pc - pS);
+                y += w2[m] * (pc - pT);
+                y += wB * (pc - pB);
+                pnew[bi + coff[m]] = pc;
+                qo[bi + coff[m]] = y;
+                acc[0] += (double)pc * (double)y;
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                pW[m] = pC[m];
+                pC[m] = pE[m];
+                pE[m] = zN2[m] + fb * oN2[m];
+                w0W[m] = w0[m];
+                w0[m] = w0n[m];
+                w1[m] = w1n[m];
+                w2[m] = w2n[m];
+                ev[m] = evn[m];
+            }
+#pragma unroll
+            for (int m = 0; m < MH; ++m) {
+                pH[m] = zHn[m] + fb * oHn[m];
+                w1H[m] = w1Hn[m];
+            }
+        }
+    }
+    block_sum<1>(acc, sh);
+    if (tid == 0) a.partA[(long)T.tile_begin + C.seg] = acc[0];
+    for (long t = (long)T.tile_begin + C.seg + (long)(1 + tid) * C.nseg; t < T.tile_end; t += (long)BLOCK * C.nseg)
+        a.partA[t] = 0.0;
+}
+
 // Finished systems in slot 1 (one block).
 __global__ __launch_bounds__(BLOCK) void k_mcount(MxArgs a) {
     __shared__ double sh[BLOCK / 64];
@@ -988,7 +1209,7 @@ __global__ __launch_bounds__(BLOCK) void k_mproject(MxArgs a, int tile) {
 
 struct MixedWork {
     int fmt = -1;
-    DevBuf<double> X0, X1, r64, xout, partR, partA, partB;
+    DevBuf<double> X0, X1, r64, xout, partR, partA, partB, sumR;
     DevBuf<unsigned char> w[3];
     DevBuf<unsigned char> e;
     DevBuf<float> scale, dinv, d, r, z, q, p0, p1;
@@ -1056,6 +1277,7 @@ static int ensure_mixed(Batch& B, int fmt) {
         w->fmt = fmt;
     }
     EAFP_TRY_RC(w->partR.alloc((size_t)B.ntiles * NRES));
+    EAFP_TRY_RC(w->sumR.alloc((size_t)B.nsys * NRES));
     EAFP_TRY_RC(w->partA.alloc((size_t)B.ntiles));
     EAFP_TRY_RC(w->partB.alloc((size_t)B.ntiles * 2));
     MxArgs& a = w->args;
@@ -1089,6 +1311,8 @@ static int ensure_mixed(Batch& B, int fmt) {
     a.p[0] = w->p0.p;
     a.p[1] = w->p1.p;
     a.partR = w->partR.p;
+    a.sumR = w->sumR.p;
+    a.geom = B.geom_d.p;
     a.partA = w->partA.p;
     a.partB = w->partB.p;
     a.dyn[0] = w->dyn0.p;
@@ -1112,7 +1336,8 @@ static void launch_mupdate_t(Batch& B, int par) {
 template <int F>
 static void launch_mspmv_f(Batch& B, int par) {
     if (B.col_split > 0) {
-        k_mspmv_col<F><<<(int)B.cols_h.size(), BLOCK, 0, B.stream>>>(B.mixed->args, par);
+        if (B.quad) k_mspmv_col<F><<<(int)B.cols_h.size(), BLOCK, 0, B.stream>>>(B.mixed->args, par);
+        else k_mspmv_colg<F><<<(int)B.cols_h.size(), BLOCK, 0, B.stream>>>(B.mixed->args, par);
         return;
     }
     if (B.quad && B.cpt >= 4) {
@@ -1151,6 +1376,7 @@ static void launch_mupdate(Batch& B, int par) {
 static void launch_refine(Batch& B) {
     const int tile = BLOCK * B.cpt;
     k_res<<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, tile);
+    k_rsum<<<B.nsys, BLOCK, 0, B.stream>>>(B.mixed->args);
     k_mreplace<<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, tile);
 }
 

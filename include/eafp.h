@@ -150,10 +150,9 @@ int eafp_batch_set_tiling(eafp_batch* b, int cells_per_thread, int blocks);
 /* The mixed solver's stencil as a column march: r_splits > 0 cuts every
  * system into columns of theta-rows (about 1024 cells per r-plane) marched
  * through the r-planes, each column in r_splits segments; 0 returns to the
- * tile kernels.  EAFP_ERR_UNSUPPORTED when a system's nz is not a multiple
- * of 4 or exceeds 256, or a system has fewer tiles than segments.  Kept
- * across eafp_batch_set_tiling while the new tiles allow it.  No reference
- * counterpart. */
+ * tile kernels.  EAFP_ERR_UNSUPPORTED when a system's nz exceeds 256 or a
+ * system has fewer tiles than segments.  Kept across eafp_batch_set_tiling
+ * while the new tiles allow it.  No reference counterpart. */
 int eafp_batch_set_columns(eafp_batch* b, int r_splits);
 
 /* Column setting in use (0: tile kernels). */

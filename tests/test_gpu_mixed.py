@@ -229,10 +229,42 @@ def test_mixed_column_kernel_ragged_with_failures():
 
 
 def test_mixed_columns_refused_when_ineligible():
-    """nz not a multiple of 4: the column setting is refused and the tile
-    kernels stay in use."""
-    ps = systems.make_system(2, 0, (10, 12, 30))
+    """nz above 256: the column setting is refused and the tile kernels stay
+    in use."""
+    ps = systems.make_system(2, 0, (4, 6, 264))
     with eafp.Batch.from_systems([ps]) as b:
         with pytest.raises(eafp.EafpError):
             b.set_columns(1)
         assert b.info()["column_splits"] == 0
+
+
+@pytest.mark.parametrize("fmt", list(FORMATS))
+@pytest.mark.parametrize("splits", [1, 4])
+def test_mixed_generic_column_kernel(splits, fmt):
+    """A batch with nz not a multiple of 4 runs the generic column kernel
+    (4 strided cells per thread, z neighbours from LDS): ragged workload-3
+    shapes, a zero-RHS and a NaN system."""
+    syss = [systems.make_system(3, s) for s in range(4)]
+    z = systems.make_system(2, 2, (6, 12, 13))
+    z.Su[...] = 0.0
+    bad = systems.make_system(2, 3, (6, 12, 13))
+    bad.Su[4, 5, 6] = np.nan
+    syss += [z, bad]
+    assert any(ps.shape[2] % 4 for ps in syss)
+    with eafp.Batch.from_systems(syss) as b:
+        b.set_tiling(1, 0)
+        b.set_columns(splits)
+        st = b.solve_mixed(tol=1e-8, max_steps=200, coef_format=FORMATS[fmt])
+        assert st[4]["status"] == "zero_rhs"
+        assert st[5]["status"] == "max_iter" and math.isnan(st[5]["residual"])
+        for s in range(4):
+            ps = syss[s]
+            phi = b.download(s)
+            tr = host_true_residual(ps, phi)
+            x_ref, _ = oracle_solution(ps, ("b3", s))
+            err = oracle.gauge_error(oracle.interior(phi), x_ref, ps.labels, ps.ncomp)
+            print(f"case generic columns splits={splits} fmt={fmt} system={s} shape={ps.shape}: "
+                  f"{st[s]['status']} inner={st[s]['n_iter']} res={tr:.2e} err={err:.2e}", flush=True)
+            assert st[s]["status"] == "converged"
+            assert tr < 1.2e-8
+            assert err < 1e-5

@@ -17,7 +17,7 @@
 // kernels (a thread takes 4 consecutive cells of a z-row): every 16-byte
 // vector access must start 16-byte aligned and lie inside its system, and
 // every cell must be covered exactly once.
-// It replays the column-marching stencil the same way (column table, quad
+// It replays the column-marching stencils (quad and generic) the same way (column table, quad
 // and halo lanes, planes i0-1 .. i1 plus the prefetch of i+1 and i+2, the
 // LDS slab indices, the partial slots) and checks that every global access
 // stays inside its system, every LDS index inside its array, every cell is
@@ -229,6 +229,66 @@ static void check_columns(std::vector<SysGeom> g, int cpt, int splits) {
     ++g_cases;
 }
 
+// The generic column kernel: slab cells tid + m*BLOCK, scalar accesses.
+static void check_columns_generic(std::vector<SysGeom> g, int cpt, int splits) {
+    const long ncells = layout_systems(g);
+    std::vector<TileDesc> tiles = make_tile_table(g, BLOCK * cpt);
+    std::vector<ColDesc> cols;
+    if (!make_col_table(g, splits, cols)) return;
+    std::vector<unsigned char> hit(ncells, 0);
+    const int SP = COL_CELLS + 2 * COL_MAX_NZ, SW = COL_CELLS + COL_MAX_NZ;
+    const int M = COL_CELLS / BLOCK, MH = 2 * COL_MAX_NZ / BLOCK;
+    for (const ColDesc& C : cols) {
+        if (C.tile < 0) continue;
+        const TileDesc T = tiles[C.tile];
+        const long lo = T.off, hi = T.off + T.n, sr = (long)T.nth * T.nz;
+        const int nz = T.nz;
+        auto base = [&](int i) { return T.off + (long)(i < T.nr ? i : T.nr - 1) * sr; };
+        auto in = [&](long a) { CHECK(a >= lo && a < hi, "generic column access %ld outside [%ld,%ld)", a, lo, hi); };
+        for (int tid = 0; tid < BLOCK; ++tid) {
+            for (int m = 0; m < M; ++m) {
+                const int c = tid + m * BLOCK;
+                const bool act = c < C.nj * nz;
+                const int cc = act ? c : 0;
+                const int row = fdiv(cc, nz, T.inv_nz), k = cc - row * nz;
+                CHECK(row >= 0 && row < C.nj && k >= 0 && k < nz, "generic slab cell %d", cc);
+                const long coff = (long)(C.j0 + row) * nz + k;
+                in(base(C.i0 > 0 ? C.i0 - 1 : C.i0) + coff);
+                in(base(C.i0) + coff);
+                in(base(C.i0 + 1) + coff);
+                for (int i = C.i0; i < C.i1; ++i) {
+                    if (i + 1 < C.i1) {
+                        in(base(i + 2) + coff);
+                        in(base(i + 1) + coff);
+                    }
+                    if (!act) continue;
+                    const int sl = (row + 1) * nz + k;
+                    CHECK(sl - nz >= 0 && sl + nz < SP && sl - 1 >= 0 && sl + 1 < SP, "generic Sp index");
+                    CHECK(sl < SW && sl - nz >= 0, "generic Sw1 index");
+                    CHECK(row * nz + k < COL_CELLS && (k == 0 || row * nz + k - 1 >= 0), "generic Sw2 index");
+                    const long o = base(i) + coff;
+                    in(o);
+                    CHECK(hit[o] == 0, "generic column cell %ld computed twice", o);
+                    hit[o] = 1;
+                }
+            }
+            for (int m = 0; m < MH; ++m) {
+                const int h = tid + m * BLOCK;
+                if (h >= 2 * nz) continue;
+                const int hrow = h < nz ? -1 : C.nj;
+                const int hk = h < nz ? h : h - nz;
+                const int hslot = (hrow + 1) * nz + hk;
+                CHECK(hslot >= 0 && hslot < SP && (hslot >= nz || hslot < SW), "generic halo slot");
+                const long hoff = (long)(((C.j0 + hrow) % T.nth + T.nth) % T.nth) * nz + hk;
+                for (int i = C.i0; i <= C.i1; ++i) in(base(i) + hoff);
+            }
+        }
+    }
+    for (const SysGeom& S : g)
+        for (long c = 0; c < S.n; ++c) CHECK(hit[S.off + c] == 1, "generic column cell %ld missed", c);
+    ++g_cases;
+}
+
 static SysGeom sys(int nr, int nth, int nz) {
     SysGeom s{};
     s.nr = nr;
@@ -247,6 +307,7 @@ int main() {
         {sys(2, 1, 300), sys(1, 257, 1), sys(5, 2, 1)},               // degenerate axes
         {sys(120, 240, 168)},                                         // the large single system
         {sys(60, 120, 84), sys(3, 1, 4), sys(1, 7, 8), sys(20, 36, 28)},   // nz multiples of 4, ragged
+        {sys(26, 54, 29), sys(18, 36, 50), sys(35, 66, 24), sys(9, 12, 256), sys(4, 5, 1)},   // any nz
     };
     const int cpts[] = {1, 2, 4, 8};
     for (auto& bt : batches) {
@@ -266,7 +327,10 @@ int main() {
                 check_shape(bt, cpt, G);
                 if (quad && cpt >= 4) check_quads(bt, cpt, G);
             }
-            for (int splits : {1, 2, 3, 4, 8}) check_columns(bt, cpt, splits);
+            for (int splits : {1, 2, 3, 4, 8}) {
+                if (quad) check_columns(bt, cpt, splits);
+                check_columns_generic(bt, cpt, splits);
+            }
         }
     }
     std::printf("ok %ld\n", g_cases);
