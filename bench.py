@@ -109,19 +109,27 @@ def _counter_rows(path):
     return rows
 
 
-def derive_traffic(fetch_path, write_path, workload, ncells, coef):
-    """HBM bytes per launch of k_mspmv and k_mupdate from two rocprofv3
-    --pmc runs of `bench.py --kernels-only` (FETCH_SIZE, then WRITE_SIZE, in
-    KiB per dispatch).  Each counter is scaled by what it reads for the
+def kernel_key(kname):
+    """The solver kernel a profiled dispatch belongs to (None: other)."""
+    for key in ("k64_spmv", "k64_update", "k_mspmv", "k_mupdate"):
+        if key in kname:
+            return key
+    return "copy" if "copyBuffer" in kname else None
+
+
+def derive_traffic(fetch_path, write_path, workload, algo):
+    """HBM bytes per launch of the solver kernels from two rocprofv3 --pmc
+    runs of `bench.py --kernels-only` (FETCH_SIZE, then WRITE_SIZE, in KiB
+    per dispatch).  Each counter is scaled by what it reads for the
     calibration copy of CALIB_BYTES (the runtime's device-to-device copy
-    kernel), so the result is in bytes the HBM actually moved."""
+    kernel), so the result is in bytes the HBM actually moved.  `algo` maps
+    each kernel key to its algorithmic bytes per launch."""
     per = {}
     for path, counter in ((fetch_path, "FETCH_SIZE"), (write_path, "WRITE_SIZE")):
         for kname, cname, val in _counter_rows(path):
             if cname != counter:
                 continue
-            key = ("copy" if "copyBuffer" in kname else
-                   "k_mspmv" if "k_mspmv" in kname else "k_mupdate" if "k_mupdate" in kname else None)
+            key = kernel_key(kname)
             if key:
                 per.setdefault((key, counter), []).append(val * 1024.0)
     calib = {}
@@ -130,10 +138,9 @@ def derive_traffic(fetch_path, write_path, workload, ncells, coef):
         if not v:
             raise RuntimeError(f"calibration copy not found in the {counter} run")
         calib[counter] = CALIB_BYTES / max(v)     # the 1 GiB copy is the largest copy dispatch
-    algo = {"k_mspmv": BYTES_MSPMV[coef] * ncells, "k_mupdate": BYTES_MUPDATE * ncells}
     rec = {"calibration": {"copy_bytes": CALIB_BYTES, "fetch_factor": round(calib["FETCH_SIZE"], 4),
                            "write_factor": round(calib["WRITE_SIZE"], 4)}}
-    for k in ("k_mspmv", "k_mupdate"):
+    for k in algo:
         f = per.get((k, "FETCH_SIZE"), [])
         w = per.get((k, "WRITE_SIZE"), [])
         if not f or not w:
@@ -215,6 +222,9 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the roofline leg: a 1 GiB device copy (counter calibration) and the timed "
                          "k_mspmv/k_mupdate launches, as measured for the bench line")
+    ap.add_argument("--single", action="store_true",
+                    help="with --kernels-only / --derive-traffic: the single 60x120x84 production system and the "
+                         "FP64 drop-in's kernels (k64_spmv, k64_update) instead of the batch")
     ap.add_argument("--derive-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write profiles/hbm_traffic.json from the rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
                          "outputs of two `bench.py --kernels-only` runs with the same options; no GPU use")
@@ -228,7 +238,13 @@ def main():
     B = args.systems
     workload = f"batch{B}_60x120x84_{args.coef}"
     if args.derive_traffic:
-        rec = derive_traffic(args.derive_traffic[0], args.derive_traffic[1], workload, B * 60 * 120 * 84, args.coef)
+        if args.single:
+            n1 = 60 * 120 * 84
+            workload, algo = "single_60x120x84_fp64", {"k64_spmv": 88 * n1, "k64_update": 64 * n1}
+        else:
+            nb = B * 60 * 120 * 84
+            algo = {"k_mspmv": BYTES_MSPMV[args.coef] * nb, "k_mupdate": BYTES_MUPDATE * nb}
+        rec = derive_traffic(args.derive_traffic[0], args.derive_traffic[1], workload, algo)
         print(json.dumps({workload: rec}, indent=1), flush=True)
         return 0
 
@@ -236,6 +252,25 @@ def main():
     from eafp import systems
     from eafp._lib import COEF_BF16, COEF_FP16, COEF_FP32
     fmt = {"fp32": COEF_FP32, "bf16": COEF_BF16, "fp16": COEF_FP16}[args.coef]
+
+    if args.kernels_only and args.single:
+        # the drop-in's two FP64 kernels on the production system
+        ps = systems.make_system(2)
+        single = eafp.Batch.from_systems([ps])
+        if args.tiling:
+            shape = [int(v) for v in args.tiling.split(",")]
+            single.set_tiling(shape[0], shape[1])
+        else:
+            single.autotune(eafp.Batch.SOLVER_FP64, reps=5)
+        calibration_copy(CALIB_BYTES)
+        ms_sp, ms_up = single.time_kernels(eafp.Batch.SOLVER_FP64, reps=20)
+        out = {"kernels_only": True, "workload": "single 60x120x84 production system, FP64 drop-in kernels",
+               "launch_shape": single.info(), "k64_spmv_avg_us": round(ms_sp * 1e3, 2),
+               "k64_update_avg_us": round(ms_up * 1e3, 2), "k64_spmv_algorithmic_bytes": 88 * ps.ncells,
+               "k64_update_algorithmic_bytes": 64 * ps.ncells, "calibration_copy_bytes": CALIB_BYTES}
+        single.close()
+        print(json.dumps(out), flush=True)
+        return 0
 
     t0 = time.time()
     syss = [systems.make_system(5, s) for s in range(B)]
@@ -395,6 +430,16 @@ def main():
                                              f"60x120x84 production system, {tp:.2f} s on {threads} host threads",
                                    "one_core": {"value": round(1.0 / tc, 5), "seconds": round(tc, 2),
                                                 "n_iter": nit_c}}
+            ps1 = systems.make_system(1)
+            t = time.perf_counter()
+            _, _, nit1p = oracle.cg_omp(ps1, max_iter=100000, tol=1e-8)
+            t1p = time.perf_counter() - t
+            t = time.perf_counter()
+            _, _, nit1, _ = oracle.cg(ps1, max_iter=100000, tol=1e-8)
+            t1c = time.perf_counter() - t
+            out["cpu_baseline"]["config_12x24x16"] = {"threads_solves_per_s": round(1.0 / t1p, 2),
+                                                      "one_core_solves_per_s": round(1.0 / t1c, 2),
+                                                      "n_iter": nit1}
             out["mixed_single"]["rel_err_vs_cpu_oracle"] = rel_err(xmx, xc, ps)
             out["fp64_single"]["rel_err_vs_cpu_oracle"] = rel_err(x64, xc, ps)
     batch.close()
