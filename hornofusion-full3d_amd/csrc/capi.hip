@@ -69,9 +69,19 @@ int eafp_cg_solve(const eafp_desc* desc, const double* aW, const double* aE, con
 }
 
 int eafp_release(void) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    g_cache.reset();
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_cache.reset();
+    }
+    eafp::pp_release();
     return EAFP_OK;
+}
+
+int eafp_pp_assemble(const eafp_desc* desc, const eafp_pp_fields* fields, const eafp_pp_config* cfg, double* aW,
+                     double* aE, double* aS, double* aN, double* aB, double* aT, double* aP, double* Su,
+                     double* residual, double* flux_ref, int arrays_on_device) {
+    double* out[8] = {aW, aE, aS, aN, aB, aT, aP, Su};
+    return eafp::pp_assemble(desc, fields, cfg, out, residual, flux_ref, arrays_on_device);
 }
 
 int eafp_batch_create(int nsys, const eafp_desc* descs, eafp_batch** out) {

@@ -25,11 +25,29 @@ COEF_FP32, COEF_BF16, COEF_FP16 = 0, 1, 2
 
 # Every symbol declared in include/eafp.h.
 EXPORTS = (
-    "eafp_cg_solve", "eafp_release", "eafp_batch_create", "eafp_batch_upload",
+    "eafp_cg_solve", "eafp_release", "eafp_pp_assemble", "eafp_batch_create", "eafp_batch_upload",
     "eafp_batch_solve_fp64", "eafp_batch_solve_mixed", "eafp_batch_download",
     "eafp_batch_info", "eafp_batch_set_tiling", "eafp_batch_set_columns", "eafp_batch_get_columns",
     "eafp_batch_time_kernels", "eafp_batch_autotune", "eafp_batch_destroy", "eafp_version",
 )
+
+
+PP_FIELDS = ("r", "theta", "z", "vol", "Ar", "Ath", "Az", "cell_type", "outlet",
+             "alpha_l", "ur_l", "uth_l", "uz_l", "aPr_l", "aPth_l", "aPz_l", "rho_l", "T_l",
+             "alpha_g", "ur_g", "uth_g", "uz_g", "aPr_g", "aPth_g", "aPz_g", "rho_g", "T_old_g", "p",
+             "liq_cont", "pv_active", "gas_reach", "liq_room")
+PP_INT = ("cell_type", "outlet", "liq_cont", "pv_active", "gas_reach")
+
+
+class PPFields(ctypes.Structure):
+    """eafp_pp_fields: one pointer per field, in the header's order."""
+    _fields_ = [(n, ctypes.c_void_p) for n in PP_FIELDS]
+
+
+class PPConfig(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("rho_gas", ctypes.c_double), ("T_ambient", ctypes.c_double),
+                ("beta_expansion", ctypes.c_double), ("gas_in_poisson", ctypes.c_int),
+                ("solve_multiphase", ctypes.c_int), ("gas_compressibility", ctypes.c_int), ("pad_", ctypes.c_int)]
 
 
 class EafpError(RuntimeError):
@@ -98,6 +116,9 @@ def load():
     ip = ctypes.POINTER(i)
     L.eafp_batch_set_tiling.argtypes = [vp, i, i]
     L.eafp_batch_set_tiling.restype = i
+    L.eafp_pp_assemble.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(PPFields), ctypes.POINTER(PPConfig)] + \
+        [vp] * 8 + [_dp, _dp, i]
+    L.eafp_pp_assemble.restype = i
     L.eafp_batch_set_columns.argtypes = [vp, i]
     L.eafp_batch_set_columns.restype = i
     L.eafp_batch_get_columns.argtypes = [vp, ip]

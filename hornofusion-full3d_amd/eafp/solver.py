@@ -76,6 +76,38 @@ def cg_3d_mpi(aW, aE, aS, aN, aB, aT, aP, Su, phi, m, max_iter, tol):
     return res.value, nit.value
 
 
+def pp_assemble(F, cfg, m):
+    """The p' system assembled on the GPU (eafp_pp_assemble) from the fields
+    of ``F`` (padded Fortran arrays, names as in ``_lib.PP_FIELDS``; absent
+    workspace masks are not valid) and the config dict ``cfg``.  Returns
+    ``(coef, residual, flux_ref)`` with ``coef`` mapping aW..aT, aP, Su to new
+    padded arrays."""
+    L = _lib.load()
+    shape = (m.nr, m.ntheta, m.nz)
+    keep = []
+    vals = {}
+    for n in _lib.PP_FIELDS:
+        a = F.get(n)
+        if a is None:
+            vals[n] = None
+            continue
+        a = np.asfortranarray(a, dtype=np.int32 if n in _lib.PP_INT else np.float64)
+        keep.append(a)
+        vals[n] = a.ctypes.data
+    f = _lib.PPFields(**vals)
+    c = _lib.PPConfig(float(cfg["dt"]), float(cfg["rho_gas"]), float(cfg["T_ambient"]), float(cfg["beta_expansion"]),
+                      int(cfg["gas_in_poisson"]), int(cfg["solve_multiphase"]), int(cfg["gas_compressibility"]), 0)
+    names = ("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su")
+    out = {n: np.zeros((shape[0] + 4, shape[1] + 4, shape[2] + 4), order="F") for n in names}
+    res = ctypes.c_double()
+    flux = ctypes.c_double()
+    d = desc_of(m)
+    check(L.eafp_pp_assemble(ctypes.byref(d), ctypes.byref(f), ctypes.byref(c), *[out[n].ctypes.data for n in names],
+                             ctypes.byref(res), ctypes.byref(flux), 0), "eafp_pp_assemble")
+    del keep
+    return out, res.value, flux.value
+
+
 def release():
     _lib.load().eafp_release()
 

@@ -96,9 +96,63 @@ int eafp_cg_solve(const eafp_desc* desc,
                   const double* aB, const double* aT, const double* aP, const double* Su,
                   double* phi, int max_iter, double tol, double* residual, int* n_iter);
 
-/* Releases the device buffers cached by eafp_cg_solve.  No reference
- * counterpart (the reference's workspace lives until exit). */
+/* Releases the device buffers cached by eafp_cg_solve and
+ * eafp_pp_assemble.  No reference counterpart (the reference's workspace
+ * lives until exit). */
 int eafp_release(void);
+
+/* ---------------------------------------------------------------------------
+ * The p' system assembled on the GPU from the phase fields: the work that
+ * solve_pressure_correction (src/mod_pressure_3d.f90:88-306) does before
+ * its cg_3d_mpi call, for one serial mesh.  It covers:
+ *   - the active phases (:127-169);
+ *   - the cell pressure gradients (cell_gradients :612-651);
+ *   - the Rhie-Chow face terms of the liquid and, with gas_in_poisson and
+ *     solve_multiphase, of the gas (add_phase_contribution :469-610);
+ *   - the gas compressibility source (:226-246);
+ *   - the compliance and acoustic diagonal, or the trivial rows (:249-281);
+ *   - the pressure boundary rows (apply_pressure_bc,
+ *     src/mod_boundary_3d.f90:180-277);
+ *   - the continuity residual sum|Su| / flux_ref (:290-306).
+ * Every 3-D array is padded Fortran order with bounds (-1:n+2); r, theta, z
+ * are the cell-centre coordinates with bounds (-1:n+2) (the mesh_t fields
+ * m%r, m%theta, m%z); the face-area arrays index faces as the reference
+ * does (Ar(i) the face i+1/2, Az(k) the face k+1/2).  cell_type follows
+ * mesh_t (0 outside the fluid and in every r/z halo); outlet is the top
+ * layer's electrode mask, (nr+4)*(ntheta+4) in the same order
+ * (is_electrode(:,:,nz,:) of any electrode).  The four workspace masks are
+ * optional: NULL means "not valid" (ws_liq_cont_valid, ws_pv_valid,
+ * ws_gas_perc_valid, ws_liq_room_valid false).  Integer masks are 0/1.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    const double *r, *theta, *z;                /* m%r, m%theta, m%z: (-1:n+2) */
+    const double *vol, *Ar, *Ath, *Az;          /* m%vol, m%Ar, m%Ath, m%Az */
+    const int* cell_type;                       /* m%cell_type */
+    const int* outlet;                          /* any(m%is_electrode(:,:,nz,:)) */
+    const double *alpha_l, *ur_l, *uth_l, *uz_l, *aPr_l, *aPth_l, *aPz_l, *rho_l, *T_l;        /* liq */
+    const double *alpha_g, *ur_g, *uth_g, *uz_g, *aPr_g, *aPth_g, *aPz_g, *rho_g, *T_old_g;    /* gas, gas_T_old */
+    const double* p;                            /* sh%p */
+    const int *liq_cont, *pv_active, *gas_reach;   /* ws_liq_cont, ws_pv_active, ws_gas_reach */
+    const double* liq_room;                     /* ws_liq_room */
+} eafp_pp_fields;
+
+typedef struct {
+    double dt, rho_gas, T_ambient, beta_expansion;               /* cfg%dt, cfg%rho_gas, ... */
+    int gas_in_poisson, solve_multiphase, gas_compressibility;   /* cfg flags (0/1) */
+    int pad_;
+} eafp_pp_config;
+
+/* Writes the assembled aW..aT, aP, Su (padded, halos zero) and the
+ * continuity residual and its normaliser flux_ref (either may be NULL).
+ * arrays_on_device = 1: every array pointer, inputs and outputs, is a device
+ * pointer (nothing crosses PCIe).  The coefficients are bit-identical to a
+ * sequential FP64 evaluation of the reference's expressions in their written
+ * order without contraction; the two global sums of the residual are
+ * reduced in a different order.  Serial meshes only (EAFP_ERR_UNSUPPORTED
+ * when desc->is_parallel). */
+int eafp_pp_assemble(const eafp_desc* desc, const eafp_pp_fields* fields, const eafp_pp_config* cfg,
+                     double* aW, double* aE, double* aS, double* aN, double* aB, double* aT,
+                     double* aP, double* Su, double* residual, double* flux_ref, int arrays_on_device);
 
 /* ---------------------------------------------------------------------------
  * Device-resident systems.  A batch holds nsys systems of any shapes; nsys

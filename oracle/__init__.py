@@ -10,6 +10,9 @@ Contents
   ``matvec`` (``:588-605``), ``dot`` (``:607-618``) and the serial theta halo
   copy (``src/mod_mpi_topology.f90:303-310``), on the reference's padded
   Fortran layout (bounds ``(-1:n+2)``, r fastest).
+* ``pp_assemble``: ctypes binding to ``oracle_pp_assemble`` (``pp_assemble.c``),
+  the restatement of the p' system assembly of ``solve_pressure_correction``
+  (``src/mod_pressure_3d.f90:171-306``, ``src/mod_boundary_3d.f90:180-277``).
 * numpy helpers that are independent of the C code and are used to pin it:
   a scipy CSR operator built from the interior coefficient arrays, the
   per-component zero-mean projection, and gauge-fixed error metrics.
@@ -66,6 +69,8 @@ def lib():
         L.oracle_dot.restype = ctypes.c_double
         L.oracle_true_residual.argtypes = [i, i, i] + [_dp] * 9 + [_dp, _dp]
         L.oracle_true_residual.restype = None
+        L.oracle_pp_assemble.argtypes = [i, i, i, ctypes.c_void_p, ctypes.c_void_p] + [_dp] * 8 + [_dp, _dp]
+        L.oracle_pp_assemble.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -83,6 +88,59 @@ def lib_omp():
         L.oracle_cg_omp.restype = ctypes.c_int
         _omp = L
     return _omp
+
+
+PP_FIELDS = ("r", "theta", "z", "vol", "Ar", "Ath", "Az", "cell_type", "outlet",
+             "alpha_l", "ur_l", "uth_l", "uz_l", "aPr_l", "aPth_l", "aPz_l", "rho_l", "T_l",
+             "alpha_g", "ur_g", "uth_g", "uz_g", "aPr_g", "aPth_g", "aPz_g", "rho_g", "T_old_g", "p",
+             "liq_cont", "pv_active", "gas_reach", "liq_room")
+
+
+class PPFields(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in PP_FIELDS]
+
+
+class PPConfig(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("rho_gas", ctypes.c_double), ("T_ambient", ctypes.c_double),
+                ("beta_expansion", ctypes.c_double), ("gas_in_poisson", ctypes.c_int),
+                ("solve_multiphase", ctypes.c_int), ("gas_compressibility", ctypes.c_int), ("pad", ctypes.c_int)]
+
+
+def pp_structs(F, cfg):
+    """ctypes (PPFields, PPConfig) over the arrays of ``F`` (absent masks: NULL)."""
+    keep = []
+    vals = {}
+    for n in PP_FIELDS:
+        a = F.get(n)
+        if a is None:
+            vals[n] = None
+            continue
+        want = np.int32 if n in ("cell_type", "outlet", "liq_cont", "pv_active", "gas_reach") else np.float64
+        a = np.asfortranarray(a, dtype=want)
+        keep.append(a)
+        vals[n] = a.ctypes.data
+    f = PPFields(**vals)
+    c = PPConfig(float(cfg["dt"]), float(cfg["rho_gas"]), float(cfg["T_ambient"]), float(cfg["beta_expansion"]),
+                 int(cfg["gas_in_poisson"]), int(cfg["solve_multiphase"]), int(cfg["gas_compressibility"]), 0)
+    return f, c, keep
+
+
+def pp_assemble(F, cfg, shape):
+    """The restated p' assembly.  Returns (coef dict with aW..aP and Su as
+    padded arrays, residual, flux_ref)."""
+    nr, nth, nz = shape
+    f, c, keep = pp_structs(F, cfg)
+    outs = {n: np.zeros((nr + 4, nth + 4, nz + 4), order="F") for n in ("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su")}
+    res = ctypes.c_double()
+    flux = ctypes.c_double()
+    L = lib()
+    rc = L.oracle_pp_assemble(nr, nth, nz, ctypes.byref(f), ctypes.byref(c),
+                              *[outs[n].ctypes.data_as(_dp) for n in ("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su")],
+                              ctypes.byref(res), ctypes.byref(flux))
+    del keep
+    if rc != 0:
+        raise MemoryError("oracle_pp_assemble could not allocate its work arrays")
+    return outs, res.value, flux.value
 
 
 def _p(a: np.ndarray):

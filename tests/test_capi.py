@@ -84,3 +84,29 @@ def test_layout_checked():
     m = eafp.Mesh(4, 6, 5)
     with pytest.raises(ValueError):
         eafp.cg_3d_mpi(*a[:8], a[8], m, 10, 1e-8)
+
+
+def test_pp_struct_layouts():
+    # eafp_pp_fields: 32 pointers; eafp_pp_config: 4 doubles, 4 ints
+    assert ctypes.sizeof(_lib.PPFields) == 32 * 8
+    assert ctypes.sizeof(_lib.PPConfig) == 48
+    import oracle
+    assert [f[0] for f in oracle.PPFields._fields_] == [f[0] for f in _lib.PPFields._fields_]
+    assert ctypes.sizeof(oracle.PPConfig) == ctypes.sizeof(_lib.PPConfig)
+
+
+def test_pp_assemble_rejects_bad_calls():
+    """Argument checks run before any device call."""
+    from eafp import fields
+    F, cfg = fields.make_fields(4, 6, 5)
+    with pytest.raises(_lib.EafpError) as e:
+        eafp.pp_assemble(F, cfg, eafp.Mesh(4, 6, 5, is_parallel=True))
+    assert e.value.rc == _lib.EAFP_ERR_UNSUPPORTED
+    G = dict(F)
+    del G["rho_l"]
+    with pytest.raises(_lib.EafpError) as e:
+        eafp.pp_assemble(G, cfg, eafp.Mesh(4, 6, 5))
+    assert e.value.rc == _lib.EAFP_ERR_ARG
+    with pytest.raises(_lib.EafpError) as e:
+        eafp.pp_assemble(F, dict(cfg, dt=0.0), eafp.Mesh(4, 6, 5))
+    assert e.value.rc == _lib.EAFP_ERR_ARG
