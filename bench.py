@@ -197,6 +197,93 @@ def measure_workload(eafp, syss, fmt, coef, interval, label):
     return rec
 
 
+def measure_assembly(eafp, shape=(60, 120, 84), reps=20):
+    """The p' system assembled on the GPU (eafp_pp_assemble, with the
+    liquid's face exports) and the post-solve consumers (eafp_pp_correct)
+    on two-phase fields at 60x120x84: device-resident (every array already
+    in HBM, the case of a GPU-resident caller) and, for the assembly, from
+    host arrays (PCIe copies included), beside the oracle's single-core
+    times."""
+    import ctypes
+    from eafp import _lib, fields
+    F, cfg = fields.make_fields(*shape, seed=11, masks=True)
+    m = eafp.Mesh(*shape)
+    eafp.pp_assemble(F, cfg, m)
+    t = time.perf_counter()
+    for _ in range(reps):
+        out, res, flux = eafp.pp_assemble(F, cfg, m)
+    t_host = (time.perf_counter() - t) / reps
+    # device-resident: every input and output in HBM
+    hip = ctypes.CDLL("libamdhip64.so")
+    bufs = []
+
+    def dev(a=None, nbytes=None):
+        ptr = ctypes.c_void_p()
+        nb = a.nbytes if a is not None else nbytes
+        _lib.check(hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(nb)), "hipMalloc")
+        if a is not None:
+            _lib.check(hip.hipMemcpy(ptr, ctypes.c_void_p(a.ctypes.data), ctypes.c_size_t(nb), 1), "hipMemcpy")
+        bufs.append(ptr)
+        return ptr.value
+
+    vals = {}
+    for n in _lib.PP_FIELDS:
+        a = F.get(n)
+        vals[n] = None if a is None else dev(np.asfortranarray(a, dtype=np.int32 if n in _lib.PP_INT else np.float64))
+    npad = (shape[0] + 4) * (shape[1] + 4) * (shape[2] + 4)
+    outs = [dev(nbytes=8 * npad) for _ in range(8)]
+    exs = _lib.PPExports(**{n: dev(nbytes=(4 if n.startswith("lk") else 8) * npad) for n in _lib.PP_EXPORTS})
+    f = _lib.PPFields(**vals)
+    c = _lib.PPConfig(cfg["dt"], cfg["rho_gas"], cfg["T_ambient"], cfg["beta_expansion"],
+                      cfg["gas_in_poisson"], cfg["solve_multiphase"], cfg["gas_compressibility"], 0)
+    L = _lib.load()
+    d = eafp.solver.desc_of(m)
+    r_ = ctypes.c_double()
+    fl = ctypes.c_double()
+    call = lambda: _lib.check(L.eafp_pp_assemble(ctypes.byref(d), ctypes.byref(f), ctypes.byref(c), *outs,
+                                                  ctypes.byref(r_), ctypes.byref(fl), ctypes.byref(exs), 1),
+                              "eafp_pp_assemble")
+    call()
+    t = time.perf_counter()
+    for _ in range(reps):
+        call()
+    t_dev = (time.perf_counter() - t) / reps
+    # post-solve consumers on a smooth correction field (theta-periodic)
+    nr, nth, nz = shape
+    ii = np.arange(-1, nr + 3)[:, None, None]
+    jj = np.arange(-1, nth + 3)[None, :, None]
+    kk = np.arange(-1, nz + 3)[None, None, :]
+    pp = np.asfortranarray(50.0 * np.sin(0.1 * ii) * np.cos(2 * np.pi * jj / nth) + 3.0 * kk)
+    upd = {n: dev(nbytes=(4 if n.startswith("lk") else 8) * npad) for n in _lib.PP_UPDATE[:6]}
+    upd.update({n: dev(np.asfortranarray(F[n], dtype=np.float64)) for n in _lib.PP_UPDATE[6:]})
+    io = _lib.PPUpdate(**upd)
+    pp_d = dev(pp)
+    corr = lambda: _lib.check(L.eafp_pp_correct(ctypes.byref(d), ctypes.byref(f), ctypes.byref(c), 0.3, pp_d,
+                                                 ctypes.byref(exs), ctypes.byref(io), 1), "eafp_pp_correct")
+    corr()
+    t = time.perf_counter()
+    for _ in range(reps):
+        corr()
+    t_corr = (time.perf_counter() - t) / reps
+    for ptr in bufs:
+        hip.hipFree(ptr)
+    rec = {"workload": "two-phase 60x120x84 fields (bath, gas, scrap band, workspace masks) -> aW..aP, Su, "
+                       "face exports; then fluxes, velocities, p",
+           "device_resident_us_per_call": round(t_dev * 1e6, 1), "host_arrays_ms_per_call": round(t_host * 1e3, 3),
+           "post_solve_device_resident_us_per_call": round(t_corr * 1e6, 1),
+           "continuity_residual": res}
+    sys.path.insert(0, ROOT)
+    import oracle
+    oracle.lib()
+    t = time.perf_counter()
+    _, _, _, ex = oracle.pp_assemble(F, cfg, shape, exports=True)
+    rec["cpu_oracle_ms_1_core"] = round((time.perf_counter() - t) * 1e3, 2)
+    t = time.perf_counter()
+    oracle.pp_correct(F, cfg, shape, 0.3, pp, ex)
+    rec["post_solve_cpu_oracle_ms_1_core"] = round((time.perf_counter() - t) * 1e3, 2)
+    return rec
+
+
 def rel_err(x, xr, ps):
     from eafp.systems import project_zero_mean
     px = project_zero_mean(x, ps.labels, ps.ncomp)
@@ -445,6 +532,8 @@ def main():
     batch.close()
 
     if not args.no_extras:
+        out["pp_assembly"] = measure_assembly(eafp)
+        log("p' assembly done")
         # the other workloads of the metric's table, mixed against FP64 on the same GPU
         out["other_workloads"] = []
         syss3 = [systems.make_system(3, s) for s in range(256)]

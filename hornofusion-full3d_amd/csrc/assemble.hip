@@ -63,20 +63,25 @@ struct PPArgs {
     double *gr, *gth, *gz, *gzg;
     double *aW, *aE, *aS, *aN, *aB, *aT, *aP, *Su;
     double* part;            // 2 per block: flux_ref, sum|Su|
+    double *Fs[3], *ac[3];   // liquid +face exports (r, theta, z); Fs[0] == nullptr: none
+    int* lk[3];
 };
 
-__device__ __forceinline__ bool act_l(const PPArgs& a, long o) {
+template <class A>
+__device__ __forceinline__ bool act_l(const A& a, long o) {
     return a.liq_cont ? a.liq_cont[o] != 0 : a.al[o] >= ALPHA_FLOW_CUTOFF;
 }
 
-__device__ __forceinline__ bool act_g(const PPArgs& a, long o) {
+template <class A>
+__device__ __forceinline__ bool act_g(const A& a, long o) {
     bool g = a.ag[o] >= ALPHA_FLOW_CUTOFF;
     if (a.pv_active) g = g && a.pv_active[o] != 0;
     if (a.gas_reach) g = g || (a.gas_reach[o] != 0 && a.ag[o] < ALPHA_FLOW_CUTOFF && a.ct[o] != 0);
     return g;
 }
 
-__device__ __forceinline__ bool pv_cell(const PPArgs& a, long o) { return a.ct[o] != 0 && (act_l(a, o) || act_g(a, o)); }
+template <class A>
+__device__ __forceinline__ bool pv_cell(const A& a, long o) { return a.ct[o] != 0 && (act_l(a, o) || act_g(a, o)); }
 
 __device__ __forceinline__ double pgrad(double pm, double p0, double pp_, double xm, double x0, double xp, bool okm,
                                         bool okp) {
@@ -97,7 +102,8 @@ struct Pos {
     int i, j, k;
 };
 
-__device__ __forceinline__ Pos padded_pos(const PPArgs& a, long o) {
+template <class A>
+__device__ __forceinline__ Pos padded_pos(const A& a, long o) {
     const int k = (int)(o / a.sz);
     const long rem = o - (long)k * a.sz;
     const int j = (int)(rem / a.sy);
@@ -105,7 +111,8 @@ __device__ __forceinline__ Pos padded_pos(const PPArgs& a, long o) {
     return {i - 1, j - 1, k - 1};
 }
 
-__device__ __forceinline__ long at(const PPArgs& a, int i, int j, int k) {
+template <class A>
+__device__ __forceinline__ long at(const A& a, int i, int j, int k) {
     return (long)(i + 1) + a.sy * (j + 1) + a.sz * (k + 1);
 }
 
@@ -155,6 +162,8 @@ __device__ __forceinline__ bool face_open(const PPArgs& a, bool isg, double u_f,
 
 struct Acc {
     double aW, aE, aS, aN, aB, aT, Su, flux;
+    double Fs[3], ac[3];     // the liquid's +face exports of this cell
+    int lk[3];
 };
 
 // The six faces of cell o for one phase (src/mod_pressure_3d.f90:486-607).
@@ -194,6 +203,11 @@ __device__ void add_phase(const PPArgs& a, const Phase& ph, int i, int j, int k,
             c.aE = c.aE + af * d_f * a.Ar[o] / delta;
             c.Su = c.Su - af * u_f * a.Ar[o];
             c.flux += fabs(af * u_f * a.Ar[o]);
+            if (!gas) {
+                c.Fs[0] = af * u_f * a.Ar[o];
+                c.ac[0] = af * d_f * a.Ar[o] / delta;
+                c.lk[0] = 1;
+            }
         }
     }
     if (link(oS)) {
@@ -216,6 +230,11 @@ __device__ void add_phase(const PPArgs& a, const Phase& ph, int i, int j, int k,
             c.aN = c.aN + af * d_f * a.Ath[o] / delta;
             c.Su = c.Su - af * u_f * a.Ath[o];
             c.flux += fabs(af * u_f * a.Ath[o]);
+            if (!gas) {
+                c.Fs[1] = af * u_f * a.Ath[o];
+                c.ac[1] = af * d_f * a.Ath[o] / delta;
+                c.lk[1] = 1;
+            }
         }
     }
     if (link(oB)) {
@@ -240,6 +259,11 @@ __device__ void add_phase(const PPArgs& a, const Phase& ph, int i, int j, int k,
             c.aT = c.aT + af * d_f * a.Az[o] / delta;
             c.Su = c.Su - af * u_f * a.Az[o];
             c.flux += fabs(af * u_f * a.Az[o]);
+            if (!gas) {
+                c.Fs[2] = af * u_f * a.Az[o];
+                c.ac[2] = af * d_f * a.Az[o] / delta;
+                c.lk[2] = 1;
+            }
         }
     }
 }
@@ -252,7 +276,7 @@ __global__ __launch_bounds__(BLOCK) void k_pp_assemble(PPArgs a) {
         const Pos q = padded_pos(a, o);
         const int i = q.i, j = q.j, k = q.k;
         const bool interior = i >= 1 && i <= a.nr && j >= 1 && j <= a.nth && k >= 1 && k <= a.nz;
-        Acc c = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        Acc c = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0, 0, 0}};
         double aP = 0.0;
         if (interior && a.ct[o] != 0) {
             const bool al = act_l(a, o), ag = act_g(a, o);
@@ -320,6 +344,14 @@ __global__ __launch_bounds__(BLOCK) void k_pp_assemble(PPArgs a) {
         a.aT[o] = c.aT;
         a.aP[o] = aP;
         a.Su[o] = c.Su;
+        if (a.Fs[0]) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                a.Fs[d][o] = c.Fs[d];
+                a.ac[d][o] = c.ac[d];
+                a.lk[d][o] = c.lk[d];
+            }
+        }
     }
     block_sum<2>(acc, sh);
     if (threadIdx.x == 0) {
@@ -328,11 +360,133 @@ __global__ __launch_bounds__(BLOCK) void k_pp_assemble(PPArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Post-solve consumers of p' (src/mod_pressure_3d.f90:324-397 and
+// correct_velocities :689-746), one thread per padded cell:
+//   k_pp_correct  the conservative liquid fluxes of the E/N/T faces and
+//                 their link flags (theta halo cells take the value of their
+//                 periodic source, which is what the serial halo copy leaves
+//                 there), the velocity corrections of both phases in place,
+//                 and p + alpha_p pp clamped to +-P_HYDRO_CAP into a scratch
+//                 array
+//   k_pp_seal     the final pressure: sealed cells (fluid, no phase active)
+//                 take the mean of their neighbours with an active phase,
+//                 theta halo cells the value of their periodic source
+// Neither kernel reduces anything, so every output is the double a
+// sequential evaluation gives.  Bytes per padded cell: fluxes 6 double + 3
+// int reads, 3 double + 3 int writes; velocities 2 x (3 aP + 3 u) reads and
+// 3 u writes; pressure 2 reads, 1 write, then 1 read + 1 write.
+constexpr double P_HYDRO_CAP = 2.0e6;          // src/mod_pressure_3d.f90:74
+
+struct PCArgs {
+    int nr, nth, nz;
+    long sy, sz, n;
+    const double *r, *th, *z, *vol;
+    const int* ct;
+    const double *al, *ag;
+    const double* aP[6];       // aPr, aPth, aPz of the liquid, then of the gas
+    const int *liq_cont, *pv_active, *gas_reach;
+    const double* pp;
+    const double *Fs[3], *ac[3];
+    const int* lk_in[3];
+    double* Fc[3];
+    int* lk[3];
+    double* vel[6];            // ur, uth, uz of the liquid, then of the gas (in place)
+    const double* p;           // the pressure before the update
+    double* p_upd;             // p + alpha_p pp, clamped (scratch)
+    double* p_out;             // the final pressure (may be p itself)
+    double alpha_p;
+    int two_phase;
+};
+
+// Periodic source plane of theta plane j (j itself for 1 <= j <= nth).  With
+// nth = 1 the serial copy's order leaves plane 1 in every halo plane.
+__device__ __forceinline__ int theta_src(int j, int nth) {
+    int s = (j - 1) % nth;
+    if (s < 0) s += nth;
+    return s + 1;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_pp_correct(PCArgs a) {
+    const long o = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (o >= a.n) return;
+    const Pos q = padded_pos(a, o);
+    const int i = q.i, j = q.j, k = q.k;
+    const bool rz = i >= 1 && i <= a.nr && k >= 1 && k <= a.nz;
+    const int js = theta_src(j, a.nth);
+    const long s = at(a, i, js, k);
+    const bool fluid = rz && a.ct[s] != 0;
+    // conservative fluxes (:333-351)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int lk = a.lk_in[d][s];
+        double fc = 0.0;
+        if (fluid && lk == 1) {
+            const long nb = d == 0 ? at(a, i + 1, js, k) : d == 1 ? at(a, i, js + 1, k) : at(a, i, js, k + 1);
+            fc = a.Fs[d][s] + a.ac[d][s] * (a.pp[s] - a.pp[nb]);
+        }
+        a.Fc[d][o] = fc;
+        a.lk[d][o] = lk;
+    }
+    // correct_velocities (:689-746): the liquid, then the gas
+    if (fluid && s == o) {
+        const long oW = at(a, i - 1, j, k), oE = at(a, i + 1, j, k);
+        const long oS = at(a, i, j - 1, k), oN = at(a, i, j + 1, k);
+        const long oB = at(a, i, j, k - 1), oT = at(a, i, j, k + 1);
+        for (int ph = 0; ph < (a.two_phase ? 2 : 1); ++ph) {
+            if (!(ph ? act_g(a, o) : act_l(a, o))) continue;
+            const double* alpha = ph ? a.ag : a.al;
+            const double *aPr = a.aP[3 * ph], *aPth = a.aP[3 * ph + 1], *aPz = a.aP[3 * ph + 2];
+            double *ur = a.vel[3 * ph], *uth = a.vel[3 * ph + 1], *uz = a.vel[3 * ph + 2];
+            if (!(i == 1 || i == a.nr) && fabs(aPr[o]) > SMALL) {
+                const double dc = alpha[o] * a.vol[o] / aPr[o];
+                ur[o] = ur[o] - dc * (a.pp[oE] - a.pp[oW]) / (a.r[i + 2] - a.r[i]);
+            }
+            if (fabs(aPth[o]) > SMALL) {
+                const double dc = alpha[o] * a.vol[o] / aPth[o];
+                uth[o] = uth[o] - dc * (a.pp[oN] - a.pp[oS]) / (a.r[i + 1] * (a.th[j + 2] - a.th[j]));
+            }
+            if (!(k == 1 || k == a.nz) && fabs(aPz[o]) > SMALL) {
+                const double dc = alpha[o] * a.vol[o] / aPz[o];
+                uz[o] = uz[o] - dc * (a.pp[oT] - a.pp[oB]) / (a.z[k + 2] - a.z[k]);
+            }
+        }
+    }
+    // pressure update over the whole array (:358-360)
+    a.p_upd[o] = fmax(-P_HYDRO_CAP, fmin(P_HYDRO_CAP, a.p[o] + a.alpha_p * a.pp[o]));
+}
+
+__global__ __launch_bounds__(BLOCK) void k_pp_seal(PCArgs a) {
+    const long o = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (o >= a.n) return;
+    const Pos q = padded_pos(a, o);
+    const int i = q.i, k = q.k;
+    const int js = theta_src(q.j, a.nth);
+    const long s = at(a, i, js, k);
+    double v = a.p_upd[s];
+    if (i >= 1 && i <= a.nr && k >= 1 && k <= a.nz && a.ct[s] != 0 && !act_l(a, s) && !act_g(a, s)) {
+        // sealed cell (:373-397): the neighbours W, E, S, N, B, T in this order
+        const long nb[6] = {at(a, i - 1, js, k), at(a, i + 1, js, k), at(a, i, js - 1, k),
+                            at(a, i, js + 1, k), at(a, i, js, k - 1), at(a, i, js, k + 1)};
+        double psum = 0.0;
+        int nnb = 0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            const long b = nb[m];
+            if (a.ct[b] == 0 || (!act_l(a, b) && !act_g(a, b))) continue;
+            psum = psum + a.p_upd[b];
+            nnb = nnb + 1;
+        }
+        v = nnb > 0 ? psum / (double)nnb : 0.0;
+    }
+    a.p_out[o] = v;
+}
+
 struct PPWork {
     int nr = -1, nth = -1, nz = -1;
     hipStream_t stream = nullptr;
-    DevBuf<double> coord, f3, grads, outs, part, room;
-    DevBuf<int> ct, outlet, masks;
+    DevBuf<double> coord, f3, grads, outs, part, room, exd;
+    DevBuf<int> ct, outlet, masks, lki;
     std::vector<double> part_h;
 };
 
@@ -353,6 +507,8 @@ int ensure_pp(const eafp_desc* d) {
     EAFP_TRY_RC(w.ct.alloc((size_t)n));
     EAFP_TRY_RC(w.masks.alloc((size_t)n * 3));
     EAFP_TRY_RC(w.outlet.alloc((size_t)(d->nr + 4) * (d->ntheta + 4)));
+    EAFP_TRY_RC(w.exd.alloc((size_t)n * 6));
+    EAFP_TRY_RC(w.lki.alloc((size_t)n * 6));
     const long blocks = (n + BLOCK - 1) / BLOCK;
     EAFP_TRY_RC(w.part.alloc((size_t)blocks * 2));
     w.part_h.assign((size_t)blocks * 2, 0.0);
@@ -365,11 +521,21 @@ int ensure_pp(const eafp_desc* d) {
 }  // namespace
 
 int pp_assemble(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config* cfg, double* const out[8],
-                double* residual, double* flux_ref, int on_device) {
+                double* residual, double* flux_ref, const eafp_pp_exports* ex, int on_device) {
     if (!d || !f || !cfg || !out) return EAFP_ERR_ARG;
     EAFP_TRY_RC(validate_desc(d));
     for (int q = 0; q < 8; ++q)
         if (!out[q]) return EAFP_ERR_ARG;
+    double* ex_d[6] = {};
+    int* ex_i[3] = {};
+    if (ex) {
+        double* const ed[6] = {ex->Fs_r, ex->Fs_th, ex->Fs_z, ex->ac_r, ex->ac_th, ex->ac_z};
+        int* const ei[3] = {ex->lk_r, ex->lk_th, ex->lk_z};
+        for (int q = 0; q < 6; ++q)
+            if (!(ex_d[q] = ed[q])) return EAFP_ERR_ARG;
+        for (int q = 0; q < 3; ++q)
+            if (!(ex_i[q] = ei[q])) return EAFP_ERR_ARG;
+    }
     const double* f3[NF3] = {f->vol,    f->Ar,    f->Ath,    f->Az,    f->alpha_l, f->ur_l,    f->uth_l,   f->uz_l,
                              f->aPr_l,  f->aPth_l, f->aPz_l, f->rho_l, f->T_l,     f->alpha_g, f->ur_g,    f->uth_g,
                              f->uz_g,   f->aPr_g,  f->aPth_g, f->aPz_g, f->rho_g,  f->T_old_g, f->p};
@@ -455,13 +621,27 @@ int pp_assemble(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_confi
     double** dst_out[8] = {&a.aW, &a.aE, &a.aS, &a.aN, &a.aB, &a.aT, &a.aP, &a.Su};
     for (int q = 0; q < 8; ++q) *dst_out[q] = on_device ? out[q] : w.outs.p + (long)q * n;
     a.part = w.part.p;
+    if (ex)
+        for (int dd = 0; dd < 3; ++dd) {
+            a.Fs[dd] = on_device ? ex_d[dd] : w.exd.p + (long)dd * n;
+            a.ac[dd] = on_device ? ex_d[3 + dd] : w.exd.p + (long)(3 + dd) * n;
+            a.lk[dd] = on_device ? ex_i[dd] : w.lki.p + (long)dd * n;
+        }
     const int blocks = (int)((n + BLOCK - 1) / BLOCK);
     k_pp_grad<<<blocks, BLOCK, 0, w.stream>>>(a);
     k_pp_assemble<<<blocks, BLOCK, 0, w.stream>>>(a);
     EAFP_TRY(hipGetLastError());
-    if (!on_device)
+    if (!on_device) {
+        const hipMemcpyKind D2H = hipMemcpyDeviceToHost;
         for (int q = 0; q < 8; ++q)
-            EAFP_TRY(hipMemcpyAsync(out[q], w.outs.p + (long)q * n, sizeof(double) * n, hipMemcpyDeviceToHost, w.stream));
+            EAFP_TRY(hipMemcpyAsync(out[q], w.outs.p + (long)q * n, sizeof(double) * n, D2H, w.stream));
+        if (ex) {
+            for (int q = 0; q < 6; ++q)
+                EAFP_TRY(hipMemcpyAsync(ex_d[q], w.exd.p + (long)q * n, sizeof(double) * n, D2H, w.stream));
+            for (int q = 0; q < 3; ++q)
+                EAFP_TRY(hipMemcpyAsync(ex_i[q], w.lki.p + (long)q * n, sizeof(int) * n, D2H, w.stream));
+        }
+    }
     EAFP_TRY(hipMemcpyAsync(w.part_h.data(), w.part.p, sizeof(double) * 2 * blocks, hipMemcpyDeviceToHost, w.stream));
     EAFP_TRY(hipStreamSynchronize(w.stream));
     double fl = 0.0, su = 0.0;
@@ -471,6 +651,130 @@ int pp_assemble(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_confi
     }
     if (flux_ref) *flux_ref = fl;
     if (residual) *residual = su / std::max(fl, SMALL);
+    return EAFP_OK;
+}
+
+int pp_correct(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config* cfg, double alpha_p,
+               const double* pp, const eafp_pp_exports* ex, const eafp_pp_update* io, int on_device) {
+    if (!d || !f || !cfg || !pp || !ex || !io) return EAFP_ERR_ARG;
+    EAFP_TRY_RC(validate_desc(d));
+    // inputs in staging order: vol, alpha_l, alpha_g, aP of both phases
+    const double* fin[9] = {f->vol, f->alpha_l, f->alpha_g, f->aPr_l, f->aPth_l, f->aPz_l, f->aPr_g, f->aPth_g, f->aPz_g};
+    for (const double* q : fin)
+        if (!q) return EAFP_ERR_ARG;
+    if (!f->r || !f->theta || !f->z || !f->cell_type) return EAFP_ERR_ARG;
+    const double* exd[6] = {ex->Fs_r, ex->Fs_th, ex->Fs_z, ex->ac_r, ex->ac_th, ex->ac_z};
+    const int* exi[3] = {ex->lk_r, ex->lk_th, ex->lk_z};
+    double* const fc[3] = {io->Fc_r, io->Fc_th, io->Fc_z};
+    int* const lko[3] = {io->lk_r, io->lk_th, io->lk_z};
+    double* const vel[6] = {io->ur_l, io->uth_l, io->uz_l, io->ur_g, io->uth_g, io->uz_g};
+    for (int q = 0; q < 6; ++q)
+        if (!exd[q] || !vel[q]) return EAFP_ERR_ARG;
+    for (int q = 0; q < 3; ++q)
+        if (!exi[q] || !fc[q] || !lko[q]) return EAFP_ERR_ARG;
+    if (!io->p) return EAFP_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_pp_mutex);
+    EAFP_TRY_RC(ensure_pp(d));
+    PPWork& w = *g_pp;
+    const int nr = d->nr, nth = d->ntheta, nz = d->nz;
+    PCArgs a{};
+    a.nr = nr;
+    a.nth = nth;
+    a.nz = nz;
+    a.sy = nr + 4;
+    a.sz = (long)(nr + 4) * (nth + 4);
+    a.n = a.sz * (nz + 4);
+    const long n = a.n;
+    a.alpha_p = alpha_p;
+    a.two_phase = cfg->gas_in_poisson && cfg->solve_multiphase;
+    a.p_upd = w.grads.p + 3 * n;
+    const double* din[9];
+    if (on_device) {
+        a.r = f->r;
+        a.th = f->theta;
+        a.z = f->z;
+        for (int q = 0; q < 9; ++q) din[q] = fin[q];
+        a.ct = f->cell_type;
+        a.liq_cont = f->liq_cont;
+        a.pv_active = f->pv_active;
+        a.gas_reach = f->gas_reach;
+        a.pp = pp;
+        for (int q = 0; q < 3; ++q) {
+            a.Fs[q] = exd[q];
+            a.ac[q] = exd[3 + q];
+            a.lk_in[q] = exi[q];
+            a.Fc[q] = fc[q];
+            a.lk[q] = lko[q];
+        }
+        for (int q = 0; q < 6; ++q) a.vel[q] = vel[q];
+        a.p = io->p;
+        a.p_out = io->p;
+    } else {
+        const hipMemcpyKind H2D = hipMemcpyHostToDevice;
+        const size_t nd = sizeof(double) * n, ni = sizeof(int) * n;
+        double* c = w.coord.p;
+        EAFP_TRY(hipMemcpyAsync(c, f->r, sizeof(double) * (nr + 4), H2D, w.stream));
+        EAFP_TRY(hipMemcpyAsync(c + nr + 4, f->theta, sizeof(double) * (nth + 4), H2D, w.stream));
+        EAFP_TRY(hipMemcpyAsync(c + nr + nth + 8, f->z, sizeof(double) * (nz + 4), H2D, w.stream));
+        a.r = c;
+        a.th = c + nr + 4;
+        a.z = c + nr + nth + 8;
+        // f3 slots: 0..8 the inputs, 9..14 the velocities, 15 p, 16 pp
+        for (int q = 0; q < 9; ++q) {
+            EAFP_TRY(hipMemcpyAsync(w.f3.p + (long)q * n, fin[q], nd, H2D, w.stream));
+            din[q] = w.f3.p + (long)q * n;
+        }
+        for (int q = 0; q < 6; ++q) {
+            a.vel[q] = w.f3.p + (long)(9 + q) * n;
+            EAFP_TRY(hipMemcpyAsync(a.vel[q], vel[q], nd, H2D, w.stream));
+        }
+        double* p_d = w.f3.p + 15 * n;
+        EAFP_TRY(hipMemcpyAsync(p_d, io->p, nd, H2D, w.stream));
+        a.p = p_d;
+        a.p_out = p_d;
+        EAFP_TRY(hipMemcpyAsync(w.f3.p + 16 * n, pp, nd, H2D, w.stream));
+        a.pp = w.f3.p + 16 * n;
+        EAFP_TRY(hipMemcpyAsync(w.ct.p, f->cell_type, ni, H2D, w.stream));
+        a.ct = w.ct.p;
+        const int* m[3] = {f->liq_cont, f->pv_active, f->gas_reach};
+        const int** dst[3] = {&a.liq_cont, &a.pv_active, &a.gas_reach};
+        for (int q = 0; q < 3; ++q) {
+            if (m[q]) {
+                EAFP_TRY(hipMemcpyAsync(w.masks.p + (long)q * n, m[q], ni, H2D, w.stream));
+                *dst[q] = w.masks.p + (long)q * n;
+            } else {
+                *dst[q] = nullptr;
+            }
+        }
+        for (int q = 0; q < 6; ++q) EAFP_TRY(hipMemcpyAsync(w.exd.p + (long)q * n, exd[q], nd, H2D, w.stream));
+        for (int q = 0; q < 3; ++q) {
+            EAFP_TRY(hipMemcpyAsync(w.lki.p + (long)q * n, exi[q], ni, H2D, w.stream));
+            a.Fs[q] = w.exd.p + (long)q * n;
+            a.ac[q] = w.exd.p + (long)(3 + q) * n;
+            a.lk_in[q] = w.lki.p + (long)q * n;
+            a.Fc[q] = w.grads.p + (long)q * n;
+            a.lk[q] = w.lki.p + (long)(3 + q) * n;
+        }
+    }
+    a.vol = din[0];
+    a.al = din[1];
+    a.ag = din[2];
+    for (int q = 0; q < 6; ++q) a.aP[q] = din[3 + q];
+    const int blocks = (int)((n + BLOCK - 1) / BLOCK);
+    k_pp_correct<<<blocks, BLOCK, 0, w.stream>>>(a);
+    k_pp_seal<<<blocks, BLOCK, 0, w.stream>>>(a);
+    EAFP_TRY(hipGetLastError());
+    if (!on_device) {
+        const hipMemcpyKind D2H = hipMemcpyDeviceToHost;
+        const size_t nd = sizeof(double) * n, ni = sizeof(int) * n;
+        for (int q = 0; q < 3; ++q) {
+            EAFP_TRY(hipMemcpyAsync(fc[q], a.Fc[q], nd, D2H, w.stream));
+            EAFP_TRY(hipMemcpyAsync(lko[q], a.lk[q], ni, D2H, w.stream));
+        }
+        for (int q = 0; q < 6; ++q) EAFP_TRY(hipMemcpyAsync(vel[q], a.vel[q], nd, D2H, w.stream));
+        EAFP_TRY(hipMemcpyAsync(io->p, a.p_out, nd, D2H, w.stream));
+    }
+    EAFP_TRY(hipStreamSynchronize(w.stream));
     return EAFP_OK;
 }
 

@@ -115,7 +115,9 @@ struct Batch {
 
 int validate_desc(const eafp_desc* d);
 int pp_assemble(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config* cfg, double* const out[8],
-                double* residual, double* flux_ref, int on_device);
+                double* residual, double* flux_ref, const eafp_pp_exports* ex, int on_device);
+int pp_correct(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config* cfg, double alpha_p,
+               const double* pp, const eafp_pp_exports* ex, const eafp_pp_update* io, int on_device);
 void pp_release();
 int choose_grid(int ntiles);
 int device_cus();

@@ -79,9 +79,15 @@ int eafp_release(void) {
 
 int eafp_pp_assemble(const eafp_desc* desc, const eafp_pp_fields* fields, const eafp_pp_config* cfg, double* aW,
                      double* aE, double* aS, double* aN, double* aB, double* aT, double* aP, double* Su,
-                     double* residual, double* flux_ref, int arrays_on_device) {
+                     double* residual, double* flux_ref, eafp_pp_exports* exports, int arrays_on_device) {
     double* out[8] = {aW, aE, aS, aN, aB, aT, aP, Su};
-    return eafp::pp_assemble(desc, fields, cfg, out, residual, flux_ref, arrays_on_device);
+    return eafp::pp_assemble(desc, fields, cfg, out, residual, flux_ref, exports, arrays_on_device);
+}
+
+int eafp_pp_correct(const eafp_desc* desc, const eafp_pp_fields* fields, const eafp_pp_config* cfg, double alpha_p,
+                    const double* pp, const eafp_pp_exports* exports, const eafp_pp_update* io,
+                    int arrays_on_device) {
+    return eafp::pp_correct(desc, fields, cfg, alpha_p, pp, exports, io, arrays_on_device);
 }
 
 int eafp_batch_create(int nsys, const eafp_desc* descs, eafp_batch** out) {

@@ -8,6 +8,6 @@ the build step (``build``).
 """
 from . import systems  # noqa: F401
 from ._lib import EafpError  # noqa: F401
-from .solver import Batch, Mesh, cg_3d_mpi, pp_assemble, release  # noqa: F401
+from .solver import Batch, Mesh, cg_3d_mpi, pp_assemble, pp_correct, release  # noqa: F401
 
-__all__ = ["Batch", "EafpError", "Mesh", "cg_3d_mpi", "pp_assemble", "release", "systems"]
+__all__ = ["Batch", "EafpError", "Mesh", "cg_3d_mpi", "pp_assemble", "pp_correct", "release", "systems"]

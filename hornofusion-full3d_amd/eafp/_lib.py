@@ -25,7 +25,7 @@ COEF_FP32, COEF_BF16, COEF_FP16 = 0, 1, 2
 
 # Every symbol declared in include/eafp.h.
 EXPORTS = (
-    "eafp_cg_solve", "eafp_release", "eafp_pp_assemble", "eafp_batch_create", "eafp_batch_upload",
+    "eafp_cg_solve", "eafp_release", "eafp_pp_assemble", "eafp_pp_correct", "eafp_batch_create", "eafp_batch_upload",
     "eafp_batch_solve_fp64", "eafp_batch_solve_mixed", "eafp_batch_download",
     "eafp_batch_info", "eafp_batch_set_tiling", "eafp_batch_set_columns", "eafp_batch_get_columns",
     "eafp_batch_time_kernels", "eafp_batch_autotune", "eafp_batch_destroy", "eafp_version",
@@ -42,6 +42,21 @@ PP_INT = ("cell_type", "outlet", "liq_cont", "pv_active", "gas_reach")
 class PPFields(ctypes.Structure):
     """eafp_pp_fields: one pointer per field, in the header's order."""
     _fields_ = [(n, ctypes.c_void_p) for n in PP_FIELDS]
+
+
+PP_EXPORTS = ("Fs_r", "Fs_th", "Fs_z", "ac_r", "ac_th", "ac_z", "lk_r", "lk_th", "lk_z")
+PP_UPDATE = ("Fc_r", "Fc_th", "Fc_z", "lk_r", "lk_th", "lk_z",
+             "ur_l", "uth_l", "uz_l", "ur_g", "uth_g", "uz_g", "p")
+
+
+class PPExports(ctypes.Structure):
+    """eafp_pp_exports: the liquid's face terms (lk_* are int arrays)."""
+    _fields_ = [(n, ctypes.c_void_p) for n in PP_EXPORTS]
+
+
+class PPUpdate(ctypes.Structure):
+    """eafp_pp_update: what the post-solve consumers write."""
+    _fields_ = [(n, ctypes.c_void_p) for n in PP_UPDATE]
 
 
 class PPConfig(ctypes.Structure):
@@ -117,8 +132,11 @@ def load():
     L.eafp_batch_set_tiling.argtypes = [vp, i, i]
     L.eafp_batch_set_tiling.restype = i
     L.eafp_pp_assemble.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(PPFields), ctypes.POINTER(PPConfig)] + \
-        [vp] * 8 + [_dp, _dp, i]
+        [vp] * 8 + [_dp, _dp, ctypes.POINTER(PPExports), i]
     L.eafp_pp_assemble.restype = i
+    L.eafp_pp_correct.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(PPFields), ctypes.POINTER(PPConfig), d, vp,
+                                  ctypes.POINTER(PPExports), ctypes.POINTER(PPUpdate), i]
+    L.eafp_pp_correct.restype = i
     L.eafp_batch_set_columns.argtypes = [vp, i]
     L.eafp_batch_set_columns.restype = i
     L.eafp_batch_get_columns.argtypes = [vp, ip]

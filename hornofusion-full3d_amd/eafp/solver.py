@@ -76,14 +76,10 @@ def cg_3d_mpi(aW, aE, aS, aN, aB, aT, aP, Su, phi, m, max_iter, tol):
     return res.value, nit.value
 
 
-def pp_assemble(F, cfg, m):
-    """The p' system assembled on the GPU (eafp_pp_assemble) from the fields
-    of ``F`` (padded Fortran arrays, names as in ``_lib.PP_FIELDS``; absent
-    workspace masks are not valid) and the config dict ``cfg``.  Returns
-    ``(coef, residual, flux_ref)`` with ``coef`` mapping aW..aT, aP, Su to new
-    padded arrays."""
-    L = _lib.load()
-    shape = (m.nr, m.ntheta, m.nz)
+def _pp_structs(F, cfg):
+    """eafp_pp_fields / eafp_pp_config for the arrays of ``F`` (absent
+    workspace masks are not valid) and the config dict ``cfg``; ``keep``
+    holds the converted arrays alive."""
     keep = []
     vals = {}
     for n in _lib.PP_FIELDS:
@@ -97,15 +93,58 @@ def pp_assemble(F, cfg, m):
     f = _lib.PPFields(**vals)
     c = _lib.PPConfig(float(cfg["dt"]), float(cfg["rho_gas"]), float(cfg["T_ambient"]), float(cfg["beta_expansion"]),
                       int(cfg["gas_in_poisson"]), int(cfg["solve_multiphase"]), int(cfg["gas_compressibility"]), 0)
+    return f, c, keep
+
+
+def _padded_zeros(m, dtype=np.float64):
+    return np.zeros((m.nr + 4, m.ntheta + 4, m.nz + 4), dtype=dtype, order="F")
+
+
+def pp_assemble(F, cfg, m, exports=False):
+    """The p' system assembled on the GPU (eafp_pp_assemble) from the fields
+    of ``F`` (padded Fortran arrays, names as in ``_lib.PP_FIELDS``; absent
+    workspace masks are not valid) and the config dict ``cfg``.  Returns
+    ``(coef, residual, flux_ref)`` with ``coef`` mapping aW..aT, aP, Su to new
+    padded arrays, plus the dict of the liquid's face exports (names of
+    ``_lib.PP_EXPORTS``) when ``exports``."""
+    L = _lib.load()
+    f, c, keep = _pp_structs(F, cfg)
     names = ("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su")
-    out = {n: np.zeros((shape[0] + 4, shape[1] + 4, shape[2] + 4), order="F") for n in names}
+    out = {n: _padded_zeros(m) for n in names}
+    ex = {n: _padded_zeros(m, np.int32 if n.startswith("lk") else np.float64) for n in _lib.PP_EXPORTS} if exports else None
+    exs = _lib.PPExports(**{n: ex[n].ctypes.data for n in _lib.PP_EXPORTS}) if exports else None
     res = ctypes.c_double()
     flux = ctypes.c_double()
     d = desc_of(m)
     check(L.eafp_pp_assemble(ctypes.byref(d), ctypes.byref(f), ctypes.byref(c), *[out[n].ctypes.data for n in names],
-                             ctypes.byref(res), ctypes.byref(flux), 0), "eafp_pp_assemble")
+                             ctypes.byref(res), ctypes.byref(flux), ctypes.byref(exs) if exports else None, 0),
+          "eafp_pp_assemble")
     del keep
+    if exports:
+        return out, res.value, flux.value, ex
     return out, res.value, flux.value
+
+
+def pp_correct(F, cfg, m, alpha_p, pp, ex):
+    """The post-solve consumers on the GPU (eafp_pp_correct): returns a dict
+    with the conservative fluxes Fc_r/th/z, their link flags lk_r/th/z, the
+    corrected velocities (ur_l .. uz_g) and the updated p, all new padded
+    arrays; ``F`` is not modified.  ``pp`` is the solved correction (theta
+    halos filled), ``ex`` the exports of ``pp_assemble(..., exports=True)``."""
+    L = _lib.load()
+    f, c, keep = _pp_structs(F, cfg)
+    exa = {n: np.asfortranarray(ex[n], dtype=np.int32 if n.startswith("lk") else np.float64) for n in _lib.PP_EXPORTS}
+    exs = _lib.PPExports(**{n: a.ctypes.data for n, a in exa.items()})
+    out = {n: _padded_zeros(m, np.int32 if n.startswith("lk") else np.float64) for n in _lib.PP_UPDATE[:6]}
+    for n in _lib.PP_UPDATE[6:]:
+        out[n] = np.array(F[n], dtype=np.float64, order="F")
+    io = _lib.PPUpdate(**{n: out[n].ctypes.data for n in _lib.PP_UPDATE})
+    ppa = np.asfortranarray(pp, dtype=np.float64)
+    d = desc_of(m)
+    check(L.eafp_pp_correct(ctypes.byref(d), ctypes.byref(f), ctypes.byref(c), float(alpha_p), ppa.ctypes.data,
+                            ctypes.byref(exs), ctypes.byref(io), 0), "eafp_pp_correct")
+    del keep
+    return out
 
 
 def release():

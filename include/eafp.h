@@ -96,8 +96,8 @@ int eafp_cg_solve(const eafp_desc* desc,
                   const double* aB, const double* aT, const double* aP, const double* Su,
                   double* phi, int max_iter, double tol, double* residual, int* n_iter);
 
-/* Releases the device buffers cached by eafp_cg_solve and
- * eafp_pp_assemble.  No reference counterpart (the reference's workspace
+/* Releases the device buffers cached by eafp_cg_solve, eafp_pp_assemble
+ * and eafp_pp_correct.  No reference counterpart (the reference's workspace
  * lives until exit). */
 int eafp_release(void);
 
@@ -142,17 +142,63 @@ typedef struct {
     int pad_;
 } eafp_pp_config;
 
+/* The liquid's +face terms that the assembly keeps for the post-solve
+ * fluxes: F* and a_nb of every open E, N and T face of the liquid (Fs_r,
+ * Fs_th, Fs_z, ac_r, ac_th, ac_z, locals of solve_pressure_correction,
+ * src/mod_pressure_3d.f90:117-118, set by add_phase_contribution :525-527,
+ * :562-564, :601-603) and their link flags (ws_Fc_lk_r/th/z, 0/1).  Padded
+ * arrays, zero where no open liquid face. */
+typedef struct {
+    double *Fs_r, *Fs_th, *Fs_z, *ac_r, *ac_th, *ac_z;
+    int *lk_r, *lk_th, *lk_z;
+} eafp_pp_exports;
+
 /* Writes the assembled aW..aT, aP, Su (padded, halos zero) and the
  * continuity residual and its normaliser flux_ref (either may be NULL).
- * arrays_on_device = 1: every array pointer, inputs and outputs, is a device
- * pointer (nothing crosses PCIe).  The coefficients are bit-identical to a
- * sequential FP64 evaluation of the reference's expressions in their written
- * order without contraction; the two global sums of the residual are
- * reduced in a different order.  Serial meshes only (EAFP_ERR_UNSUPPORTED
- * when desc->is_parallel). */
+ * exports: NULL, or where to write the liquid's face terms (every pointer
+ * set).  arrays_on_device = 1: every array pointer, inputs and outputs, is a
+ * device pointer (nothing crosses PCIe).  The coefficients and exports are
+ * bit-identical to a sequential FP64 evaluation of the reference's
+ * expressions in their written order without contraction; the two global
+ * sums of the residual are reduced in a different order.  Serial meshes
+ * only (EAFP_ERR_UNSUPPORTED when desc->is_parallel). */
 int eafp_pp_assemble(const eafp_desc* desc, const eafp_pp_fields* fields, const eafp_pp_config* cfg,
                      double* aW, double* aE, double* aS, double* aN, double* aB, double* aT,
-                     double* aP, double* Su, double* residual, double* flux_ref, int arrays_on_device);
+                     double* aP, double* Su, double* residual, double* flux_ref,
+                     eafp_pp_exports* exports, int arrays_on_device);
+
+/* State the post-solve consumers write (padded arrays). */
+typedef struct {
+    double *Fc_r, *Fc_th, *Fc_z;       /* out: ws_Fc_r, ws_Fc_th, ws_Fc_z */
+    int *lk_r, *lk_th, *lk_z;          /* out: ws_Fc_lk_r/th/z after the halo copy */
+    double *ur_l, *uth_l, *uz_l;       /* in/out: liq%ur, liq%uth, liq%uz */
+    double *ur_g, *uth_g, *uz_g;       /* in/out: gas%ur, gas%uth, gas%uz */
+    double* p;                         /* in/out: sh%p */
+} eafp_pp_update;
+
+/* What solve_pressure_correction does with p' after its cg_3d_mpi call
+ * (src/mod_pressure_3d.f90:324-397), for one serial mesh:
+ *   - the conservative liquid fluxes Fc = Fs + ac (pp_P - pp_nb) of the
+ *     linked E, N and T faces (:333-351), zero elsewhere; the theta halos of
+ *     Fc and of the link flags by periodic copy;
+ *   - correct_velocities (:689-746) of the liquid and, with gas_in_poisson
+ *     and solve_multiphase, of the gas: u -= alpha vol / aP_u * (centred
+ *     difference of pp), skipped where |aP_u| <= 1e-30, the r part on
+ *     i = 1, nr and the z part on k = 1, nz;
+ *   - p = min(max(p + alpha_p pp, -2e6), 2e6) over the whole array (:358-360);
+ *   - sealed cells (fluid, neither phase active) take the mean p of their
+ *     neighbours with an active phase, 0 without one (:373-397); then the
+ *     theta halo copy of p.
+ * fields: the assembly's inputs, of which r, theta, z, vol, cell_type,
+ * alpha_l/alpha_g, the six aP_u arrays and the masks are read (the
+ * velocities and p updated are io's).  pp: the solved correction with its
+ * theta halos filled (cg_3d_mpi and eafp_cg_solve leave them so).
+ * exports: as eafp_pp_assemble wrote them for the same fields.  Every
+ * output is bit-identical to a sequential FP64 evaluation (nothing is
+ * reduced).  arrays_on_device as for eafp_pp_assemble. */
+int eafp_pp_correct(const eafp_desc* desc, const eafp_pp_fields* fields, const eafp_pp_config* cfg,
+                    double alpha_p, const double* pp, const eafp_pp_exports* exports, const eafp_pp_update* io,
+                    int arrays_on_device);
 
 /* ---------------------------------------------------------------------------
  * Device-resident systems.  A batch holds nsys systems of any shapes; nsys

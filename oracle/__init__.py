@@ -71,6 +71,13 @@ def lib():
         L.oracle_true_residual.restype = None
         L.oracle_pp_assemble.argtypes = [i, i, i, ctypes.c_void_p, ctypes.c_void_p] + [_dp] * 8 + [_dp, _dp]
         L.oracle_pp_assemble.restype = ctypes.c_int
+        L.oracle_pp_assemble_ex.argtypes = [i, i, i, ctypes.c_void_p, ctypes.c_void_p] + [_dp] * 8 + \
+            [_dp, _dp, ctypes.c_void_p]
+        L.oracle_pp_assemble_ex.restype = ctypes.c_int
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_pp_correct.argtypes = [i, i, i, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, _dp,
+                                        ctypes.c_void_p, _dp, _dp, _dp, ip, ip, ip, ctypes.POINTER(_dp), _dp]
+        L.oracle_pp_correct.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -125,22 +132,63 @@ def pp_structs(F, cfg):
     return f, c, keep
 
 
-def pp_assemble(F, cfg, shape):
-    """The restated p' assembly.  Returns (coef dict with aW..aP and Su as
-    padded arrays, residual, flux_ref)."""
+EXPORTS = ("Fs_r", "Fs_th", "Fs_z", "ac_r", "ac_th", "ac_z", "lk_r", "lk_th", "lk_z")
+
+
+class PPExports(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in EXPORTS]
+
+
+def _padded_zeros(shape, dtype=np.float64):
     nr, nth, nz = shape
+    return np.zeros((nr + 4, nth + 4, nz + 4), dtype=dtype, order="F")
+
+
+def pp_assemble(F, cfg, shape, exports=False):
+    """The restated p' assembly.  Returns (coef dict with aW..aP and Su as
+    padded arrays, residual, flux_ref), plus the dict of liquid face exports
+    (Fs_*, ac_*, lk_*) when ``exports``."""
     f, c, keep = pp_structs(F, cfg)
-    outs = {n: np.zeros((nr + 4, nth + 4, nz + 4), order="F") for n in ("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su")}
+    outs = {n: _padded_zeros(shape) for n in ("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su")}
+    ex = {n: _padded_zeros(shape, np.int32 if n.startswith("lk") else np.float64) for n in EXPORTS} if exports else None
+    exs = PPExports(**{n: ex[n].ctypes.data for n in EXPORTS}) if exports else None
     res = ctypes.c_double()
     flux = ctypes.c_double()
-    L = lib()
-    rc = L.oracle_pp_assemble(nr, nth, nz, ctypes.byref(f), ctypes.byref(c),
-                              *[outs[n].ctypes.data_as(_dp) for n in ("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su")],
-                              ctypes.byref(res), ctypes.byref(flux))
+    rc = lib().oracle_pp_assemble_ex(*shape, ctypes.byref(f), ctypes.byref(c),
+                                     *[outs[n].ctypes.data_as(_dp) for n in ("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su")],
+                                     ctypes.byref(res), ctypes.byref(flux), ctypes.byref(exs) if exports else None)
     del keep
     if rc != 0:
         raise MemoryError("oracle_pp_assemble could not allocate its work arrays")
+    if exports:
+        return outs, res.value, flux.value, ex
     return outs, res.value, flux.value
+
+
+VEL = ("ur_l", "uth_l", "uz_l", "ur_g", "uth_g", "uz_g")
+
+
+def pp_correct(F, cfg, shape, alpha_p, pp, ex):
+    """The restated post-solve consumers.  Returns a dict with the
+    conservative fluxes Fc_r/th/z, their link flags lk_r/th/z, the corrected
+    velocities (names of VEL) and the updated p; F is not modified."""
+    f, c, keep = pp_structs(F, cfg)
+    exs = PPExports(**{n: ex[n].ctypes.data for n in EXPORTS})
+    out = {n: _padded_zeros(shape) for n in ("Fc_r", "Fc_th", "Fc_z")}
+    out.update({n: _padded_zeros(shape, np.int32) for n in ("lk_r", "lk_th", "lk_z")})
+    for n in VEL + ("p",):
+        out[n] = np.array(F[n], dtype=np.float64, order="F")
+    vel = (_dp * 6)(*[out[n].ctypes.data_as(_dp) for n in VEL])
+    ppa = np.asfortranarray(pp, dtype=np.float64)
+    ip = ctypes.POINTER(ctypes.c_int)
+    rc = lib().oracle_pp_correct(*shape, ctypes.byref(f), ctypes.byref(c), float(alpha_p), ppa.ctypes.data_as(_dp),
+                                 ctypes.byref(exs), *[out[n].ctypes.data_as(_dp) for n in ("Fc_r", "Fc_th", "Fc_z")],
+                                 *[out[n].ctypes.data_as(ip) for n in ("lk_r", "lk_th", "lk_z")], vel,
+                                 out["p"].ctypes.data_as(_dp))
+    del keep
+    if rc != 0:
+        raise MemoryError("oracle_pp_correct could not allocate its work arrays")
+    return out
 
 
 def _p(a: np.ndarray):

@@ -27,6 +27,15 @@
  *                                      C_SOUND_LIQ: :48, :63, :67)
  *   boundary conditions      src/mod_boundary_3d.f90:180-277
  *   continuity residual      :290-306
+ *   liquid face exports      :525-530, :557-562, :597-602 (F* and a_nb of
+ *                            the +faces, their link flags)
+ * and, in oracle_pp_correct, the post-solve consumers:
+ *   conservative fluxes      :333-351 (F = F* + a (pp_P - pp_nb), theta
+ *                            halos by periodic copy)
+ *   correct_velocities       :689-746, both phases
+ *   pressure update / clamp  :358-360 (P_HYDRO_CAP :74)
+ *   sealed cells             :373-397 (mean of the neighbours with fluid),
+ *                            then the theta halo copy of p
  * The expressions keep the reference's operand order (left to right, as
  * written); built with -ffp-contract=off.
  *
@@ -49,6 +58,7 @@
 #define P0_THERMO 101325.0            /* src/mod_pressure_3d.f90:63 */
 #define C_SOUND_LIQ 4000.0            /* src/mod_pressure_3d.f90:67 */
 #define ROOM_EPS 1.0e-4               /* src/mod_pressure_3d.f90:663 */
+#define P_HYDRO_CAP 2.0e6             /* src/mod_pressure_3d.f90:74 */
 
 /* Same layout as eafp_pp_fields / eafp_pp_config in include/eafp.h. */
 typedef struct {
@@ -90,6 +100,8 @@ typedef struct {
     const pp_config* c;
     unsigned char *act_l, *act_g;
     double *gr, *gth, *gz, *gz_g;     /* liquid gradients; the gas phase differs in gz only */
+    double *Fs[3], *ac[3];            /* liquid +face exports (r, theta, z), NULL: none */
+    int* lk[3];
 } ctx_t;
 
 static int pv_cell(const ctx_t* x, long o) {
@@ -154,6 +166,7 @@ typedef struct {
     const unsigned char* act;
     const double* gz;
     int isg;
+    int exp;        /* write the +face exports (the liquid) */
 } phase_t;
 
 static int link(const ctx_t* x, const unsigned char* act, long o) { return x->f->cell_type[o] != 0 && act[o]; }
@@ -194,6 +207,11 @@ static void add_phase(const ctx_t* x, const phase_t* ph, double* aW, double* aE,
                         aE[o] = aE[o] + af * d_f * f->Ar[o] / delta;
                         Su[o] = Su[o] - af * u_f * f->Ar[o];
                         *flux_ref = *flux_ref + fabs(af * u_f * f->Ar[o]);
+                        if (ph->exp && x->Fs[0]) {
+                            x->Fs[0][o] = af * u_f * f->Ar[o];
+                            x->ac[0][o] = af * d_f * f->Ar[o] / delta;
+                            x->lk[0][o] = 1;
+                        }
                     }
                 }
                 if (link(x, ph->act, oS)) {
@@ -216,6 +234,11 @@ static void add_phase(const ctx_t* x, const phase_t* ph, double* aW, double* aE,
                         aN[o] = aN[o] + af * d_f * f->Ath[o] / delta;
                         Su[o] = Su[o] - af * u_f * f->Ath[o];
                         *flux_ref = *flux_ref + fabs(af * u_f * f->Ath[o]);
+                        if (ph->exp && x->Fs[1]) {
+                            x->Fs[1][o] = af * u_f * f->Ath[o];
+                            x->ac[1][o] = af * d_f * f->Ath[o] / delta;
+                            x->lk[1][o] = 1;
+                        }
                     }
                 }
                 if (link(x, ph->act, oB)) {
@@ -242,22 +265,44 @@ static void add_phase(const ctx_t* x, const phase_t* ph, double* aW, double* aE,
                         aT[o] = aT[o] + af * d_f * f->Az[o] / delta;
                         Su[o] = Su[o] - af * u_f * f->Az[o];
                         *flux_ref = *flux_ref + fabs(af * u_f * f->Az[o]);
+                        if (ph->exp && x->Fs[2]) {
+                            x->Fs[2][o] = af * u_f * f->Az[o];
+                            x->ac[2][o] = af * d_f * f->Az[o] / delta;
+                            x->lk[2][o] = 1;
+                        }
                     }
                 }
             }
 }
 
+/* Liquid +face exports: F* and a_nb of the E, N, T faces and their link
+ * flags (all zero where no open liquid face), padded arrays. */
+typedef struct {
+    double *Fs_r, *Fs_th, *Fs_z, *ac_r, *ac_th, *ac_z;
+    int *lk_r, *lk_th, *lk_z;
+} pp_exports;
+
 /* Returns 0 on success, -1 if work arrays cannot be allocated.  Outputs are
- * the full padded arrays (halos zero) and the continuity residual
- * sum|Su| / max(flux_ref, SMALL) with flux_ref. */
-int oracle_pp_assemble(int nr, int nth, int nz, const pp_fields* f, const pp_config* c, double* aW, double* aE,
-                       double* aS, double* aN, double* aB, double* aT, double* aP, double* Su, double* residual,
-                       double* flux_ref_out) {
+ * the full padded arrays (halos zero), the continuity residual
+ * sum|Su| / max(flux_ref, SMALL) with flux_ref, and (ex non-NULL) the liquid
+ * face exports. */
+int oracle_pp_assemble_ex(int nr, int nth, int nz, const pp_fields* f, const pp_config* c, double* aW, double* aE,
+                          double* aS, double* aN, double* aB, double* aT, double* aP, double* Su, double* residual,
+                          double* flux_ref_out, const pp_exports* ex) {
     grid_t g;
     g.nr = nr; g.nth = nth; g.nz = nz;
     g.sy = nr + 4; g.sz = (long)(nr + 4) * (nth + 4); g.n = g.sz * (nz + 4);
     ctx_t x;
     x.g = &g; x.f = f; x.c = c;
+    x.Fs[0] = ex ? ex->Fs_r : NULL; x.Fs[1] = ex ? ex->Fs_th : NULL; x.Fs[2] = ex ? ex->Fs_z : NULL;
+    x.ac[0] = ex ? ex->ac_r : NULL; x.ac[1] = ex ? ex->ac_th : NULL; x.ac[2] = ex ? ex->ac_z : NULL;
+    x.lk[0] = ex ? ex->lk_r : NULL; x.lk[1] = ex ? ex->lk_th : NULL; x.lk[2] = ex ? ex->lk_z : NULL;
+    for (int d = 0; d < 3; ++d)
+        if (x.Fs[d]) {
+            memset(x.Fs[d], 0, sizeof(double) * (size_t)g.n);
+            memset(x.ac[d], 0, sizeof(double) * (size_t)g.n);
+            memset(x.lk[d], 0, sizeof(int) * (size_t)g.n);
+        }
     x.act_l = malloc((size_t)g.n);
     x.act_g = malloc((size_t)g.n);
     x.gr = calloc((size_t)g.n, sizeof(double));
@@ -281,10 +326,10 @@ int oracle_pp_assemble(int nr, int nth, int nz, const pp_fields* f, const pp_con
     for (int a = 0; a < 8; ++a) memset(outs[a], 0, sizeof(double) * (size_t)g.n);
     gradients(&x);
     double flux_ref = 0.0;
-    phase_t liq = {f->alpha_l, f->ur_l, f->uth_l, f->uz_l, f->aPr_l, f->aPth_l, f->aPz_l, x.act_l, x.gz, 0};
+    phase_t liq = {f->alpha_l, f->ur_l, f->uth_l, f->uz_l, f->aPr_l, f->aPth_l, f->aPz_l, x.act_l, x.gz, 0, 1};
     add_phase(&x, &liq, aW, aE, aS, aN, aB, aT, Su, &flux_ref);
     if (two_phase) {
-        phase_t gas = {f->alpha_g, f->ur_g, f->uth_g, f->uz_g, f->aPr_g, f->aPth_g, f->aPz_g, x.act_g, x.gz_g, 1};
+        phase_t gas = {f->alpha_g, f->ur_g, f->uth_g, f->uz_g, f->aPr_g, f->aPth_g, f->aPz_g, x.act_g, x.gz_g, 1, 0};
         add_phase(&x, &gas, aW, aE, aS, aN, aB, aT, Su, &flux_ref);
     }
     if (compress) {
@@ -362,5 +407,124 @@ int oracle_pp_assemble(int nr, int nth, int nz, const pp_fields* f, const pp_con
     *residual = sum_su / fmax(flux_ref, SMALL);
     *flux_ref_out = flux_ref;
     free(x.act_l); free(x.act_g); free(x.gr); free(x.gth); free(x.gz); free(x.gz_g);
+    return 0;
+}
+
+int oracle_pp_assemble(int nr, int nth, int nz, const pp_fields* f, const pp_config* c, double* aW, double* aE,
+                       double* aS, double* aN, double* aB, double* aT, double* aP, double* Su, double* residual,
+                       double* flux_ref_out) {
+    return oracle_pp_assemble_ex(nr, nth, nz, f, c, aW, aE, aS, aN, aB, aT, aP, Su, residual, flux_ref_out, NULL);
+}
+
+static void theta_copy_d(const grid_t* g, double* a) {
+    const int dst[4] = {0, -1, g->nth + 1, g->nth + 2};
+    const int src[4] = {g->nth, g->nth - 1, 1, 2};
+    for (int q = 0; q < 4; ++q)
+        for (int k = -1; k <= g->nz + 2; ++k)
+            for (int i = -1; i <= g->nr + 2; ++i) a[O(g, i, dst[q], k)] = a[O(g, i, src[q], k)];
+}
+
+static void theta_copy_i(const grid_t* g, int* a) {
+    const int dst[4] = {0, -1, g->nth + 1, g->nth + 2};
+    const int src[4] = {g->nth, g->nth - 1, 1, 2};
+    for (int q = 0; q < 4; ++q)
+        for (int k = -1; k <= g->nz + 2; ++k)
+            for (int i = -1; i <= g->nr + 2; ++i) a[O(g, i, dst[q], k)] = a[O(g, i, src[q], k)];
+}
+
+/* Post-solve consumers (see the file header).  pp: the solved correction
+ * (padded, theta halos filled by the CG).  ex: the assembly's exports;
+ * Fc/lk_out: conservative fluxes and link flags (padded, zero where no open
+ * liquid face, theta halos by periodic copy).  vel: ur, uth, uz of the
+ * liquid then of the gas, corrected in place; p updated in place. */
+int oracle_pp_correct(int nr, int nth, int nz, const pp_fields* f, const pp_config* c, double alpha_p, const double* pp,
+                      const pp_exports* ex, double* Fc_r, double* Fc_th, double* Fc_z, int* lk_r, int* lk_th,
+                      int* lk_z, double* const vel[6], double* p) {
+    grid_t g;
+    g.nr = nr; g.nth = nth; g.nz = nz;
+    g.sy = nr + 4; g.sz = (long)(nr + 4) * (nth + 4); g.n = g.sz * (nz + 4);
+    unsigned char* act_l = malloc((size_t)g.n);
+    unsigned char* act_g = malloc((size_t)g.n);
+    if (!act_l || !act_g) { free(act_l); free(act_g); return -1; }
+    for (long o = 0; o < g.n; ++o) {
+        act_l[o] = f->liq_cont ? (f->liq_cont[o] != 0) : (f->alpha_l[o] >= ALPHA_FLOW_CUTOFF);
+        int ag = f->alpha_g[o] >= ALPHA_FLOW_CUTOFF;
+        if (f->pv_active) ag = ag && f->pv_active[o] != 0;
+        if (f->gas_reach) ag = ag || (f->gas_reach[o] != 0 && f->alpha_g[o] < ALPHA_FLOW_CUTOFF && f->cell_type[o] != 0);
+        act_g[o] = (unsigned char)ag;
+    }
+    /* conservative liquid fluxes (:333-351) */
+    memset(Fc_r, 0, sizeof(double) * (size_t)g.n);
+    memset(Fc_th, 0, sizeof(double) * (size_t)g.n);
+    memset(Fc_z, 0, sizeof(double) * (size_t)g.n);
+    memcpy(lk_r, ex->lk_r, sizeof(int) * (size_t)g.n);
+    memcpy(lk_th, ex->lk_th, sizeof(int) * (size_t)g.n);
+    memcpy(lk_z, ex->lk_z, sizeof(int) * (size_t)g.n);
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= nth; ++j)
+            for (int i = 1; i <= nr; ++i) {
+                const long o = O(&g, i, j, k);
+                if (f->cell_type[o] == 0) continue;
+                if (ex->lk_r[o] == 1) Fc_r[o] = ex->Fs_r[o] + ex->ac_r[o] * (pp[o] - pp[O(&g, i + 1, j, k)]);
+                if (ex->lk_th[o] == 1) Fc_th[o] = ex->Fs_th[o] + ex->ac_th[o] * (pp[o] - pp[O(&g, i, j + 1, k)]);
+                if (ex->lk_z[o] == 1) Fc_z[o] = ex->Fs_z[o] + ex->ac_z[o] * (pp[o] - pp[O(&g, i, j, k + 1)]);
+            }
+    theta_copy_d(&g, Fc_r); theta_copy_d(&g, Fc_th); theta_copy_d(&g, Fc_z);
+    theta_copy_i(&g, lk_r); theta_copy_i(&g, lk_th); theta_copy_i(&g, lk_z);
+    /* correct_velocities (:689-746): liquid, then the gas with the two-phase coupling */
+    const int two_phase = c->gas_in_poisson && c->solve_multiphase;
+    for (int ph = 0; ph < (two_phase ? 2 : 1); ++ph) {
+        const unsigned char* act = ph ? act_g : act_l;
+        const double* alpha = ph ? f->alpha_g : f->alpha_l;
+        const double* aPr = ph ? f->aPr_g : f->aPr_l;
+        const double* aPth = ph ? f->aPth_g : f->aPth_l;
+        const double* aPz = ph ? f->aPz_g : f->aPz_l;
+        double *ur = vel[3 * ph], *uth = vel[3 * ph + 1], *uz = vel[3 * ph + 2];
+        for (int k = 1; k <= nz; ++k)
+            for (int j = 1; j <= nth; ++j)
+                for (int i = 1; i <= nr; ++i) {
+                    const long o = O(&g, i, j, k);
+                    if (f->cell_type[o] == 0 || !act[o]) continue;
+                    double d_coeff;
+                    if (!(i == 1 || i == nr) && fabs(aPr[o]) > SMALL) {
+                        d_coeff = alpha[o] * f->vol[o] / aPr[o];
+                        ur[o] = ur[o] - d_coeff * (pp[O(&g, i + 1, j, k)] - pp[O(&g, i - 1, j, k)]) / (f->r[i + 2] - f->r[i]);
+                    }
+                    if (fabs(aPth[o]) > SMALL) {
+                        d_coeff = alpha[o] * f->vol[o] / aPth[o];
+                        uth[o] = uth[o] - d_coeff * (pp[O(&g, i, j + 1, k)] - pp[O(&g, i, j - 1, k)]) /
+                                              (f->r[i + 1] * (f->theta[j + 2] - f->theta[j]));
+                    }
+                    if (!(k == 1 || k == nz) && fabs(aPz[o]) > SMALL) {
+                        d_coeff = alpha[o] * f->vol[o] / aPz[o];
+                        uz[o] = uz[o] - d_coeff * (pp[O(&g, i, j, k + 1)] - pp[O(&g, i, j, k - 1)]) / (f->z[k + 2] - f->z[k]);
+                    }
+                }
+    }
+    /* pressure update and clamp over the whole array (:358-360) */
+    for (long o = 0; o < g.n; ++o) {
+        p[o] = p[o] + alpha_p * pp[o];
+        p[o] = fmax(-P_HYDRO_CAP, fmin(P_HYDRO_CAP, p[o]));
+    }
+    /* sealed cells: mean of the neighbours with fluid (:373-397) */
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= nth; ++j)
+            for (int i = 1; i <= nr; ++i) {
+                const long o = O(&g, i, j, k);
+                if (f->cell_type[o] == 0 || act_l[o] || act_g[o]) continue;
+                const long nb[6] = {O(&g, i - 1, j, k), O(&g, i + 1, j, k), O(&g, i, j - 1, k),
+                                    O(&g, i, j + 1, k), O(&g, i, j, k - 1), O(&g, i, j, k + 1)};
+                double psum = 0.0;
+                int nnb = 0;
+                for (int q = 0; q < 6; ++q) {
+                    const long b = nb[q];
+                    if (f->cell_type[b] == 0 || (!act_l[b] && !act_g[b])) continue;
+                    psum = psum + p[b];
+                    nnb = nnb + 1;
+                }
+                p[o] = nnb > 0 ? psum / (double)nnb : 0.0;
+            }
+    theta_copy_d(&g, p);
+    free(act_l); free(act_g);
     return 0;
 }

@@ -110,3 +110,46 @@ def test_pp_assemble_rejects_bad_calls():
     with pytest.raises(_lib.EafpError) as e:
         eafp.pp_assemble(F, dict(cfg, dt=0.0), eafp.Mesh(4, 6, 5))
     assert e.value.rc == _lib.EAFP_ERR_ARG
+
+
+def test_pp_post_solve_struct_layouts():
+    # eafp_pp_exports: 9 pointers; eafp_pp_update: 13 pointers
+    assert ctypes.sizeof(_lib.PPExports) == 9 * 8
+    assert ctypes.sizeof(_lib.PPUpdate) == 13 * 8
+    import oracle
+    assert [f[0] for f in oracle.PPExports._fields_] == [f[0] for f in _lib.PPExports._fields_]
+
+
+def test_pp_correct_rejects_bad_calls():
+    """Argument checks of eafp_pp_correct run before any device call."""
+    from eafp import fields
+    shape = (4, 6, 5)
+    F, cfg = fields.make_fields(*shape)
+    pad = tuple(n + 4 for n in shape)
+    ex = {n: np.zeros(pad, dtype=np.int32 if n.startswith("lk") else np.float64, order="F") for n in _lib.PP_EXPORTS}
+    pp = np.zeros(pad, order="F")
+    with pytest.raises(_lib.EafpError) as e:
+        eafp.pp_correct(F, cfg, eafp.Mesh(*shape, is_parallel=True), 0.3, pp, ex)
+    assert e.value.rc == _lib.EAFP_ERR_UNSUPPORTED
+    G = dict(F)
+    del G["aPz_g"]
+    with pytest.raises(_lib.EafpError) as e:
+        eafp.pp_correct(G, cfg, eafp.Mesh(*shape), 0.3, pp, ex)
+    assert e.value.rc == _lib.EAFP_ERR_ARG
+    # a NULL export array, through the C-ABI directly
+    L = _lib.load()
+    f, c, keep = eafp.solver._pp_structs(F, cfg)
+    vals = {n: a.ctypes.data for n, a in ex.items()}
+    vals["ac_th"] = None
+    exs = _lib.PPExports(**vals)
+    io_arrays = {n: np.zeros(pad, dtype=np.int32 if n.startswith("lk") else np.float64, order="F")
+                 for n in _lib.PP_UPDATE}
+    io = _lib.PPUpdate(**{n: a.ctypes.data for n, a in io_arrays.items()})
+    d = eafp.solver.desc_of(eafp.Mesh(*shape))
+    rc = L.eafp_pp_correct(ctypes.byref(d), ctypes.byref(f), ctypes.byref(c), 0.3, pp.ctypes.data, ctypes.byref(exs),
+                           ctypes.byref(io), 0)
+    assert rc == _lib.EAFP_ERR_ARG
+    rc = L.eafp_pp_correct(ctypes.byref(d), ctypes.byref(f), ctypes.byref(c), 0.3, None, ctypes.byref(exs),
+                           ctypes.byref(io), 0)
+    assert rc == _lib.EAFP_ERR_ARG
+    del keep
