@@ -312,6 +312,8 @@ def main():
     ap.add_argument("--single", action="store_true",
                     help="with --kernels-only / --derive-traffic: the single 60x120x84 production system and the "
                          "FP64 drop-in's kernels (k64_spmv, k64_update) instead of the batch")
+    ap.add_argument("--assembly-only", action="store_true",
+                    help="only the p' assembly and post-solve leg (60x120x84 two-phase fields), for profiling")
     ap.add_argument("--derive-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write profiles/hbm_traffic.json from the rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
                          "outputs of two `bench.py --kernels-only` runs with the same options; no GPU use")
@@ -339,6 +341,12 @@ def main():
     from eafp import systems
     from eafp._lib import COEF_BF16, COEF_FP16, COEF_FP32
     fmt = {"fp32": COEF_FP32, "bf16": COEF_BF16, "fp16": COEF_FP16}[args.coef]
+
+    if args.assembly_only:
+        calibration_copy(CALIB_BYTES)
+        print(json.dumps({"assembly_only": True, "pp_assembly": measure_assembly(eafp),
+                          "calibration_copy_bytes": CALIB_BYTES}), flush=True)
+        return 0
 
     if args.kernels_only and args.single:
         # the drop-in's two FP64 kernels on the production system

@@ -63,6 +63,7 @@ struct PPArgs {
     double *gr, *gth, *gz, *gzg;
     double *aW, *aE, *aS, *aN, *aB, *aT, *aP, *Su;
     double* part;            // 2 per block: flux_ref, sum|Su|
+    int chunk;               // blocks per XCD run (cell_of_thread)
     double *Fs[3], *ac[3];   // liquid +face exports (r, theta, z); Fs[0] == nullptr: none
     int* lk[3];
 };
@@ -83,6 +84,32 @@ __device__ __forceinline__ bool act_g(const A& a, long o) {
 template <class A>
 __device__ __forceinline__ bool pv_cell(const A& a, long o) { return a.ct[o] != 0 && (act_l(a, o) || act_g(a, o)); }
 
+// Cell o's fluid flag and both activity flags, the same logic as above with
+// every load issued up front (no short-circuit between loads), so the loads
+// of several cells overlap instead of forming a chain.
+struct CellAct {
+    bool fluid, l, g;
+};
+
+template <class A>
+__device__ __forceinline__ CellAct cell_act(const A& a, long o) {
+    const int ct = a.ct[o];
+    const double al = a.al[o], ag = a.ag[o];
+    const int lc = a.liq_cont ? a.liq_cont[o] : 0;
+    const int pv = a.pv_active ? a.pv_active[o] : 1;
+    const int gr = a.gas_reach ? a.gas_reach[o] : 0;
+    CellAct c;
+    c.fluid = ct != 0;
+    c.l = a.liq_cont ? lc != 0 : al >= ALPHA_FLOW_CUTOFF;
+    bool g = ag >= ALPHA_FLOW_CUTOFF;
+    if (a.pv_active) g = g && pv != 0;
+    if (a.gas_reach) g = g || (gr != 0 && ag < ALPHA_FLOW_CUTOFF && ct != 0);
+    c.g = g;
+    return c;
+}
+
+__device__ __forceinline__ bool pv_of(const CellAct& c) { return c.fluid && (c.l || c.g); }
+
 __device__ __forceinline__ double pgrad(double pm, double p0, double pp_, double xm, double x0, double xp, bool okm,
                                         bool okp) {
     if (okm && okp) return (pp_ - pm) / (xp - xm);
@@ -102,13 +129,38 @@ struct Pos {
     int i, j, k;
 };
 
+// Padded position of cell o (the host keeps every padded length below 2^31,
+// so 32-bit division suffices).
 template <class A>
 __device__ __forceinline__ Pos padded_pos(const A& a, long o) {
-    const int k = (int)(o / a.sz);
-    const long rem = o - (long)k * a.sz;
-    const int j = (int)(rem / a.sy);
-    const int i = (int)(rem - (long)j * a.sy);
-    return {i - 1, j - 1, k - 1};
+    const unsigned u = (unsigned)o, sz = (unsigned)a.sz, sy = (unsigned)a.sy;
+    const unsigned k = u / sz;
+    const unsigned rem = u - k * sz;
+    const unsigned j = rem / sy;
+    const unsigned i = rem - j * sy;
+    return {(int)i - 1, (int)j - 1, (int)k - 1};
+}
+
+// Padded cell of this thread.  The grid is a multiple of 8*chunk blocks,
+// chunk = the blocks of two z-planes.  Block b runs on XCD x = b mod 8 and
+// the XCD's t-th block (t = b/8) takes logical block
+// ((t / chunk)*8 + x)*chunk + t mod chunk: each XCD sweeps runs of two
+// planes (the plane below the current one is in its own L2) and the runs
+// are dealt round-robin, so the XCDs share the heavy (fluid) and light
+// (solid, halo) planes evenly.
+template <class A>
+__device__ __forceinline__ long cell_of_thread(const A& a) {
+    const unsigned b = blockIdx.x, ch = (unsigned)a.chunk;
+    const unsigned t = b >> 3, sc = t / ch;
+    return (long)((sc * 8u + (b & 7u)) * ch + (t - sc * ch)) * BLOCK + threadIdx.x;
+}
+
+inline int pp_chunk(long sz) { return (int)(2 * ((sz + BLOCK - 1) / BLOCK)); }
+
+// Blocks for n padded cells of plane size sz: a multiple of 8*chunk.
+inline int pp_blocks(long n, long sz) {
+    const long q = 8L * pp_chunk(sz);
+    return (int)(((n + BLOCK - 1) / BLOCK + q - 1) / q * q);
 }
 
 template <class A>
@@ -117,7 +169,7 @@ __device__ __forceinline__ long at(const A& a, int i, int j, int k) {
 }
 
 __global__ __launch_bounds__(BLOCK) void k_pp_grad(PPArgs a) {
-    const long o = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const long o = cell_of_thread(a);
     if (o >= a.n) return;
     const Pos q = padded_pos(a, o);
     const int i = q.i, j = q.j, k = q.k;
@@ -129,11 +181,11 @@ __global__ __launch_bounds__(BLOCK) void k_pp_grad(PPArgs a) {
         const long oB = at(a, i, j, k - 1), oT = at(a, i, j, k + 1);
         const double* p = a.p;
         const double ri = a.r[i + 1];
-        gr = pgrad(p[oW], p[o], p[oE], a.r[i], a.r[i + 1], a.r[i + 2], i > 1 && pv_cell(a, oW),
-                   i < a.nr && pv_cell(a, oE));
-        gth = pgrad(p[oS], p[o], p[oN], ri * a.th[j], ri * a.th[j + 1], ri * a.th[j + 2], pv_cell(a, oS),
-                    pv_cell(a, oN));
-        const bool okm = k > 1 && pv_cell(a, oB), okp = k < a.nz && pv_cell(a, oT);
+        const CellAct cW = cell_act(a, oW), cE = cell_act(a, oE), cS = cell_act(a, oS);
+        const CellAct cN = cell_act(a, oN), cB = cell_act(a, oB), cT = cell_act(a, oT);
+        gr = pgrad(p[oW], p[o], p[oE], a.r[i], a.r[i + 1], a.r[i + 2], i > 1 && pv_of(cW), i < a.nr && pv_of(cE));
+        gth = pgrad(p[oS], p[o], p[oN], ri * a.th[j], ri * a.th[j + 1], ri * a.th[j + 2], pv_of(cS), pv_of(cN));
+        const bool okm = k > 1 && pv_of(cB), okp = k < a.nz && pv_of(cT);
         gz = pgrad(p[oB], p[o], p[oT], a.z[k], a.z[k + 1], a.z[k + 2], okm, okp);
         double gb = 0.0, gt = 0.0;
         if (okm) gb = (p[o] - p[oB]) / (a.z[k + 1] - a.z[k]) + liq_weight(a, oB, o);
@@ -167,10 +219,12 @@ struct Acc {
 };
 
 // The six faces of cell o for one phase (src/mod_pressure_3d.f90:486-607).
-__device__ void add_phase(const PPArgs& a, const Phase& ph, int i, int j, int k, long o, Acc& c) {
+// nb: the activity of the neighbours W, E, S, N, B, T.
+__device__ void add_phase(const PPArgs& a, const Phase& ph, int i, int j, int k, long o, const CellAct* nb, Acc& c) {
     const bool gas = ph.gas;
-    auto act = [&](long q) { return gas ? act_g(a, q) : act_l(a, q); };
-    auto link = [&](long q) { return a.ct[q] != 0 && act(q); };
+    bool lk[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) lk[q] = nb[q].fluid && (gas ? nb[q].g : nb[q].l);
     const double* aef = ph.alpha;
     const double* pf = a.p;
     const long oW = at(a, i - 1, j, k), oE = at(a, i + 1, j, k);
@@ -183,7 +237,7 @@ __device__ void add_phase(const PPArgs& a, const Phase& ph, int i, int j, int k,
     const double* gth = a.gth;
     const double* gz = ph.gz;
     double d_f, af, u_f, delta;
-    if (link(oW)) {
+    if (lk[0]) {
         d_f = 0.5 * (aef[o] * a.vol[o] / fmax(ph.aPr[o], SMALL) + aef[oW] * a.vol[oW] / fmax(ph.aPr[oW], SMALL));
         af = 0.5 * (aef[o] + aef[oW]);
         delta = a.r[i + 1] - a.r[i];
@@ -194,7 +248,7 @@ __device__ void add_phase(const PPArgs& a, const Phase& ph, int i, int j, int k,
             c.flux += fabs(af * u_f * a.Ar[oW]);
         }
     }
-    if (link(oE)) {
+    if (lk[1]) {
         d_f = 0.5 * (aef[o] * a.vol[o] / fmax(ph.aPr[o], SMALL) + aef[oE] * a.vol[oE] / fmax(ph.aPr[oE], SMALL));
         af = 0.5 * (aef[o] + aef[oE]);
         delta = a.r[i + 2] - a.r[i + 1];
@@ -210,7 +264,7 @@ __device__ void add_phase(const PPArgs& a, const Phase& ph, int i, int j, int k,
             }
         }
     }
-    if (link(oS)) {
+    if (lk[2]) {
         d_f = 0.5 * (aef[o] * a.vol[o] / fmax(ph.aPth[o], SMALL) + aef[oS] * a.vol[oS] / fmax(ph.aPth[oS], SMALL));
         af = 0.5 * (aef[o] + aef[oS]);
         delta = a.r[i + 1] * (a.th[j + 1] - a.th[j]);
@@ -221,7 +275,7 @@ __device__ void add_phase(const PPArgs& a, const Phase& ph, int i, int j, int k,
             c.flux += fabs(af * u_f * a.Ath[o]);
         }
     }
-    if (link(oN)) {
+    if (lk[3]) {
         d_f = 0.5 * (aef[o] * a.vol[o] / fmax(ph.aPth[o], SMALL) + aef[oN] * a.vol[oN] / fmax(ph.aPth[oN], SMALL));
         af = 0.5 * (aef[o] + aef[oN]);
         delta = a.r[i + 1] * (a.th[j + 2] - a.th[j + 1]);
@@ -237,7 +291,7 @@ __device__ void add_phase(const PPArgs& a, const Phase& ph, int i, int j, int k,
             }
         }
     }
-    if (link(oB)) {
+    if (lk[4]) {
         d_f = 0.5 * (aef[o] * a.vol[o] / fmax(ph.aPz[o], SMALL) + aef[oB] * a.vol[oB] / fmax(ph.aPz[oB], SMALL));
         af = 0.5 * (aef[o] + aef[oB]);
         delta = a.z[k + 1] - a.z[k];
@@ -249,7 +303,7 @@ __device__ void add_phase(const PPArgs& a, const Phase& ph, int i, int j, int k,
             c.flux += fabs(af * u_f * a.Az[oB]);
         }
     }
-    if (link(oT)) {
+    if (lk[5]) {
         d_f = 0.5 * (aef[o] * a.vol[o] / fmax(ph.aPz[o], SMALL) + aef[oT] * a.vol[oT] / fmax(ph.aPz[oT], SMALL));
         af = 0.5 * (aef[o] + aef[oT]);
         delta = a.z[k + 2] - a.z[k + 1];
@@ -270,7 +324,7 @@ __device__ void add_phase(const PPArgs& a, const Phase& ph, int i, int j, int k,
 
 __global__ __launch_bounds__(BLOCK) void k_pp_assemble(PPArgs a) {
     __shared__ double sh[2 * (BLOCK / 64)];
-    const long o = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const long o = cell_of_thread(a);
     double acc[2] = {0.0, 0.0};
     if (o < a.n) {
         const Pos q = padded_pos(a, o);
@@ -280,13 +334,16 @@ __global__ __launch_bounds__(BLOCK) void k_pp_assemble(PPArgs a) {
         double aP = 0.0;
         if (interior && a.ct[o] != 0) {
             const bool al = act_l(a, o), ag = act_g(a, o);
+            const CellAct nb[6] = {cell_act(a, at(a, i - 1, j, k)), cell_act(a, at(a, i + 1, j, k)),
+                                   cell_act(a, at(a, i, j - 1, k)), cell_act(a, at(a, i, j + 1, k)),
+                                   cell_act(a, at(a, i, j, k - 1)), cell_act(a, at(a, i, j, k + 1))};
             if (al) {
                 const Phase L = {a.al, a.url, a.uthl, a.uzl, a.aPrl, a.aPthl, a.aPzl, a.gz, false};
-                add_phase(a, L, i, j, k, o, c);
+                add_phase(a, L, i, j, k, o, nb, c);
             }
             if (a.two_phase && ag) {
                 const Phase Gp = {a.ag, a.urg, a.uthg, a.uzg, a.aPrg, a.aPthg, a.aPzg, a.gzg, true};
-                add_phase(a, Gp, i, j, k, o, c);
+                add_phase(a, Gp, i, j, k, o, nb, c);
             }
             if (a.compress && ag) {
                 const double src = a.ag[o] * a.vol[o] / a.dt *
@@ -397,6 +454,7 @@ struct PCArgs {
     double* p_out;             // the final pressure (may be p itself)
     double alpha_p;
     int two_phase;
+    int chunk;                 // blocks per XCD run (cell_of_thread)
 };
 
 // Periodic source plane of theta plane j (j itself for 1 <= j <= nth).  With
@@ -408,7 +466,7 @@ __device__ __forceinline__ int theta_src(int j, int nth) {
 }
 
 __global__ __launch_bounds__(BLOCK) void k_pp_correct(PCArgs a) {
-    const long o = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const long o = cell_of_thread(a);
     if (o >= a.n) return;
     const Pos q = padded_pos(a, o);
     const int i = q.i, j = q.j, k = q.k;
@@ -433,8 +491,9 @@ __global__ __launch_bounds__(BLOCK) void k_pp_correct(PCArgs a) {
         const long oW = at(a, i - 1, j, k), oE = at(a, i + 1, j, k);
         const long oS = at(a, i, j - 1, k), oN = at(a, i, j + 1, k);
         const long oB = at(a, i, j, k - 1), oT = at(a, i, j, k + 1);
+        const CellAct co = cell_act(a, o);
         for (int ph = 0; ph < (a.two_phase ? 2 : 1); ++ph) {
-            if (!(ph ? act_g(a, o) : act_l(a, o))) continue;
+            if (!(ph ? co.g : co.l)) continue;
             const double* alpha = ph ? a.ag : a.al;
             const double *aPr = a.aP[3 * ph], *aPth = a.aP[3 * ph + 1], *aPz = a.aP[3 * ph + 2];
             double *ur = a.vel[3 * ph], *uth = a.vel[3 * ph + 1], *uz = a.vel[3 * ph + 2];
@@ -457,27 +516,37 @@ __global__ __launch_bounds__(BLOCK) void k_pp_correct(PCArgs a) {
 }
 
 __global__ __launch_bounds__(BLOCK) void k_pp_seal(PCArgs a) {
-    const long o = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const long o = cell_of_thread(a);
     if (o >= a.n) return;
     const Pos q = padded_pos(a, o);
     const int i = q.i, k = q.k;
     const int js = theta_src(q.j, a.nth);
     const long s = at(a, i, js, k);
     double v = a.p_upd[s];
-    if (i >= 1 && i <= a.nr && k >= 1 && k <= a.nz && a.ct[s] != 0 && !act_l(a, s) && !act_g(a, s)) {
-        // sealed cell (:373-397): the neighbours W, E, S, N, B, T in this order
-        const long nb[6] = {at(a, i - 1, js, k), at(a, i + 1, js, k), at(a, i, js - 1, k),
-                            at(a, i, js + 1, k), at(a, i, js, k - 1), at(a, i, js, k + 1)};
-        double psum = 0.0;
-        int nnb = 0;
+    const bool rz = i >= 1 && i <= a.nr && k >= 1 && k <= a.nz;
+    if (rz) {
+        const CellAct cs = cell_act(a, s);
+        if (cs.fluid && !cs.l && !cs.g) {
+            // sealed cell (:373-397): the neighbours W, E, S, N, B, T in this order
+            const long nb[6] = {at(a, i - 1, js, k), at(a, i + 1, js, k), at(a, i, js - 1, k),
+                                at(a, i, js + 1, k), at(a, i, js, k - 1), at(a, i, js, k + 1)};
+            CellAct cb[6];
+            double pb[6];
 #pragma unroll
-        for (int m = 0; m < 6; ++m) {
-            const long b = nb[m];
-            if (a.ct[b] == 0 || (!act_l(a, b) && !act_g(a, b))) continue;
-            psum = psum + a.p_upd[b];
-            nnb = nnb + 1;
+            for (int m = 0; m < 6; ++m) {
+                cb[m] = cell_act(a, nb[m]);
+                pb[m] = a.p_upd[nb[m]];
+            }
+            double psum = 0.0;
+            int nnb = 0;
+#pragma unroll
+            for (int m = 0; m < 6; ++m) {
+                if (!pv_of(cb[m])) continue;
+                psum = psum + pb[m];
+                nnb = nnb + 1;
+            }
+            v = nnb > 0 ? psum / (double)nnb : 0.0;
         }
-        v = nnb > 0 ? psum / (double)nnb : 0.0;
     }
     a.p_out[o] = v;
 }
@@ -499,6 +568,7 @@ int ensure_pp(const eafp_desc* d) {
     if (!w.stream) EAFP_TRY(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
     if (w.nr == d->nr && w.nth == d->ntheta && w.nz == d->nz) return EAFP_OK;
     const long n = (long)(d->nr + 4) * (d->ntheta + 4) * (d->nz + 4);
+    if (n > (1L << 31) - 8L * BLOCK) return EAFP_ERR_UNSUPPORTED;   // 32-bit cell positions
     EAFP_TRY_RC(w.coord.alloc((size_t)(d->nr + d->ntheta + d->nz + 12)));
     EAFP_TRY_RC(w.f3.alloc((size_t)n * NF3));
     EAFP_TRY_RC(w.grads.alloc((size_t)n * 4));
@@ -509,7 +579,7 @@ int ensure_pp(const eafp_desc* d) {
     EAFP_TRY_RC(w.outlet.alloc((size_t)(d->nr + 4) * (d->ntheta + 4)));
     EAFP_TRY_RC(w.exd.alloc((size_t)n * 6));
     EAFP_TRY_RC(w.lki.alloc((size_t)n * 6));
-    const long blocks = (n + BLOCK - 1) / BLOCK;
+    const long blocks = pp_blocks(n, (long)(d->nr + 4) * (d->ntheta + 4));
     EAFP_TRY_RC(w.part.alloc((size_t)blocks * 2));
     w.part_h.assign((size_t)blocks * 2, 0.0);
     w.nr = d->nr;
@@ -627,7 +697,8 @@ int pp_assemble(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_confi
             a.ac[dd] = on_device ? ex_d[3 + dd] : w.exd.p + (long)(3 + dd) * n;
             a.lk[dd] = on_device ? ex_i[dd] : w.lki.p + (long)dd * n;
         }
-    const int blocks = (int)((n + BLOCK - 1) / BLOCK);
+    const int blocks = pp_blocks(n, a.sz);
+    a.chunk = pp_chunk(a.sz);
     k_pp_grad<<<blocks, BLOCK, 0, w.stream>>>(a);
     k_pp_assemble<<<blocks, BLOCK, 0, w.stream>>>(a);
     EAFP_TRY(hipGetLastError());
@@ -760,7 +831,8 @@ int pp_correct(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config
     a.al = din[1];
     a.ag = din[2];
     for (int q = 0; q < 6; ++q) a.aP[q] = din[3 + q];
-    const int blocks = (int)((n + BLOCK - 1) / BLOCK);
+    const int blocks = pp_blocks(n, a.sz);
+    a.chunk = pp_chunk(a.sz);
     k_pp_correct<<<blocks, BLOCK, 0, w.stream>>>(a);
     k_pp_seal<<<blocks, BLOCK, 0, w.stream>>>(a);
     EAFP_TRY(hipGetLastError());
