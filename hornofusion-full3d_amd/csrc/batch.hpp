@@ -119,6 +119,8 @@ int pp_assemble(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_confi
 int pp_correct(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config* cfg, double alpha_p,
                const double* pp, const eafp_pp_exports* ex, const eafp_pp_update* io, int on_device);
 void pp_release();
+int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* sys, int n_sweep, int on_device);
+void tdma_release();
 int choose_grid(int ntiles);
 int device_cus();
 bool valid_cpt(int cpt);

@@ -74,7 +74,20 @@ int eafp_release(void) {
         g_cache.reset();
     }
     eafp::pp_release();
+    eafp::tdma_release();
     return EAFP_OK;
+}
+
+int eafp_tdma_solve(const eafp_desc* desc, const double* aW, const double* aE, const double* aS, const double* aN,
+                    const double* aB, const double* aT, const double* aP, const double* Su, double* phi,
+                    int n_sweep, int arrays_on_device) {
+    const eafp_tdma_system s = {aW, aE, aS, aN, aB, aT, aP, Su, phi};
+    return eafp::tdma_solve_batch(1, desc, &s, n_sweep, arrays_on_device);
+}
+
+int eafp_tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* systems, int n_sweep,
+                          int arrays_on_device) {
+    return eafp::tdma_solve_batch(nsys, descs, systems, n_sweep, arrays_on_device);
 }
 
 int eafp_pp_assemble(const eafp_desc* desc, const eafp_pp_fields* fields, const eafp_pp_config* cfg, double* aW,

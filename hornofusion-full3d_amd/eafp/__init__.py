@@ -8,6 +8,7 @@ the build step (``build``).
 """
 from . import systems  # noqa: F401
 from ._lib import EafpError  # noqa: F401
-from .solver import Batch, Mesh, cg_3d_mpi, pp_assemble, pp_correct, release  # noqa: F401
+from .solver import Batch, Mesh, cg_3d_mpi, pp_assemble, pp_correct, release, tdma_3d_mpi, tdma_batch  # noqa: F401
 
-__all__ = ["Batch", "EafpError", "Mesh", "cg_3d_mpi", "pp_assemble", "pp_correct", "release", "systems"]
+__all__ = ["Batch", "EafpError", "Mesh", "cg_3d_mpi", "pp_assemble", "pp_correct", "release", "systems",
+           "tdma_3d_mpi", "tdma_batch"]

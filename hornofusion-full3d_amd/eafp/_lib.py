@@ -25,7 +25,7 @@ COEF_FP32, COEF_BF16, COEF_FP16 = 0, 1, 2
 
 # Every symbol declared in include/eafp.h.
 EXPORTS = (
-    "eafp_cg_solve", "eafp_release", "eafp_pp_assemble", "eafp_pp_correct", "eafp_batch_create", "eafp_batch_upload",
+    "eafp_cg_solve", "eafp_release", "eafp_pp_assemble", "eafp_pp_correct", "eafp_tdma_solve", "eafp_tdma_solve_batch", "eafp_batch_create", "eafp_batch_upload",
     "eafp_batch_solve_fp64", "eafp_batch_solve_mixed", "eafp_batch_download",
     "eafp_batch_info", "eafp_batch_set_tiling", "eafp_batch_set_columns", "eafp_batch_get_columns",
     "eafp_batch_time_kernels", "eafp_batch_autotune", "eafp_batch_destroy", "eafp_version",
@@ -57,6 +57,11 @@ class PPExports(ctypes.Structure):
 class PPUpdate(ctypes.Structure):
     """eafp_pp_update: what the post-solve consumers write."""
     _fields_ = [(n, ctypes.c_void_p) for n in PP_UPDATE]
+
+
+class TdmaSystem(ctypes.Structure):
+    """eafp_tdma_system: aW..aT, aP, Su and phi of one system."""
+    _fields_ = [(n, ctypes.c_void_p) for n in ("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su", "phi")]
 
 
 class PPConfig(ctypes.Structure):
@@ -137,6 +142,10 @@ def load():
     L.eafp_pp_correct.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(PPFields), ctypes.POINTER(PPConfig), d, vp,
                                   ctypes.POINTER(PPExports), ctypes.POINTER(PPUpdate), i]
     L.eafp_pp_correct.restype = i
+    L.eafp_tdma_solve.argtypes = [ctypes.POINTER(Desc)] + [vp] * 9 + [i, i]
+    L.eafp_tdma_solve.restype = i
+    L.eafp_tdma_solve_batch.argtypes = [i, ctypes.POINTER(Desc), ctypes.POINTER(TdmaSystem), i, i]
+    L.eafp_tdma_solve_batch.restype = i
     L.eafp_batch_set_columns.argtypes = [vp, i]
     L.eafp_batch_set_columns.restype = i
     L.eafp_batch_get_columns.argtypes = [vp, ip]

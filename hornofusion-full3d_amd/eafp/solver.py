@@ -147,6 +147,36 @@ def pp_correct(F, cfg, m, alpha_p, pp, ex):
     return out
 
 
+def tdma_3d_mpi(aW, aE, aS, aN, aB, aT, aP, Su, phi, m, n_sweep):
+    """Mirror of ``tdma_3d_mpi(aW, aE, aS, aN, aB, aT, aP, Su, phi, m, n_sweep)``
+    (src/mod_solver_3d.f90:252-366) on the GPU (eafp_tdma_solve): ``n_sweep``
+    line-by-line sweeps, ``phi`` (padded Fortran float64) updated in place,
+    halos included, exactly as the reference leaves it.  Raises on failure
+    (the reference's subroutine has no status to report one)."""
+    L = _lib.load()
+    d = desc_of(m)
+    shape = (m.nr, m.ntheta, m.nz)
+    ptrs = [_padded(a, shape, n) for a, n in zip((aW, aE, aS, aN, aB, aT, aP, Su), COEF_NAMES + ("Su",))]
+    ptrs.append(_padded(phi, shape, "phi", writable=True))
+    check(L.eafp_tdma_solve(ctypes.byref(d), *ptrs, int(n_sweep), 0), "eafp_tdma_solve")
+
+
+def tdma_batch(systems, meshes, n_sweep):
+    """``n_sweep`` sweeps on several independent systems at once
+    (eafp_tdma_solve_batch).  ``systems``: dicts with aW..aT, aP, Su and phi
+    (padded float64, phi updated in place); ``meshes``: one Mesh each."""
+    L = _lib.load()
+    nsys = len(systems)
+    descs = (_lib.Desc * nsys)(*[desc_of(m) for m in meshes])
+    names = COEF_NAMES + ("Su", "phi")
+    tab = (_lib.TdmaSystem * nsys)()
+    for q, (sy, m) in enumerate(zip(systems, meshes)):
+        shape = (m.nr, m.ntheta, m.nz)
+        vals = {n: _padded(sy[n], shape, n, writable=(n == "phi")) for n in names}
+        tab[q] = _lib.TdmaSystem(**vals)
+    check(L.eafp_tdma_solve_batch(nsys, descs, tab, int(n_sweep), 0), "eafp_tdma_solve_batch")
+
+
 def release():
     _lib.load().eafp_release()
 

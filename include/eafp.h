@@ -96,9 +96,9 @@ int eafp_cg_solve(const eafp_desc* desc,
                   const double* aB, const double* aT, const double* aP, const double* Su,
                   double* phi, int max_iter, double tol, double* residual, int* n_iter);
 
-/* Releases the device buffers cached by eafp_cg_solve, eafp_pp_assemble
- * and eafp_pp_correct.  No reference counterpart (the reference's workspace
- * lives until exit). */
+/* Releases the device buffers cached by eafp_cg_solve, eafp_pp_assemble,
+ * eafp_pp_correct and eafp_tdma_solve(_batch).  No reference counterpart
+ * (the reference's workspace lives until exit). */
 int eafp_release(void);
 
 /* ---------------------------------------------------------------------------
@@ -277,6 +277,37 @@ int eafp_batch_autotune(eafp_batch* b, int solver, int coef_format, int reps, in
                         int* blocks, float* ms_iter);
 
 int eafp_batch_destroy(eafp_batch* b);
+
+/* ---------------------------------------------------------------------------
+ * Line-by-line TDMA smoother: tdma_3d_mpi (src/mod_solver_3d.f90:252-366)
+ * on a serial mesh, with its Thomas solve `tdma` (:22-46).  Each sweep:
+ * the theta halo copy, the r lines (k outer, j inner), the copy, the theta
+ * lines (k outer, i inner), the copy, the z lines (j outer, i inner); each
+ * line takes the other directions' neighbours from phi as it stands and its
+ * two end neighbours from the halos; no copy follows the last z pass.  The
+ * GPU keeps this order exactly: lines on one anti-diagonal of the line grid
+ * never neighbour each other, so they are solved together, one diagonal
+ * after another.  Every value of phi (halos included) is bit-identical to a
+ * sequential FP64 evaluation of the reference's expressions without
+ * contraction.  Serial meshes only (EAFP_ERR_UNSUPPORTED otherwise);
+ * n_sweep = 0 leaves phi as it is.
+ * ------------------------------------------------------------------------- */
+int eafp_tdma_solve(const eafp_desc* desc, const double* aW, const double* aE, const double* aS, const double* aN,
+                    const double* aB, const double* aT, const double* aP, const double* Su, double* phi,
+                    int n_sweep, int arrays_on_device);
+
+/* One system of a batched TDMA call (padded Fortran arrays). */
+typedef struct {
+    const double *aW, *aE, *aS, *aN, *aB, *aT, *aP, *Su;
+    double* phi;
+} eafp_tdma_system;
+
+/* n_sweep sweeps of eafp_tdma_solve on each of nsys independent systems of
+ * any shapes (the momentum components, energy, species, ... of one time
+ * step), solved concurrently.  No reference counterpart (the reference
+ * calls tdma_3d_mpi once per equation). */
+int eafp_tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* systems, int n_sweep,
+                          int arrays_on_device);
 
 /* Library version string. */
 const char* eafp_version(void);

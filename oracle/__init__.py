@@ -10,9 +10,13 @@ Contents
   ``matvec`` (``:588-605``), ``dot`` (``:607-618``) and the serial theta halo
   copy (``src/mod_mpi_topology.f90:303-310``), on the reference's padded
   Fortran layout (bounds ``(-1:n+2)``, r fastest).
-* ``pp_assemble``: ctypes binding to ``oracle_pp_assemble`` (``pp_assemble.c``),
+* ``pp_assemble`` / ``pp_correct``: ctypes bindings to ``pp_assemble.c``,
   the restatement of the p' system assembly of ``solve_pressure_correction``
-  (``src/mod_pressure_3d.f90:171-306``, ``src/mod_boundary_3d.f90:180-277``).
+  (``src/mod_pressure_3d.f90:171-306``, ``src/mod_boundary_3d.f90:180-277``)
+  and of its post-solve steps (``:324-397``, ``:689-746``).
+* ``tdma`` / ``tdma_3d``: ctypes bindings to ``tdma.c``, the restatement of
+  the Thomas algorithm and of the line-by-line smoother ``tdma_3d_mpi``
+  (``src/mod_solver_3d.f90:22-46``, ``:252-366``).
 * numpy helpers that are independent of the C code and are used to pin it:
   a scipy CSR operator built from the interior coefficient arrays, the
   per-component zero-mean projection, and gauge-fixed error metrics.
@@ -44,18 +48,16 @@ _dp = ctypes.POINTER(ctypes.c_double)
 
 
 def build(force: bool = False) -> str:
-    """Compile ``liboracle.so`` and ``liboracle_omp.so`` with the committed Makefile."""
-    if force or not os.path.exists(_LIB_PATH) or not os.path.exists(_OMP_PATH):
-        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True,
-                       stdout=subprocess.DEVNULL)
+    """Compile ``liboracle.so`` and ``liboracle_omp.so`` with the committed
+    Makefile (make rebuilds whatever is older than its sources)."""
+    subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True, stdout=subprocess.DEVNULL)
     return _LIB_PATH
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB_PATH):
-            build()
+        build()
         L = ctypes.CDLL(_LIB_PATH)
         i = ctypes.c_int
         L.oracle_cg.argtypes = [i, i, i] + [_dp] * 8 + [_dp, i, ctypes.c_double, _dp,
@@ -78,6 +80,10 @@ def lib():
         L.oracle_pp_correct.argtypes = [i, i, i, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, _dp,
                                         ctypes.c_void_p, _dp, _dp, _dp, ip, ip, ip, ctypes.POINTER(_dp), _dp]
         L.oracle_pp_correct.restype = ctypes.c_int
+        L.oracle_tdma.argtypes = [_dp] * 5 + [i]
+        L.oracle_tdma.restype = ctypes.c_int
+        L.oracle_tdma_3d.argtypes = [i, i, i] + [_dp] * 9 + [i]
+        L.oracle_tdma_3d.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -188,6 +194,36 @@ def pp_correct(F, cfg, shape, alpha_p, pp, ex):
     del keep
     if rc != 0:
         raise MemoryError("oracle_pp_correct could not allocate its work arrays")
+    return out
+
+
+def tdma(a, b, c, d):
+    """The restated Thomas algorithm (``tdma``, src/mod_solver_3d.f90:22-46):
+    x with a(i) x(i-1) + b(i) x(i) + c(i) x(i+1) = d(i)."""
+    arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, c, d)]
+    n = arrs[0].size
+    x = np.empty(n)
+    if lib().oracle_tdma(*[v.ctypes.data_as(_dp) for v in arrs], x.ctypes.data_as(_dp), n) != 0:
+        raise ValueError("oracle_tdma: n < 1 or no memory")
+    return x
+
+
+TDMA_COEF = ("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su")
+
+
+def tdma_3d(coef, phi, shape, n_sweep):
+    """The restated ``tdma_3d_mpi`` on a serial mesh: ``coef`` maps aW..aT,
+    aP, Su to padded Fortran arrays; returns the updated copy of ``phi``
+    (padded, every halo as the reference leaves it)."""
+    nr, nth, nz = shape
+    arrs = [np.asfortranarray(coef[n], dtype=np.float64) for n in TDMA_COEF]
+    out = np.array(phi, dtype=np.float64, order="F")
+    for v in arrs + [out]:
+        assert v.shape == (nr + 4, nth + 4, nz + 4)
+    rc = lib().oracle_tdma_3d(nr, nth, nz, *[v.ctypes.data_as(_dp) for v in arrs], out.ctypes.data_as(_dp),
+                              int(n_sweep))
+    if rc != 0:
+        raise ValueError("oracle_tdma_3d: bad shape or no memory")
     return out
 
 

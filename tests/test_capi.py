@@ -153,3 +153,26 @@ def test_pp_correct_rejects_bad_calls():
                            ctypes.byref(io), 0)
     assert rc == _lib.EAFP_ERR_ARG
     del keep
+
+
+def test_tdma_rejects_bad_calls():
+    """Argument checks of the line TDMA run before any device call; n_sweep = 0 is a no-op."""
+    shape = (4, 6, 5)
+    a = _arrays()
+    with pytest.raises(_lib.EafpError) as e:
+        eafp.tdma_3d_mpi(*a[:8], a[8], eafp.Mesh(*shape, is_parallel=True), 2)
+    assert e.value.rc == _lib.EAFP_ERR_UNSUPPORTED
+    with pytest.raises(_lib.EafpError) as e:
+        eafp.tdma_3d_mpi(*a[:8], a[8], eafp.Mesh(*shape), -1)
+    assert e.value.rc == _lib.EAFP_ERR_ARG
+    phi = a[8].copy(order="F")
+    eafp.tdma_3d_mpi(*a[:8], phi, eafp.Mesh(*shape), 0)
+    np.testing.assert_array_equal(phi, a[8])
+    L = _lib.load()
+    d = (_lib.Desc * 2)(eafp.solver.desc_of(eafp.Mesh(*shape)), eafp.solver.desc_of(eafp.Mesh(*shape)))
+    vals = {n: x.ctypes.data for n, x in zip(("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su", "phi"), a)}
+    good = _lib.TdmaSystem(**vals)
+    vals["aN"] = None
+    tab = (_lib.TdmaSystem * 2)(good, _lib.TdmaSystem(**vals))
+    assert L.eafp_tdma_solve_batch(2, d, tab, 1, 0) == _lib.EAFP_ERR_ARG
+    assert L.eafp_tdma_solve_batch(0, d, tab, 1, 0) == _lib.EAFP_ERR_ARG

@@ -13,6 +13,7 @@ import pytest
 import eafp
 import oracle
 from eafp import fields
+from hipbuf import HipBuffers
 
 pytestmark = pytest.mark.gpu
 
@@ -143,46 +144,6 @@ def test_post_solve_matches_oracle_bitwise(shape, kw, alpha_p, scale):
         assert sealed > 0
 
 
-class _Hip:
-    """Device buffers from the HIP runtime libeafp.so itself links
-    (/opt/rocm's libamdhip64): torch's wheel carries another HIP runtime,
-    and in one process with libeafp's it has failed to find the GPU."""
-
-    def __init__(self):
-        import ctypes
-        self.ct = ctypes
-        self.h = ctypes.CDLL("libamdhip64.so")
-        self.bufs = []
-
-    def alloc(self, nbytes):
-        ptr = self.ct.c_void_p()
-        assert self.h.hipMalloc(self.ct.byref(ptr), self.ct.c_size_t(nbytes)) == 0
-        self.bufs.append(ptr)
-        return ptr.value
-
-    def put(self, a):
-        a = np.ascontiguousarray(a.ravel(order="F"))
-        p = self.alloc(a.nbytes)
-        assert self.h.hipMemcpy(self.ct.c_void_p(p), self.ct.c_void_p(a.ctypes.data), self.ct.c_size_t(a.nbytes), 1) == 0
-        return p
-
-    def zeros(self, nbytes):
-        p = self.alloc(nbytes)
-        assert self.h.hipMemset(self.ct.c_void_p(p), 0, self.ct.c_size_t(nbytes)) == 0
-        assert self.h.hipDeviceSynchronize() == 0    # libeafp's stream does not wait for the null stream
-        return p
-
-    def get(self, p, pad, dtype):
-        a = np.empty(int(np.prod(pad)), dtype=dtype)
-        assert self.h.hipMemcpy(self.ct.c_void_p(a.ctypes.data), self.ct.c_void_p(p), self.ct.c_size_t(a.nbytes), 2) == 0
-        return a.reshape(pad, order="F")
-
-    def free(self):
-        for p in self.bufs:
-            self.h.hipFree(p)
-        self.bufs = []
-
-
 def test_device_resident_assemble_and_correct():
     """arrays_on_device = 1: every array pointer a device pointer."""
     import ctypes
@@ -194,7 +155,7 @@ def test_device_resident_assemble_and_correct():
     upd_ref = oracle.pp_correct(F, cfg, shape, 0.4, pp, ex_ref)
     pad = tuple(n + 4 for n in shape)
     npad = int(np.prod(pad))
-    hip = _Hip()
+    hip = HipBuffers()
     try:
         vals = {n: (hip.put(np.asarray(F[n], dtype=np.int32 if n in _lib.PP_INT else np.float64)) if n in F else None)
                 for n in _lib.PP_FIELDS}
