@@ -284,6 +284,68 @@ def measure_assembly(eafp, shape=(60, 120, 84), reps=20):
     return rec
 
 
+def measure_tdma(eafp, batches=(1, 8, 64), n_sweep=3, reps=5):
+    """The line TDMA (eafp_tdma_solve_batch, device-resident) on the 60x120x84
+    production p' system: one system, the 8 equations of a two-phase time
+    step side by side, and 64 systems; beside the oracle's sequential
+    tdma_3d_mpi on one host core (the reference's smoother is serial per
+    rank).  Sweep time = call time / n_sweep; the pivot pass (once per call)
+    is included."""
+    import ctypes
+    from eafp import _lib, systems
+    ps = systems.make_system(2)
+    shape = ps.shape
+    names = ("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su")
+    arrs = [np.asfortranarray(ps.padded(n)) for n in names]
+    npad = arrs[0].size
+    hip = ctypes.CDLL("libamdhip64.so")
+    L = _lib.load()
+    rec = {"workload": f"tdma_3d_mpi, {n_sweep} sweeps of the 60x120x84 production p' system (604,800 cells), "
+                       "device-resident; B independent copies solved side by side",
+           "n_sweep": n_sweep, "batches": []}
+    for B in batches:
+        bufs = []
+
+        def dev(a):
+            ptr = ctypes.c_void_p()
+            _lib.check(hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(a.nbytes)), "hipMalloc")
+            _lib.check(hip.hipMemcpy(ptr, ctypes.c_void_p(a.ctypes.data), ctypes.c_size_t(a.nbytes), 1), "hipMemcpy")
+            bufs.append(ptr)
+            return ptr.value
+
+        zero = np.zeros(npad)
+        tab = (_lib.TdmaSystem * B)()
+        for q in range(B):
+            tab[q] = _lib.TdmaSystem(*[dev(a) for a in arrs], dev(zero))
+        descs = (_lib.Desc * B)(*[eafp.solver.desc_of(eafp.Mesh(*shape)) for _ in range(B)])
+        call = lambda: _lib.check(L.eafp_tdma_solve_batch(B, descs, tab, n_sweep, 1), "eafp_tdma_solve_batch")
+        call()
+        t = time.perf_counter()
+        for _ in range(reps):
+            call()
+        dt = (time.perf_counter() - t) / reps
+        for ptr in bufs:
+            hip.hipFree(ptr)
+        sweep_ms = dt / n_sweep * 1e3
+        # unique bytes per cell and pass: the six arrays a pass reads (Su, four off-line
+        # coefficients, the line's minus coefficient), pivot and c', phi read and written,
+        # the forward scratch written and read: 96 B; three passes per sweep
+        gbps = B * ps.ncells * 3 * 96 / (sweep_ms * 1e-3) / 1e9
+        rec["batches"].append({"systems": B, "ms_per_call": round(dt * 1e3, 3), "ms_per_sweep": round(sweep_ms, 3),
+                               "system_sweeps_per_s": round(B / (sweep_ms * 1e-3), 1),
+                               "algorithmic_GBps": round(gbps, 1)})
+    sys.path.insert(0, ROOT)
+    import oracle
+    oracle.lib()
+    coef = dict(zip(names, arrs))
+    t = time.perf_counter()
+    oracle.tdma_3d(coef, np.zeros_like(arrs[0], order="F"), shape, 1)
+    cpu = time.perf_counter() - t
+    rec["cpu_oracle_ms_per_sweep_1_core"] = round(cpu * 1e3, 2)
+    rec["speedup_single_vs_cpu"] = round(cpu * 1e3 / rec["batches"][0]["ms_per_sweep"], 1)
+    return rec
+
+
 def rel_err(x, xr, ps):
     from eafp.systems import project_zero_mean
     px = project_zero_mean(x, ps.labels, ps.ncomp)
@@ -314,6 +376,8 @@ def main():
                          "FP64 drop-in's kernels (k64_spmv, k64_update) instead of the batch")
     ap.add_argument("--assembly-only", action="store_true",
                     help="only the p' assembly and post-solve leg (60x120x84 two-phase fields), for profiling")
+    ap.add_argument("--tdma-only", action="store_true",
+                    help="only the line-TDMA leg (60x120x84, 1/8/64 systems), for profiling")
     ap.add_argument("--derive-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write profiles/hbm_traffic.json from the rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
                          "outputs of two `bench.py --kernels-only` runs with the same options; no GPU use")
@@ -341,6 +405,10 @@ def main():
     from eafp import systems
     from eafp._lib import COEF_BF16, COEF_FP16, COEF_FP32
     fmt = {"fp32": COEF_FP32, "bf16": COEF_BF16, "fp16": COEF_FP16}[args.coef]
+
+    if args.tdma_only:
+        print(json.dumps({"tdma_only": True, "line_tdma": measure_tdma(eafp)}), flush=True)
+        return 0
 
     if args.assembly_only:
         calibration_copy(CALIB_BYTES)
@@ -542,6 +610,8 @@ def main():
     if not args.no_extras:
         out["pp_assembly"] = measure_assembly(eafp)
         log("p' assembly done")
+        out["line_tdma"] = measure_tdma(eafp)
+        log("line TDMA done")
         # the other workloads of the metric's table, mixed against FP64 on the same GPU
         out["other_workloads"] = []
         syss3 = [systems.make_system(3, s) for s in range(256)]

@@ -1,35 +1,45 @@
 // tdma.hip -- EAF3D's line-by-line TDMA smoother on the GPU: tdma_3d_mpi
 // (src/mod_solver_3d.f90:252-366) with its Thomas solve (:22-46), for one
-// serial mesh or a batch of independent ones, in the caller's padded Fortran
-// layout (bounds (-1:n+2), r fastest).
+// serial mesh or a batch of independent ones; the caller's arrays are in the
+// padded Fortran layout (bounds (-1:n+2), r fastest).
 //
 // The reference sweeps lines in a fixed order and each line reads the
 // neighbours of the other directions from phi as it stands: the lines before
 // it in that order are already updated, the ones after it are not.  For the
 // r pass (loops k outer, j inner) line (j,k) therefore needs lines (j-1,k)
 // and (j,k-1) done and (j+1,k), (j,k+1) not started; the theta wrap is read
-// through the halo copied before the pass, so it never creates a dependence.
-// All lines with the same j + k (one anti-diagonal of the line grid) are
-// mutually independent: solving the diagonals one after another, the lines of
-// a diagonal in parallel, performs the same arithmetic on the same values as
-// the sequential loops.  The theta pass (k outer, i inner) and the z pass
-// (j outer, i inner) are scheduled the same way over (i,k) and (i,j).
+// through the halo copied before the pass.  All lines with the same j + k
+// (one anti-diagonal of the line grid) are mutually independent: solving the
+// diagonals one after another, the lines of a diagonal in parallel, performs
+// the same arithmetic on the same values as the sequential loops.  The theta
+// pass (k outer, i inner) and the z pass (j outer, i inner) are scheduled the
+// same way over (i,k) and (i,j).
 //
-// Kernels:
-//   k_tdma_pivots  the pivots m(i) and c'(i) of every line of the three
-//                  passes: they depend on the coefficients only, so they are
-//                  computed once per call for all lines at once, with the
-//                  reference's expressions and clamp (|m| < 1e-30 becomes
-//                  sign(1e-30, m)), and reused by every sweep
-//   k_tdma_sweeps  one work-group per system runs all sweeps: the theta halo
-//                  copy, then each pass one diagonal at a time (one thread
-//                  per line, a work-group barrier between diagonals).  Per
-//                  line: forward elimination d'(i) = (d(i) - a(i) d'(i-1)) /
-//                  m(i), back substitution x(i) = d'(i) - c'(i) x(i+1).
-// The critical path of one sweep is (nth+nz-1) r lines + (nr+nz-1) theta
-// lines + (nr+nth-1) z lines, each a chain of dependent FP64 divisions; a
-// single system is latency-bound by design, and independent systems (the
-// equations of one time step) fill the GPU side by side.
+// Each pass keeps its own copy of phi and of its coefficients in a
+// diagonal-major layout (tdma_line.hpp): for every element index the lines of
+// one diagonal are contiguous, so a work-group loads a whole diagonal
+// coalesced whatever the pass direction.  Kernels:
+//   k_tdma_setup   once per call, one thread per line of each pass: packs the
+//                  right-hand-side coefficients, a(e), the pivots m(e) and
+//                  c'(e) of the forward elimination (they depend on the
+//                  coefficients only), the wall halo lines, and pass 0's phi
+//   k_tdma_pass<P> one 1024-thread work-group per system runs one pass: per
+//                  diagonal, all threads stage d(e) (the right-hand side from
+//                  the neighbour lines, whose indices follow from the
+//                  diagonal starts), a(e), m(e) and c'(e) of the diagonal's
+//                  lines into 160 KiB of LDS with many loads in flight, then
+//                  one thread per line runs the dependent Thomas recurrences
+//                  from LDS, a chunk of elements read ahead; a barrier
+//                  between diagonals.  A diagonal too long for the LDS is taken in
+//                  groups of lines.
+//   k_tdma_move    moves phi between the passes' layouts (and into the
+//                  caller's array at the end), all CUs
+//   k_tdma_wall_halo  the theta halo planes of the caller's array on the r/z
+//                  wall columns
+// A single system is latency-bound by design: one CU walks the
+// (nth+nz-1) + (nr+nz-1) + (nr+nth-1) diagonals of a sweep, each a chain of
+// dependent FP64 divisions; independent systems (the equations of one time
+// step) run on other CUs side by side.
 //
 // Built without floating-point contraction, with the reference's operand
 // order, so every value of phi is the double the sequential loops compute.
@@ -45,67 +55,97 @@ namespace eafp {
 
 namespace {
 
-constexpr int TD_BLOCK = 256;          // threads per system in the sweep kernel
+constexpr int TD_BLOCK = 256;          // threads per work-group of the all-CU kernels
+constexpr int TD_PASS_BLOCK = 1024;    // threads of a pass work-group (one per system)
 
 using namespace tdma;
 
-__global__ __launch_bounds__(TD_BLOCK) void k_tdma_pivots(const TdSys* __restrict__ sys) {
-    const TdSys s = sys[blockIdx.y];
-    int p, u, v;
-    if (pivot_line(s, (long)blockIdx.x * TD_BLOCK + threadIdx.x, p, u, v)) line_pivots(s, p, u, v);
+__global__ __launch_bounds__(TD_BLOCK) void k_tdma_setup(const TdSys* __restrict__ sys) {
+    const TdSys& s = sys[blockIdx.y];
+    long t = (long)blockIdx.x * TD_BLOCK + threadIdx.x;
+    for (int P = 0; P < 3; ++P) {
+        const TdPass& p = s.ps[P];
+        if (t < p.NL) {
+            int u, v;
+            line_at(p, (int)t, u, v);
+            setup_line(s, P, u, v);
+            return;
+        }
+        t -= p.NL;
+    }
 }
 
-// The serial periodic copy by the whole work-group, one (i,k) column per thread.
-__device__ void theta_copy(const TdSys& s) {
-    const int ni = s.nr + 4, nk = s.nz + 4;
-    for (int t = threadIdx.x; t < ni * nk; t += TD_BLOCK) theta_copy_column(s, t % ni - 1, t / ni - 1);
-}
-
-// One pass over the line grid nu x nv (line (u,v)), diagonal w = (u-1)+(v-1)
-// at a time, in the order of the reference's loops (v outer, u inner).
 template <int P>
-__device__ void run_pass(const TdSys& s, const double* __restrict__ const* c, int nu, int nv) {
-    for (int w = 0; w <= nu + nv - 2; ++w) {
-        int v_lo, v_hi;
-        diagonal_range(w, nu, nv, v_lo, v_hi);
-        for (int v = v_lo + (int)threadIdx.x; v <= v_hi; v += TD_BLOCK) solve_line<P>(s, c, w + 2 - v, v);
+__global__ __launch_bounds__(TD_PASS_BLOCK) void k_tdma_pass(const TdSys* __restrict__ sys) {
+    __shared__ double sd[TD_LDS_ELEMS], sa[TD_LDS_ELEMS], sm[TD_LDS_ELEMS], sc[TD_LDS_ELEMS];
+    const TdSys& s = sys[blockIdx.x];
+    const TdPass& p = s.ps[P];
+    if (P != 1) {     // the halo copy before the pass: snapshot of the lines the wrap reads
+        const long nsnap = (long)p.n * (P == 0 ? p.nv : p.nu);
+        for (long t = threadIdx.x; t < nsnap; t += TD_PASS_BLOCK) snapshot_value(p, P, t);
         __syncthreads();
+    }
+    const int gmax = group_lines(p.n);
+    for (int w = 0; w <= p.nu + p.nv - 2; ++w) {
+        const DiagCtx dc = diag_ctx(p, w);
+        int v_lo, v_hi;
+        diagonal_range(w, p.nu, p.nv, v_lo, v_hi);
+        const int m = v_hi - v_lo + 1, l0 = p.dstart[w];
+        for (int g0 = 0; g0 < m; g0 += gmax) {
+            const int gm = min(gmax, m - g0), cnt = gm * p.n;
+#pragma unroll 2
+            for (int idx = threadIdx.x; idx < cnt; idx += TD_PASS_BLOCK) {
+                const int q = idx % gm, e = idx / gm;
+                const int v = v_lo + g0 + q, u = w + 2 - v;
+                int nb[4];
+                neighbours_in(p, P, dc, u, v, nb);
+                stage_elem<P>(p, l0 + g0 + q, e, nb, sd, sa, sm, sc, idx);
+            }
+            __syncthreads();
+            for (int q = threadIdx.x; q < gm; q += TD_PASS_BLOCK) solve_staged(p, l0 + g0 + q, sd, sa, sm, sc, q, gm);
+            __syncthreads();
+        }
     }
 }
 
-__global__ __launch_bounds__(TD_BLOCK) void k_tdma_sweeps(const TdSys* __restrict__ sys, int n_sweep) {
-    const TdSys s = sys[blockIdx.x];
-    const double* c[TD_NCOEF];
-#pragma unroll
-    for (int q = 0; q < TD_NCOEF; ++q) c[q] = s.c[q];
-    for (int sweep = 0; sweep < n_sweep; ++sweep) {
-        theta_copy(s);
-        __syncthreads();
-        run_pass<0>(s, c, s.nth, s.nz);     // r lines (j,k)
-        theta_copy(s);
-        __syncthreads();
-        run_pass<1>(s, c, s.nr, s.nz);      // theta lines (i,k)
-        theta_copy(s);
-        __syncthreads();
-        run_pass<2>(s, c, s.nr, s.nth);     // z lines (i,j)
-    }
+template <int P, int Q>
+__global__ __launch_bounds__(TD_BLOCK) void k_tdma_move(const TdSys* __restrict__ sys) {
+    const TdSys& s = sys[blockIdx.y];
+    const long c = (long)blockIdx.x * TD_BLOCK + threadIdx.x;
+    if (c < s.ncells) move_cell(s, P, Q, c);
+}
+
+__global__ __launch_bounds__(TD_BLOCK) void k_tdma_wall_halo(const TdSys* __restrict__ sys) {
+    const TdSys& s = sys[blockIdx.y];
+    const long t = (long)blockIdx.x * TD_BLOCK + threadIdx.x;
+    if (t < (long)(s.nr + 4) * (s.nz + 4)) wall_halo_column(s, t);
 }
 
 struct TdWork {
     hipStream_t stream = nullptr;
-    DevBuf<double> work;       // pivots and scratch of every system: 7 n each
-    DevBuf<double> stage;      // host arrays: 9 n each (8 coefficients, phi)
+    DevBuf<double> work;       // packed arrays and pass copies of phi of every system
+    DevBuf<int> iwork;         // diagonal starts
+    DevBuf<double> stage;      // host arrays: 9 padded arrays per system
     DevBuf<TdSys> table;
 };
 
 std::mutex g_td_mutex;
 TdWork* g_td = nullptr;
 
+// Line grid of pass P: (nu, nv, n).
+void pass_shape(const eafp_desc& d, int P, int& nu, int& nv, int& n) {
+    if (P == 0) { nu = d.ntheta; nv = d.nz; n = d.nr; }
+    else if (P == 1) { nu = d.nr; nv = d.nz; n = d.ntheta; }
+    else { nu = d.nr; nv = d.ntheta; n = d.nz; }
+}
+
+int halo_lines(int P, int nu, int nv) { return P == 0 ? 2 * nu + nv : P == 1 ? 2 * nv + 2 * nu : 2 * nv + nu; }
+
 }  // namespace
 
 int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* sys, int n_sweep, int on_device) {
     if (nsys < 1 || !descs || !sys || n_sweep < 0) return EAFP_ERR_ARG;
-    long total = 0, max_lines = 0;
+    long dwork = 0, iwork = 0, stage = 0, max_lines = 0, max_cells = 0, max_cols = 0;
     for (int q = 0; q < nsys; ++q) {
         EAFP_TRY_RC(validate_desc(&descs[q]));
         const eafp_tdma_system& y = sys[q];
@@ -113,19 +153,36 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
         for (const double* p : ptrs)
             if (!p) return EAFP_ERR_ARG;
         const eafp_desc& d = descs[q];
-        total += (long)(d.nr + 4) * (d.ntheta + 4) * (d.nz + 4);
-        max_lines = std::max(max_lines, (long)d.ntheta * d.nz + (long)d.nr * d.nz + (long)d.nr * d.ntheta);
+        const long ncells = (long)d.nr * d.ntheta * d.nz;
+        if (std::max(d.nr, std::max(d.ntheta, d.nz)) > TD_LDS_ELEMS) return EAFP_ERR_UNSUPPORTED;
+        long lines = 0;
+        for (int P = 0; P < 3; ++P) {
+            int nu, nv, n;
+            pass_shape(d, P, nu, nv, n);
+            const long NL = (long)nu * nv, NLX = NL + halo_lines(P, nu, nv);
+            dwork += PK_N * ncells + LN_N * NL + (long)n * NLX;
+            iwork += nu + nv - 1;
+            lines += NL;
+        }
+        stage += 9 * (long)(d.nr + 4) * (d.ntheta + 4) * (d.nz + 4);
+        max_lines = std::max(max_lines, lines);
+        max_cells = std::max(max_cells, ncells);
+        max_cols = std::max(max_cols, (long)(d.nr + 4) * (d.nz + 4));
     }
     if (n_sweep == 0) return EAFP_OK;
     std::lock_guard<std::mutex> lock(g_td_mutex);
     if (!g_td) g_td = new TdWork();
     TdWork& w = *g_td;
     if (!w.stream) EAFP_TRY(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
-    if (w.work.n < (size_t)(7 * total)) EAFP_TRY_RC(w.work.alloc((size_t)(7 * total)));
-    if (!on_device && w.stage.n < (size_t)(9 * total)) EAFP_TRY_RC(w.stage.alloc((size_t)(9 * total)));
+    if (w.work.n < (size_t)dwork) EAFP_TRY_RC(w.work.alloc((size_t)dwork));
+    if (w.iwork.n < (size_t)iwork) EAFP_TRY_RC(w.iwork.alloc((size_t)iwork));
+    if (!on_device && w.stage.n < (size_t)stage) EAFP_TRY_RC(w.stage.alloc((size_t)stage));
     if (w.table.n < (size_t)nsys) EAFP_TRY_RC(w.table.alloc((size_t)nsys));
     std::vector<TdSys> tab((size_t)nsys);
-    long off = 0;
+    std::vector<int> ihost((size_t)iwork);
+    double* dcur = w.work.p;
+    int* icur = w.iwork.p;
+    long ioff = 0, soff = 0;
     for (int q = 0; q < nsys; ++q) {
         const eafp_desc& d = descs[q];
         const eafp_tdma_system& y = sys[q];
@@ -137,32 +194,67 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
         t.sy = d.nr + 4;
         t.sz = (long)(d.nr + 4) * (d.ntheta + 4);
         t.n = t.sz * (d.nz + 4);
+        t.ncells = (long)d.nr * d.ntheta * d.nz;
         const double* src[TD_NCOEF] = {y.aW, y.aE, y.aS, y.aN, y.aB, y.aT, y.aP, y.Su};
         if (on_device) {
-            for (int c = 0; c < TD_NCOEF; ++c) t.c[c] = src[c];
-            t.phi = y.phi;
+            for (int c = 0; c < TD_NCOEF; ++c) t.c[c] = (cgdptr)src[c];
+            t.home = (gdptr)y.phi;
         } else {
-            double* base = w.stage.p + 9 * off;
+            double* base = w.stage.p + soff;
             for (int c = 0; c < TD_NCOEF; ++c) {
                 EAFP_TRY(hipMemcpyAsync(base + c * t.n, src[c], sizeof(double) * t.n, hipMemcpyHostToDevice,
                                         w.stream));
-                t.c[c] = base + c * t.n;
+                t.c[c] = (cgdptr)(base + c * t.n);
             }
-            t.phi = base + TD_NCOEF * t.n;
-            EAFP_TRY(hipMemcpyAsync(t.phi, y.phi, sizeof(double) * t.n, hipMemcpyHostToDevice, w.stream));
+            t.home = (gdptr)(base + TD_NCOEF * t.n);
+            EAFP_TRY(hipMemcpyAsync((void*)t.home, y.phi, sizeof(double) * t.n, hipMemcpyHostToDevice, w.stream));
+            soff += 9 * t.n;
         }
-        t.piv = w.work.p + 7 * off;
-        t.dp = t.piv + 6 * t.n;
-        off += t.n;
+        for (int P = 0; P < 3; ++P) {
+            TdPass& p = t.ps[P];
+            pass_shape(d, P, p.nu, p.nv, p.n);
+            p.NL = p.nu * p.nv;
+            p.NH = halo_lines(P, p.nu, p.nv);
+            p.NLX = p.NL + p.NH;
+            p.pk = (gdptr)dcur;
+            dcur += PK_N * t.ncells;
+            p.ln = (gdptr)dcur;
+            dcur += LN_N * (long)p.NL;
+            p.phi = (gdptr)dcur;
+            dcur += (long)p.n * p.NLX;
+            p.dstart = (cgiptr)(icur + ioff);
+            int acc = 0;
+            for (int wd = 0; wd <= p.nu + p.nv - 2; ++wd) {
+                int lo, hi;
+                diagonal_range(wd, p.nu, p.nv, lo, hi);
+                ihost[(size_t)ioff + wd] = acc;
+                acc += hi - lo + 1;
+            }
+            ioff += p.nu + p.nv - 1;
+        }
     }
+    EAFP_TRY(hipMemcpyAsync(w.iwork.p, ihost.data(), sizeof(int) * iwork, hipMemcpyHostToDevice, w.stream));
     EAFP_TRY(hipMemcpyAsync(w.table.p, tab.data(), sizeof(TdSys) * nsys, hipMemcpyHostToDevice, w.stream));
-    const dim3 pgrid((unsigned)((max_lines + TD_BLOCK - 1) / TD_BLOCK), (unsigned)nsys);
-    k_tdma_pivots<<<pgrid, TD_BLOCK, 0, w.stream>>>(w.table.p);
-    k_tdma_sweeps<<<nsys, TD_BLOCK, 0, w.stream>>>(w.table.p, n_sweep);
+    const unsigned ny = (unsigned)nsys;
+    const dim3 g_lines((unsigned)((max_lines + TD_BLOCK - 1) / TD_BLOCK), ny);
+    const dim3 g_cells((unsigned)((max_cells + TD_BLOCK - 1) / TD_BLOCK), ny);
+    const dim3 g_cols((unsigned)((max_cols + TD_BLOCK - 1) / TD_BLOCK), ny);
+    const TdSys* tp = w.table.p;
+    k_tdma_setup<<<g_lines, TD_BLOCK, 0, w.stream>>>(tp);
+    for (int sweep = 0; sweep < n_sweep; ++sweep) {
+        k_tdma_pass<0><<<nsys, TD_PASS_BLOCK, 0, w.stream>>>(tp);
+        k_tdma_move<0, 1><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
+        k_tdma_pass<1><<<nsys, TD_PASS_BLOCK, 0, w.stream>>>(tp);
+        k_tdma_move<1, 2><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
+        k_tdma_pass<2><<<nsys, TD_PASS_BLOCK, 0, w.stream>>>(tp);
+        if (sweep + 1 < n_sweep) k_tdma_move<2, 0><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
+        else k_tdma_move<2, 3><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
+    }
+    k_tdma_wall_halo<<<g_cols, TD_BLOCK, 0, w.stream>>>(tp);
     EAFP_TRY(hipGetLastError());
     if (!on_device)
         for (int q = 0; q < nsys; ++q)
-            EAFP_TRY(hipMemcpyAsync(sys[q].phi, tab[(size_t)q].phi, sizeof(double) * tab[(size_t)q].n,
+            EAFP_TRY(hipMemcpyAsync(sys[q].phi, (const void*)tab[(size_t)q].home, sizeof(double) * tab[(size_t)q].n,
                                     hipMemcpyDeviceToHost, w.stream));
     EAFP_TRY(hipStreamSynchronize(w.stream));
     return EAFP_OK;

@@ -28,3 +28,31 @@ def test_fortran_drop_in(config, shape, max_iter):
     # theta halos are the periodic copies (the final halo exchange, src/mod_solver_3d.f90:567)
     np.testing.assert_array_equal(phi_out[:, 1, :], phi_out[:, ps.nth + 1, :])
     np.testing.assert_array_equal(phi_out[:, ps.nth + 2, :], phi_out[:, 2, :])
+
+
+@pytest.mark.parametrize("shape,n_sweep", [((12, 24, 16), 3), ((60, 120, 84), 1)])
+def test_fortran_tdma_drop_in(shape, n_sweep):
+    """tdma_3d_mpi called from Fortran: phi bit-identical to the oracle's restatement."""
+    from fortran_io import run_tdma_driver
+    print(f"case fortran tdma shape={shape} n_sweep={n_sweep}", flush=True)
+    rng = np.random.default_rng(sum(shape))
+    pad = tuple(n + 4 for n in shape)
+    coef = {n: np.asfortranarray(rng.uniform(0.1, 1.0, pad)) for n in ("aW", "aE", "aS", "aN", "aB", "aT")}
+    coef["aP"] = np.asfortranarray(sum(coef.values()) * 1.1)
+    coef["Su"] = np.asfortranarray(rng.normal(size=pad))
+    phi = np.asfortranarray(rng.normal(size=pad))
+    status, phi_out = run_tdma_driver(coef, phi, shape, n_sweep)
+    assert status == 0
+    want = oracle.tdma_3d(coef, phi, shape, n_sweep)
+    assert np.count_nonzero(phi_out.view(np.uint64) != want.view(np.uint64)) == 0
+
+
+def test_fortran_tdma_refuses_parallel_mesh():
+    """A parallel mesh is refused and the program stops (the subroutine has no status)."""
+    from fortran_io import run_tdma_driver
+    shape = (4, 6, 5)
+    pad = tuple(n + 4 for n in shape)
+    coef = {n: np.ones(pad, order="F") for n in ("aW", "aE", "aS", "aN", "aB", "aT", "Su")}
+    coef["aP"] = np.full(pad, 7.0, order="F")
+    status, _ = run_tdma_driver(coef, np.zeros(pad, order="F"), shape, 1, is_parallel=True)
+    assert status is None

@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 NAMES = ("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su")
 
 
-def random_system(shape, seed, zero_pivots=False):
+def random_system(shape, seed, zero_pivots=False, huge=False):
     nr, nth, nz = shape
     rng = np.random.default_rng(seed)
     pad = (nr + 4, nth + 4, nz + 4)
@@ -36,6 +36,15 @@ def random_system(shape, seed, zero_pivots=False):
         for a in flat[:6]:
             a[sel] = 0.0
         flat[6][sel] = np.where(sel % 2 == 1, 0.0, -0.0)
+    if huge:
+        # pivots near the top of the double range: the division's operands get rescaled (the GPU's
+        # fallback path); tiny right-hand sides give quotients near the bottom
+        flat = coef["aP"].reshape(-1, order="F")
+        sel = rng.random(flat.size) < 0.02
+        flat[sel] = 4.0e307 * rng.uniform(1.0, 2.0, np.count_nonzero(sel))
+        su = coef["Su"].reshape(-1, order="F")
+        tiny = rng.random(su.size) < 0.02
+        su[tiny] = 1e-300 * rng.normal(size=np.count_nonzero(tiny))
     phi = np.asfortranarray(rng.normal(size=pad))
     return coef, phi
 
@@ -58,6 +67,17 @@ def test_tdma_matches_oracle_bitwise(shape, n_sweep, zero_pivots):
     phi = phi0.copy(order="F")
     eafp.tdma_3d_mpi(*[coef[n] for n in NAMES], phi, eafp.Mesh(*shape), n_sweep)
     assert_bitwise(phi, want, f"{shape} x{n_sweep}")
+
+
+@pytest.mark.parametrize("shape", [(12, 24, 16), (30, 20, 25)])
+def test_tdma_extreme_magnitudes(shape):
+    """Pivots near 1e308 and tiny sources: divisions whose operands the hardware rescales."""
+    print(f"case tdma extreme shape={shape}", flush=True)
+    coef, phi0 = random_system(shape, 77 + shape[0], huge=True)
+    want = oracle.tdma_3d(coef, phi0, shape, 2)
+    phi = phi0.copy(order="F")
+    eafp.tdma_3d_mpi(*[coef[n] for n in NAMES], phi, eafp.Mesh(*shape), 2)
+    assert_bitwise(phi, want, f"extreme {shape}")
 
 
 def test_tdma_production_system():
