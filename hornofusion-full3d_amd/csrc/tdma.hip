@@ -29,8 +29,9 @@
 //                  diagonal starts), a(e), m(e) and c'(e) of the diagonal's
 //                  lines into 160 KiB of LDS with many loads in flight, then
 //                  one thread per line runs the dependent Thomas recurrences
-//                  from LDS, a chunk of elements read ahead; a barrier
-//                  between diagonals.  A diagonal too long for the LDS is taken in
+//                  from LDS, a chunk of elements read ahead, leaving x in
+//                  LDS; then all threads copy the diagonal's x to phi,
+//                  coalesced; barriers between the three steps.  A diagonal too long for the LDS is taken in
 //                  groups of lines.
 //   k_tdma_move    moves phi between the passes' layouts (and into the
 //                  caller's array at the end), all CUs
@@ -103,6 +104,8 @@ __global__ __launch_bounds__(TD_PASS_BLOCK) void k_tdma_pass(const TdSys* __rest
             }
             __syncthreads();
             for (int q = threadIdx.x; q < gm; q += TD_PASS_BLOCK) solve_staged(p, l0 + g0 + q, sd, sa, sm, sc, q, gm);
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < cnt; idx += TD_PASS_BLOCK) store_staged(p, l0 + g0, gm, sd, idx);
             __syncthreads();
         }
     }

@@ -346,8 +346,8 @@ __host__ __device__ inline void stage_elem(const TdPass& p, int l, int e, const 
 
 // The Thomas recurrences of line l from its staged values (element e at
 // slot e*stride + q): d'(0) = d(0)/m(0), d'(e) = (d(e) - a(e) d'(e-1))/m(e)
-// into the d slots; x(n-1) = d'(n-1), x(e) = d'(e) - c'(e) x(e+1) into phi
-// row l.  Each chunk of TD_RING elements is read from the staging arrays,
+// into the d slots; x(n-1) = d'(n-1), x(e) = d'(e) - c'(e) x(e+1) into the
+// d slots too (the work-group then copies them to phi, coalesced).  Each chunk of TD_RING elements is read from the staging arrays,
 // and its reciprocals formed, before its recurrences run, so the reads and
 // the denominator-only work overlap the chain of divisions.
 __host__ __device__ inline void solve_staged(const TdPass& p, int l, double* sd, const double* sa,
@@ -391,7 +391,6 @@ __host__ __device__ inline void solve_staged(const TdPass& p, int l, double* sd,
         sd[at] = dp;
     }
     double x = dp;
-    p.phi[(long)(n - 1) * p.NLX + l] = x;
     for (int e1 = n - 2; e1 >= 0; e1 -= TD_RING) {
         double bd[TD_RING], bc[TD_RING];
 #pragma unroll
@@ -405,10 +404,17 @@ __host__ __device__ inline void solve_staged(const TdPass& p, int l, double* sd,
             const int e = e1 - r;
             if (e >= 0) {
                 x = bd[r] - bc[r] * x;
-                p.phi[(long)e * p.NLX + l] = x;
+                sd[e * stride + q] = x;
             }
         }
     }
+}
+
+// Element `at` = e*gm + q of a staged group of gm lines from l0 on: its x,
+// left in the d slots by solve_staged, copied to phi.
+__host__ __device__ inline void store_staged(const TdPass& p, int l0, int gm, const double* sd, int at) {
+    const int q = at % gm, e = at / gm;
+    p.phi[(long)e * p.NLX + l0 + q] = sd[at];
 }
 
 // Lines of diagonal w taken per group so that group lines x n elements fit

@@ -114,6 +114,7 @@ static void replay_pass(const TdSys& s, int order) {
                 solve_staged(p, l, sd, sa, sm, sc, q, gm);
                 solved[(size_t)l]++;
             }
+            for (int idx = cnt - 1; idx >= 0; --idx) store_staged(p, l0 + g0, gm, sd, idx);
         }
     }
     for (int l = 0; l < p.NL; ++l) CHECK(solved[(size_t)l] == 1, "pass %d line %d solved %d times", P, l, solved[(size_t)l]);
