@@ -284,6 +284,104 @@ def measure_assembly(eafp, shape=(60, 120, 84), reps=20):
     return rec
 
 
+def measure_step(eafp, shape=(60, 120, 84), reps=5):
+    """One whole p' step (eafp_pp_step: assembly, solve from pp = 0,
+    post-solve steps) on two-phase fields at 60x120x84, device-resident:
+    with the reference's defaults (cfg%max_inner_pres = 50 FP64 CG
+    iterations, tol 1e-8), and solved to 1e-8 by the FP64 CG and by the
+    mixed solver (BF16).  Beside it the same 50-iteration step from host
+    arrays, fused and as the three unfused calls, and the oracle's three
+    steps on one host core."""
+    import ctypes
+    from eafp import _lib, fields
+    F, cfg = fields.make_fields(*shape, seed=11, masks=True)
+    m = eafp.Mesh(*shape)
+    hip = ctypes.CDLL("libamdhip64.so")
+    bufs = []
+
+    def dev(a=None, nbytes=None):
+        ptr = ctypes.c_void_p()
+        nb = a.nbytes if a is not None else nbytes
+        _lib.check(hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(nb)), "hipMalloc")
+        if a is not None:
+            _lib.check(hip.hipMemcpy(ptr, ctypes.c_void_p(a.ctypes.data), ctypes.c_size_t(nb), 1), "hipMemcpy")
+        bufs.append(ptr)
+        return ptr.value
+
+    vals = {}
+    for n in _lib.PP_FIELDS:
+        a = F.get(n)
+        vals[n] = None if a is None else dev(np.asfortranarray(a, dtype=np.int32 if n in _lib.PP_INT else np.float64))
+    npad = (shape[0] + 4) * (shape[1] + 4) * (shape[2] + 4)
+    f = _lib.PPFields(**vals)
+    c = _lib.PPConfig(cfg["dt"], cfg["rho_gas"], cfg["T_ambient"], cfg["beta_expansion"],
+                      cfg["gas_in_poisson"], cfg["solve_multiphase"], cfg["gas_compressibility"], 0)
+    # io arrays apart from the fields: every call assembles from the same inputs
+    upd = {n: dev(nbytes=(4 if n.startswith("lk") else 8) * npad) for n in _lib.PP_UPDATE[:6]}
+    upd.update({n: dev(np.asfortranarray(F[n], dtype=np.float64)) for n in _lib.PP_UPDATE[6:]})
+    io = _lib.PPUpdate(**upd)
+    pp_d = dev(nbytes=8 * npad)
+    L = _lib.load()
+    d = eafp.solver.desc_of(m)
+    mixed = _lib.MixedOpts(1e-8, 32, 2000, _lib.COEF_BF16, 0)
+    rec = {"workload": "one p' step on two-phase 60x120x84 fields: assembly, solve from pp = 0, fluxes, velocities, p",
+           "device_resident": {}}
+    for label, o in (("fp64_50_iterations", _lib.PPStepOpts(_lib.STEP_FP64, 50, 1e-8, 0.3, mixed)),
+                     ("fp64_to_1e-8", _lib.PPStepOpts(_lib.STEP_FP64, 100000, 1e-8, 0.3, mixed)),
+                     ("mixed_bf16_to_1e-8", _lib.PPStepOpts(_lib.STEP_MIXED, 0, 0.0, 0.3, mixed))):
+        res = ctypes.c_double()
+        st = _lib.Stats()
+        call = lambda: _lib.check(L.eafp_pp_step(ctypes.byref(d), ctypes.byref(f), ctypes.byref(c), ctypes.byref(o),
+                                                 pp_d, ctypes.byref(io), ctypes.byref(res), ctypes.byref(st), 1),
+                                  "eafp_pp_step")
+        call()
+        t = time.perf_counter()
+        for _ in range(reps):
+            call()
+        dt = (time.perf_counter() - t) / reps
+        rec["device_resident"][label] = {"ms_per_step": round(dt * 1e3, 3), "steps_per_s": round(1.0 / dt, 2),
+                                         "n_iter": st.n_iter, "n_refine": st.n_refine,
+                                         "solver_residual": st.residual, "true_rel_l2": st.true_rel_l2}
+        log(f"pp_step {label}: {dt * 1e3:.2f} ms, n_iter {st.n_iter}, true rel {st.true_rel_l2:.2e}")
+    for ptr in bufs:
+        hip.hipFree(ptr)
+    # host arrays, 50 iterations: fused, then the three unfused calls
+    eafp.pp_step(F, cfg, m, 0.3, max_iter=50)
+    t = time.perf_counter()
+    for _ in range(reps):
+        eafp.pp_step(F, cfg, m, 0.3, max_iter=50)
+    rec["host_arrays_fused_ms_per_step"] = round((time.perf_counter() - t) / reps * 1e3, 3)
+    names = ("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su")
+
+    def unfused():
+        out, _, _, ex = eafp.pp_assemble(F, cfg, m, exports=True)
+        pp = np.zeros_like(out["aP"], order="F")
+        eafp.cg_3d_mpi(*[out[n] for n in names], pp, m, 50, 1e-8)
+        eafp.pp_correct(F, cfg, m, 0.3, pp, ex)
+    unfused()
+    t = time.perf_counter()
+    for _ in range(reps):
+        unfused()
+    rec["host_arrays_three_calls_ms_per_step"] = round((time.perf_counter() - t) / reps * 1e3, 3)
+    sys.path.insert(0, ROOT)
+    import oracle
+    oracle.lib()
+    t = time.perf_counter()
+    ref, _, _, ex = oracle.pp_assemble(F, cfg, shape, exports=True)
+
+    class S:
+        pass
+    ps = S()
+    ps.shape = shape
+    ps.padded = lambda n: ref[n]
+    pp_ref, _, _, _ = oracle.cg(ps, max_iter=50, tol=1e-8)
+    oracle.pp_correct(F, cfg, shape, 0.3, pp_ref, ex)
+    cpu = time.perf_counter() - t
+    rec["cpu_oracle_ms_per_step_50_iterations_1_core"] = round(cpu * 1e3, 1)
+    rec["speedup_device_resident_vs_cpu"] = round(cpu * 1e3 / rec["device_resident"]["fp64_50_iterations"]["ms_per_step"], 1)
+    return rec
+
+
 def measure_tdma(eafp, batches=(1, 8, 64), n_sweep=3, reps=5):
     """The line TDMA (eafp_tdma_solve_batch, device-resident) on the 60x120x84
     production p' system: one system, the 8 equations of a two-phase time
@@ -376,6 +474,8 @@ def main():
                          "FP64 drop-in's kernels (k64_spmv, k64_update) instead of the batch")
     ap.add_argument("--assembly-only", action="store_true",
                     help="only the p' assembly and post-solve leg (60x120x84 two-phase fields), for profiling")
+    ap.add_argument("--step-only", action="store_true",
+                    help="only the fused p' step leg (assembly + solve + post-solve, 60x120x84), for profiling")
     ap.add_argument("--tdma-only", action="store_true",
                     help="only the line-TDMA leg (60x120x84, 1/8/64 systems), for profiling")
     ap.add_argument("--derive-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -405,6 +505,10 @@ def main():
     from eafp import systems
     from eafp._lib import COEF_BF16, COEF_FP16, COEF_FP32
     fmt = {"fp32": COEF_FP32, "bf16": COEF_BF16, "fp16": COEF_FP16}[args.coef]
+
+    if args.step_only:
+        print(json.dumps({"step_only": True, "pp_step": measure_step(eafp)}), flush=True)
+        return 0
 
     if args.tdma_only:
         print(json.dumps({"tdma_only": True, "line_tdma": measure_tdma(eafp)}), flush=True)
@@ -610,6 +714,8 @@ def main():
     if not args.no_extras:
         out["pp_assembly"] = measure_assembly(eafp)
         log("p' assembly done")
+        out["pp_step"] = measure_step(eafp)
+        log("p' step done")
         out["line_tdma"] = measure_tdma(eafp)
         log("line TDMA done")
         # the other workloads of the metric's table, mixed against FP64 on the same GPU

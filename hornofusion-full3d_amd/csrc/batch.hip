@@ -141,7 +141,7 @@ int Batch::init(int n, const eafp_desc* descs) {
 // (src/mod_solver_3d.f90:510-513); later CG vectors have zero r/z halos.
 // ---------------------------------------------------------------------------
 struct PackArgs {
-    const double* src[8];   // padded aW..aP, Su (device staging)
+    const double* src[8];   // padded aW..aP, Su (device: staging or the caller's arrays)
     const double* phi;      // padded phi0 (device)
     double* dst[7];         // compact coefficient arenas (at system offset)
     double* b;
@@ -168,22 +168,162 @@ __global__ void k_pack(PackArgs a, SysGeom g) {
     a.x0[c] = ph[o];
 }
 
-int Batch::upload(int s, const double* const host[8], const double* phi0, int on_device) {
+// ---------------------------------------------------------------------------
+// Device pre-pass of the singular-component analysis.  A component of the
+// link graph is singular when every one of its rows is a Neumann row (the
+// rule of analyze_components below); a linked row that is not a Neumann row
+// anchors its component.  k_comp_init stores per compact cell the six link
+// bits (a link exists when either side's coefficient is nonzero, theta
+// periodic, no link across the r/z walls) and the anchor bit; k_comp_sweep
+// spreads the anchor bit along links until a sweep changes nothing.  Then
+// every linked cell is anchored exactly when the system has no singular
+// component, and only otherwise does the host analysis run.
+// ---------------------------------------------------------------------------
+constexpr unsigned char LB_W = 1, LB_E = 2, LB_S = 4, LB_N = 8, LB_B = 16, LB_T = 32, LB_LINKS = 63, LB_ANCHOR = 64;
+
+struct CompArgs {
+    const double* A[7];   // padded aW..aT, aP (device)
+    unsigned char* bits;  // per compact cell of the system
+    int* flag;
+};
+
+__global__ void k_comp_init(CompArgs a, SysGeom g) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.n) return;
+    const CellIdx id = decompose(c, g);
+    const int i = id.i, j = id.j, k = id.k;
+    const long sy = g.nr + 4, sz = (long)(g.nr + 4) * (g.nth + 4);
+    const long o = (i + 2) + sy * (j + 2) + sz * (k + 2);
+    const int jn = j + 1 == g.nth ? 0 : j + 1, jp = j == 0 ? g.nth - 1 : j - 1;
+    const long on = o + sy * (jn - j), op = o + sy * (jp - j);
+    const double *AW = a.A[C_AW], *AE = a.A[C_AE], *AS = a.A[C_AS], *AN = a.A[C_AN];
+    const double *AB = a.A[C_AB], *AT = a.A[C_AT];
+    unsigned char b = 0;
+    if (i > 0 && (AE[o - 1] != 0.0 || AW[o] != 0.0)) b |= LB_W;
+    if (i + 1 < g.nr && (AE[o] != 0.0 || AW[o + 1] != 0.0)) b |= LB_E;
+    if (AN[op] != 0.0 || AS[o] != 0.0) b |= LB_S;
+    if (AN[o] != 0.0 || AS[on] != 0.0) b |= LB_N;
+    if (k > 0 && (AT[o - sz] != 0.0 || AB[o] != 0.0)) b |= LB_B;
+    if (k + 1 < g.nz && (AT[o] != 0.0 || AB[o + sz] != 0.0)) b |= LB_T;
+    // the Neumann test of analyze_components, in its order of additions
+    double links = 0.0;
+    if (i > 0) links += AW[o];
+    if (i + 1 < g.nr) links += AE[o];
+    links += AS[o] + AN[o];
+    if (k > 0) links += AB[o];
+    if (k + 1 < g.nz) links += AT[o];
+    const double ap = a.A[C_AP][o];
+    const bool wall = (i == 0 && AW[o] != 0.0) || (i + 1 == g.nr && AE[o] != 0.0) || (k == 0 && AB[o] != 0.0) ||
+                      (k + 1 == g.nz && AT[o] != 0.0);
+    const bool neumann = !wall && fabs(ap - links) <= 1e-10 * fabs(ap);
+    if (b != 0 && !neumann) b |= LB_ANCHOR;
+    a.bits[c] = b;
+}
+
+// One anchor-spreading sweep (in place: a sweep may also use what it set).
+// flag: NULL, or set to 1 when the sweep anchors a cell.
+__global__ void k_comp_sweep(unsigned char* bits, SysGeom g, int* flag) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.n) return;
+    const unsigned char b = bits[c];
+    if ((b & LB_LINKS) == 0 || (b & LB_ANCHOR)) return;
+    const CellIdx id = decompose(c, g);
+    const int plane = g.nth * g.nz;
+    const int cs = id.j == 0 ? c + (g.nth - 1) * g.nz : c - g.nz;
+    const int cn = id.j + 1 == g.nth ? c - (g.nth - 1) * g.nz : c + g.nz;
+    bool hit = false;
+    if (b & LB_W) hit |= (bits[c - plane] & LB_ANCHOR) != 0;
+    if (b & LB_E) hit |= (bits[c + plane] & LB_ANCHOR) != 0;
+    if (b & LB_S) hit |= (bits[cs] & LB_ANCHOR) != 0;
+    if (b & LB_N) hit |= (bits[cn] & LB_ANCHOR) != 0;
+    if (b & LB_B) hit |= (bits[c - 1] & LB_ANCHOR) != 0;
+    if (b & LB_T) hit |= (bits[c + 1] & LB_ANCHOR) != 0;
+    if (hit) {
+        bits[c] = b | LB_ANCHOR;
+        if (flag) *flag = 1;
+    }
+}
+
+// flag = 1 when a linked cell is still unanchored (a singular component).
+__global__ void k_comp_unanchored(const unsigned char* bits, SysGeom g, int* flag) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.n) return;
+    const unsigned char b = bits[c];
+    if ((b & LB_LINKS) != 0 && !(b & LB_ANCHOR)) *flag = 1;
+}
+
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
+__global__ void k_fill_u8(unsigned char* p, int n, unsigned char v) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) p[c] = v;
+}
+
+// Returns EAFP_OK with *singular = 0 when the system has no singular
+// component (its slots are then set to NO_COMP), *singular = 1 otherwise.
+int Batch::device_components(int s, const double* const A[7], int* singular) {
+    const SysGeom& g = geom[s];
+    EAFP_TRY_RC(comp_bits.alloc(std::max<size_t>(comp_bits.n, (size_t)g.n)));
+    if (!comp_flag.p) EAFP_TRY_RC(comp_flag.alloc(1));
+    CompArgs a;
+    for (int q = 0; q < 7; ++q) a.A[q] = A[q];
+    a.bits = comp_bits.p;
+    a.flag = comp_flag.p;
+    const int blocks = (g.n + 255) / 256;
+    k_comp_init<<<blocks, 256, 0, stream>>>(a, g);
+    EAFP_TRY(hipGetLastError());
+    // Every sweep that changes something anchors at least one cell, so
+    // g.n / SWEEPS + 1 rounds always reach the fixed point.
+    constexpr int SWEEPS = 16;
+    int changed = 1;
+    for (long round = 0; changed && round <= g.n / SWEEPS + 1; ++round) {
+        for (int q = 0; q < SWEEPS - 1; ++q) k_comp_sweep<<<blocks, 256, 0, stream>>>(comp_bits.p, g, nullptr);
+        k_set_int<<<1, 1, 0, stream>>>(comp_flag.p, 0);
+        k_comp_sweep<<<blocks, 256, 0, stream>>>(comp_bits.p, g, comp_flag.p);
+        EAFP_TRY(hipGetLastError());
+        EAFP_TRY(hipMemcpyAsync(&changed, comp_flag.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+        EAFP_TRY(hipStreamSynchronize(stream));
+    }
+    if (changed) return EAFP_ERR_STATE;   // unreachable: the bound above is exact
+    int un = 0;
+    k_set_int<<<1, 1, 0, stream>>>(comp_flag.p, 0);
+    k_comp_unanchored<<<blocks, 256, 0, stream>>>(comp_bits.p, g, comp_flag.p);
+    EAFP_TRY(hipGetLastError());
+    EAFP_TRY(hipMemcpyAsync(&un, comp_flag.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    EAFP_TRY(hipStreamSynchronize(stream));
+    *singular = un;
+    if (!un) {
+        k_fill_u8<<<blocks, 256, 0, stream>>>(comp.p + g.off, g.n, NO_COMP);
+        EAFP_TRY(hipGetLastError());
+        comp_info[s] = CompInfo{};
+    }
+    return EAFP_OK;
+}
+
+int Batch::upload(int s, const double* const host[8], const double* phi0, int on_device, bool components) {
     if (s < 0 || s >= nsys) return EAFP_ERR_STATE;
     for (int q = 0; q < 8; ++q)
         if (!host[q]) return EAFP_ERR_ARG;
     const long np = padded_len(s);
-    EAFP_TRY_RC(staging.alloc(std::max<size_t>(staging.n, (size_t)np * 8)));
     const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    for (int q = 0; q < 8; ++q)
-        EAFP_TRY(hipMemcpyAsync(staging.p + (long)q * np, host[q], sizeof(double) * np, kind, stream));
+    // device arrays are packed where they lie; host arrays go through staging
+    const double* src[8];
+    if (on_device) {
+        for (int q = 0; q < 8; ++q) src[q] = host[q];
+    } else {
+        EAFP_TRY_RC(staging.alloc(std::max<size_t>(staging.n, (size_t)np * 8)));
+        for (int q = 0; q < 8; ++q) {
+            EAFP_TRY(hipMemcpyAsync(staging.p + (long)q * np, host[q], sizeof(double) * np, kind, stream));
+            src[q] = staging.p + (long)q * np;
+        }
+    }
     if (phi0)
         EAFP_TRY(hipMemcpyAsync(phi_pad[s]->p, phi0, sizeof(double) * np, kind, stream));
     else
         EAFP_TRY(hipMemsetAsync(phi_pad[s]->p, 0, sizeof(double) * np, stream));
     const SysGeom& g = geom[s];
     PackArgs a;
-    for (int q = 0; q < 8; ++q) a.src[q] = staging.p + (long)q * np;
+    for (int q = 0; q < 8; ++q) a.src[q] = src[q];
     a.phi = phi_pad[s]->p;
     for (int q = 0; q < 7; ++q) a.dst[q] = coef[q].p + g.off;
     a.b = b.p + g.off;
@@ -192,12 +332,26 @@ int Batch::upload(int s, const double* const host[8], const double* phi0, int on
     k_pack<<<(g.n + 255) / 256, 256, 0, stream>>>(a, g);
     EAFP_TRY(hipGetLastError());
 
+    if (!components) {
+        EAFP_TRY(hipStreamSynchronize(stream));
+        uploaded[s] = 1;
+        return EAFP_OK;
+    }
+    int singular = 1;
+    EAFP_TRY_RC(device_components(s, src, &singular));
+    if (!singular) {
+        EAFP_TRY(hipStreamSynchronize(stream));
+        uploaded[s] = 2;
+        return EAFP_OK;
+    }
     // singular components (host): needs the coefficients on the host
     std::vector<double> hbuf;
     const double* hc[7];
     if (on_device) {
         hbuf.resize((size_t)np * 7);
-        EAFP_TRY(hipMemcpyAsync(hbuf.data(), staging.p, sizeof(double) * np * 7, hipMemcpyDeviceToHost, stream));
+        for (int q = 0; q < 7; ++q)
+            EAFP_TRY(hipMemcpyAsync(hbuf.data() + (size_t)q * np, src[q], sizeof(double) * np, hipMemcpyDeviceToHost,
+                                    stream));
         EAFP_TRY(hipStreamSynchronize(stream));
         for (int q = 0; q < 7; ++q) hc[q] = hbuf.data() + (size_t)q * np;
     } else {
@@ -207,7 +361,7 @@ int Batch::upload(int s, const double* const host[8], const double* phi0, int on
     EAFP_TRY_RC(analyze_components(g, hc, slot.data(), comp_info[s]));
     EAFP_TRY(hipMemcpyAsync(comp.p + g.off, slot.data(), g.n, hipMemcpyHostToDevice, stream));
     EAFP_TRY(hipStreamSynchronize(stream));
-    uploaded[s] = 1;
+    uploaded[s] = 2;
     return EAFP_OK;
 }
 

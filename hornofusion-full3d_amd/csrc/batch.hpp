@@ -94,8 +94,10 @@ struct Batch {
     DevBuf<unsigned char> comp;          // projected-component slot per cell (NO_COMP: none)
     std::vector<CompInfo> comp_info;
     std::vector<DevBuf<double>*> phi_pad;  // padded phi per system (device)
-    std::vector<char> uploaded;
+    std::vector<char> uploaded;          // 0: no; 1: yes; 2: yes, singular components analysed
     DevBuf<double> staging;              // padded staging for uploads
+    DevBuf<unsigned char> comp_bits;     // link and anchor bits of the component pre-pass
+    DevBuf<int> comp_flag;
     hipStream_t stream = nullptr;
     int grid = 0;                        // blocks per tile-walking launch
     Cg64Work* cg64 = nullptr;
@@ -105,8 +107,12 @@ struct Batch {
     int init(int n, const eafp_desc* descs);
     int set_tiling(int cells_per_thread, int blocks);
     int set_columns(int r_splits);
-    int upload(int s, const double* const host[8], const double* phi0, int arrays_on_device);
+    // components = false skips the singular-component analysis, which only
+    // the mixed solver needs (it reads the coefficients on the host).
+    int upload(int s, const double* const host[8], const double* phi0, int arrays_on_device,
+               bool components = true);
     int download(int s, double* phi_host, int to_device);
+    int device_components(int s, const double* const A[7], int* singular);
     long padded_len(int s) const {
         const SysGeom& g = geom[s];
         return (long)(g.nr + 4) * (g.nth + 4) * (g.nz + 4);
@@ -119,6 +125,9 @@ int pp_assemble(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_confi
 int pp_correct(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config* cfg, double alpha_p,
                const double* pp, const eafp_pp_exports* ex, const eafp_pp_update* io, int on_device);
 void pp_release();
+int pp_step(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config* cfg, const eafp_pp_step_opts* o,
+            double* pp, const eafp_pp_update* io, double* residual, eafp_stats* stats, int on_device);
+void step_release();
 int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* sys, int n_sweep, int on_device);
 void tdma_release();
 int choose_grid(int ntiles);

@@ -73,6 +73,7 @@ int eafp_release(void) {
         std::lock_guard<std::mutex> lk(g_mu);
         g_cache.reset();
     }
+    eafp::step_release();
     eafp::pp_release();
     eafp::tdma_release();
     return EAFP_OK;
@@ -101,6 +102,12 @@ int eafp_pp_correct(const eafp_desc* desc, const eafp_pp_fields* fields, const e
                     const double* pp, const eafp_pp_exports* exports, const eafp_pp_update* io,
                     int arrays_on_device) {
     return eafp::pp_correct(desc, fields, cfg, alpha_p, pp, exports, io, arrays_on_device);
+}
+
+int eafp_pp_step(const eafp_desc* desc, const eafp_pp_fields* fields, const eafp_pp_config* cfg,
+                 const eafp_pp_step_opts* opts, double* pp, const eafp_pp_update* io, double* residual,
+                 eafp_stats* stats, int arrays_on_device) {
+    return eafp::pp_step(desc, fields, cfg, opts, pp, io, residual, stats, arrays_on_device);
 }
 
 int eafp_batch_create(int nsys, const eafp_desc* descs, eafp_batch** out) {

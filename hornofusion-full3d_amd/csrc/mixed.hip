@@ -1435,7 +1435,7 @@ static int mixed_start(Batch& B, const eafp_mixed_opts& o, int fmt) {
 
 static int check_mixed(Batch& B, const eafp_mixed_opts& o) {
     for (int s = 0; s < B.nsys; ++s) {
-        if (!B.uploaded[s]) return EAFP_ERR_STATE;
+        if (B.uploaded[s] != 2) return EAFP_ERR_STATE;
         if (B.comp_info[s].overflow) return EAFP_ERR_COMPONENTS;
     }
     if (!(o.tol > 0.0) || o.max_steps < 1 || o.interval < 0) return EAFP_ERR_ARG;

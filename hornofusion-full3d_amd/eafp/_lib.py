@@ -25,7 +25,8 @@ COEF_FP32, COEF_BF16, COEF_FP16 = 0, 1, 2
 
 # Every symbol declared in include/eafp.h.
 EXPORTS = (
-    "eafp_cg_solve", "eafp_release", "eafp_pp_assemble", "eafp_pp_correct", "eafp_tdma_solve", "eafp_tdma_solve_batch", "eafp_batch_create", "eafp_batch_upload",
+    "eafp_cg_solve", "eafp_release", "eafp_pp_assemble", "eafp_pp_correct", "eafp_pp_step", "eafp_tdma_solve",
+    "eafp_tdma_solve_batch", "eafp_batch_create", "eafp_batch_upload",
     "eafp_batch_solve_fp64", "eafp_batch_solve_mixed", "eafp_batch_download",
     "eafp_batch_info", "eafp_batch_set_tiling", "eafp_batch_set_columns", "eafp_batch_get_columns",
     "eafp_batch_time_kernels", "eafp_batch_autotune", "eafp_batch_destroy", "eafp_version",
@@ -103,6 +104,15 @@ class MixedOpts(ctypes.Structure):
                 ("coef_format", ctypes.c_int), ("restart", ctypes.c_int)]
 
 
+STEP_FP64, STEP_MIXED = 0, 1
+
+
+class PPStepOpts(ctypes.Structure):
+    """eafp_pp_step_opts: the solver of the fused p' step and its options."""
+    _fields_ = [("solver", ctypes.c_int), ("max_iter", ctypes.c_int), ("tol", ctypes.c_double),
+                ("alpha_p", ctypes.c_double), ("mixed", MixedOpts)]
+
+
 _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -142,6 +152,9 @@ def load():
     L.eafp_pp_correct.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(PPFields), ctypes.POINTER(PPConfig), d, vp,
                                   ctypes.POINTER(PPExports), ctypes.POINTER(PPUpdate), i]
     L.eafp_pp_correct.restype = i
+    L.eafp_pp_step.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(PPFields), ctypes.POINTER(PPConfig),
+                               ctypes.POINTER(PPStepOpts), vp, ctypes.POINTER(PPUpdate), _dp, ctypes.POINTER(Stats), i]
+    L.eafp_pp_step.restype = i
     L.eafp_tdma_solve.argtypes = [ctypes.POINTER(Desc)] + [vp] * 9 + [i, i]
     L.eafp_tdma_solve.restype = i
     L.eafp_tdma_solve_batch.argtypes = [i, ctypes.POINTER(Desc), ctypes.POINTER(TdmaSystem), i, i]

@@ -147,6 +147,46 @@ def pp_correct(F, cfg, m, alpha_p, pp, ex):
     return out
 
 
+def pp_step(F, cfg, m, alpha_p, solver="fp64", max_iter=1000, tol=1e-8, mixed=None, in_place=False):
+    """One whole p' step on the GPU (eafp_pp_step): assembly, the solve from
+    pp = 0 and the post-solve steps, with the p' system kept in HBM.
+    ``solver`` "fp64" is cg_3d_mpi's Jacobi-PCG (``max_iter``, ``tol``);
+    "mixed" the mixed-precision solver with ``mixed`` (a dict of MixedOpts
+    fields; tol 1e-8 and BF16 coefficients by default).  Returns ``(out,
+    pp, residual, stats)``: ``out`` as ``pp_correct`` returns it, ``pp`` the
+    solved correction, ``residual`` the continuity residual (NaN when the
+    solve broke down) and ``stats`` the solve's outcome.  ``in_place``: the
+    velocities and p of ``F`` (Fortran-ordered float64) are updated where
+    they lie and returned in ``out``, as the reference updates liq%ur and
+    sh%p, which are also the assembly's inputs."""
+    L = _lib.load()
+    f, c, keep = _pp_structs(F, cfg)
+    out = {n: _padded_zeros(m, np.int32 if n.startswith("lk") else np.float64) for n in _lib.PP_UPDATE[:6]}
+    for n in _lib.PP_UPDATE[6:]:
+        if in_place:
+            a = F[n]
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.f_contiguous):
+                raise ValueError(f"in_place needs F[{n!r}] as a Fortran-ordered float64 array")
+            out[n] = a
+        else:
+            out[n] = np.array(F[n], dtype=np.float64, order="F")
+    io = _lib.PPUpdate(**{n: out[n].ctypes.data for n in _lib.PP_UPDATE})
+    mo = dict(tol=1e-8, interval=32, max_steps=2000, coef_format=_lib.COEF_BF16, restart=0)
+    mo.update(mixed or {})
+    if solver not in ("fp64", "mixed"):
+        raise ValueError(f"solver must be 'fp64' or 'mixed', not {solver!r}")
+    o = _lib.PPStepOpts(_lib.STEP_FP64 if solver == "fp64" else _lib.STEP_MIXED, int(max_iter), float(tol),
+                        float(alpha_p), _lib.MixedOpts(**mo))
+    pp = _padded_zeros(m)
+    res = ctypes.c_double()
+    st = _lib.Stats()
+    d = desc_of(m)
+    check(L.eafp_pp_step(ctypes.byref(d), ctypes.byref(f), ctypes.byref(c), ctypes.byref(o), pp.ctypes.data,
+                         ctypes.byref(io), ctypes.byref(res), ctypes.byref(st), 0), "eafp_pp_step")
+    del keep
+    return out, pp, res.value, st.as_dict()
+
+
 def tdma_3d_mpi(aW, aE, aS, aN, aB, aT, aP, Su, phi, m, n_sweep):
     """Mirror of ``tdma_3d_mpi(aW, aE, aS, aN, aB, aT, aP, Su, phi, m, n_sweep)``
     (src/mod_solver_3d.f90:252-366) on the GPU (eafp_tdma_solve): ``n_sweep``

@@ -97,7 +97,7 @@ int eafp_cg_solve(const eafp_desc* desc,
                   double* phi, int max_iter, double tol, double* residual, int* n_iter);
 
 /* Releases the device buffers cached by eafp_cg_solve, eafp_pp_assemble,
- * eafp_pp_correct and eafp_tdma_solve(_batch).  No reference counterpart
+ * eafp_pp_correct, eafp_pp_step and eafp_tdma_solve(_batch).  No reference counterpart
  * (the reference's workspace lives until exit). */
 int eafp_release(void);
 
@@ -199,6 +199,43 @@ typedef struct {
 int eafp_pp_correct(const eafp_desc* desc, const eafp_pp_fields* fields, const eafp_pp_config* cfg,
                     double alpha_p, const double* pp, const eafp_pp_exports* exports, const eafp_pp_update* io,
                     int arrays_on_device);
+
+/* ---------------------------------------------------------------------------
+ * One whole p' step of solve_pressure_correction (src/mod_pressure_3d.f90:
+ * 127-397) in one call, every intermediate kept in HBM: sh%pp = 0 (:174),
+ * the assembly (as eafp_pp_assemble, with the face exports), the solve of
+ * the assembled system from pp = 0 (:316-318), then the post-solve steps
+ * (as eafp_pp_correct) with alpha_p.  As in the reference a NaN solver
+ * residual replaces the continuity residual (:322) and the corrections are
+ * applied regardless.
+ *   solver EAFP_STEP_FP64: cg_3d_mpi's Jacobi-PCG (max_iter, tol, the
+ *     recurrence residual); every output equals what eafp_pp_assemble,
+ *     eafp_cg_solve (phi = 0) and eafp_pp_correct give called in turn.
+ *   solver EAFP_STEP_MIXED: the mixed-precision solver with `mixed`
+ *     (tol on the true relative residual); max_iter and tol unused.
+ * pp receives p' (padded; theta halos filled, r/z halos zero); io is updated
+ * as by eafp_pp_correct.  residual (the continuity residual of the step)
+ * and stats (the solve's outcome) may be NULL.  The exports and the p'
+ * system stay inside the library.  arrays_on_device: every array of
+ * fields, io and pp is device memory (1) or host memory (0; uploaded once,
+ * the results downloaded once).
+ * No single reference counterpart: it fuses the three steps of one
+ * subroutine, for callers whose fields already live on the GPU.
+ * ------------------------------------------------------------------------- */
+#define EAFP_STEP_FP64 0
+#define EAFP_STEP_MIXED 1
+
+typedef struct eafp_pp_step_opts {
+    int solver;               /* EAFP_STEP_* */
+    int max_iter;             /* FP64: cfg%max_inner_pres */
+    double tol;               /* FP64: SOR_TOL_PRESSURE */
+    double alpha_p;           /* cfg%alpha_p */
+    eafp_mixed_opts mixed;    /* EAFP_STEP_MIXED */
+} eafp_pp_step_opts;
+
+int eafp_pp_step(const eafp_desc* desc, const eafp_pp_fields* fields, const eafp_pp_config* cfg,
+                 const eafp_pp_step_opts* opts, double* pp, const eafp_pp_update* io, double* residual,
+                 eafp_stats* stats, int arrays_on_device);
 
 /* ---------------------------------------------------------------------------
  * Device-resident systems.  A batch holds nsys systems of any shapes; nsys

@@ -176,3 +176,58 @@ def test_tdma_rejects_bad_calls():
     tab = (_lib.TdmaSystem * 2)(good, _lib.TdmaSystem(**vals))
     assert L.eafp_tdma_solve_batch(2, d, tab, 1, 0) == _lib.EAFP_ERR_ARG
     assert L.eafp_tdma_solve_batch(0, d, tab, 1, 0) == _lib.EAFP_ERR_ARG
+
+
+def test_pp_step_struct_layout():
+    # eafp_pp_step_opts: 2 ints, 2 doubles, then an eafp_mixed_opts (24 bytes) at offset 24
+    assert ctypes.sizeof(_lib.PPStepOpts) == 48
+    assert _lib.PPStepOpts.mixed.offset == 24
+    assert _lib.PPStepOpts.alpha_p.offset == 16
+
+
+def test_pp_step_rejects_bad_calls():
+    """Argument checks of the fused p' step run before any device call."""
+    from eafp import fields
+    shape = (4, 6, 5)
+    F, cfg = fields.make_fields(*shape)
+    with pytest.raises(_lib.EafpError) as e:
+        eafp.pp_step(F, cfg, eafp.Mesh(*shape, is_parallel=True), 0.3)
+    assert e.value.rc == _lib.EAFP_ERR_UNSUPPORTED
+    G = dict(F)
+    del G["T_old_g"]
+    with pytest.raises(_lib.EafpError) as e:
+        eafp.pp_step(G, cfg, eafp.Mesh(*shape), 0.3)
+    assert e.value.rc == _lib.EAFP_ERR_ARG
+    with pytest.raises(_lib.EafpError) as e:
+        eafp.pp_step(F, dict(cfg, dt=-1.0), eafp.Mesh(*shape), 0.3)
+    assert e.value.rc == _lib.EAFP_ERR_ARG
+    for bad in (dict(tol=0.0), dict(max_steps=0), dict(interval=-2), dict(coef_format=7)):
+        with pytest.raises(_lib.EafpError) as e:
+            eafp.pp_step(F, cfg, eafp.Mesh(*shape), 0.3, solver="mixed", mixed=bad)
+        assert e.value.rc == _lib.EAFP_ERR_ARG, bad
+    with pytest.raises(ValueError):
+        eafp.pp_step(F, cfg, eafp.Mesh(*shape), 0.3, solver="sor")
+    # an unknown solver code, a NULL pp and a NULL io array, through the C-ABI directly
+    L = _lib.load()
+    f, c, keep = eafp.solver._pp_structs(F, cfg)
+    pad = tuple(n + 4 for n in shape)
+    io_arrays = {n: np.zeros(pad, dtype=np.int32 if n.startswith("lk") else np.float64, order="F")
+                 for n in _lib.PP_UPDATE}
+    io = _lib.PPUpdate(**{n: a.ctypes.data for n, a in io_arrays.items()})
+    pp = np.zeros(pad, order="F")
+    d = eafp.solver.desc_of(eafp.Mesh(*shape))
+    mixed = _lib.MixedOpts(1e-8, 32, 100, _lib.COEF_BF16, 0)
+
+    def call(o, pp_ptr, io_s):
+        return L.eafp_pp_step(ctypes.byref(d), ctypes.byref(f), ctypes.byref(c), ctypes.byref(o), pp_ptr,
+                              ctypes.byref(io_s), None, None, 0)
+
+    assert call(_lib.PPStepOpts(2, 50, 1e-8, 0.3, mixed), pp.ctypes.data, io) == _lib.EAFP_ERR_ARG
+    assert call(_lib.PPStepOpts(0, 50, 1e-8, 0.3, mixed), None, io) == _lib.EAFP_ERR_ARG
+    vals = {n: a.ctypes.data for n, a in io_arrays.items()}
+    vals["uth_g"] = None
+    assert call(_lib.PPStepOpts(0, 50, 1e-8, 0.3, mixed), pp.ctypes.data, _lib.PPUpdate(**vals)) == _lib.EAFP_ERR_ARG
+    # valid arguments reach the device check: no device here, or a GPU run
+    rc = call(_lib.PPStepOpts(0, 50, 1e-8, 0.3, mixed), pp.ctypes.data, io)
+    assert rc in (_lib.EAFP_OK, _lib.EAFP_ERR_NO_DEVICE)
+    del keep
