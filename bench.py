@@ -35,7 +35,7 @@ METRIC = "p' CG solves/s @60×120×84 to 1e-8 rel res (mixed vs FP64); HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per cell of one launch (see the kernel headers):
 BYTES_MSPMV = {"fp32": 32, "bf16": 24, "fp16": 28}   # z, p_old, 3 weights and e (+FP32 scale); p, q
-BYTES_MUPDATE = 32                                   # d, r, p, q, 1/D; d, r, z
+BYTES_MUPDATE = 32                                   # d, r, p, q, 1/D (z-line: l, 1/d); d, r, z
 BYTES_FP64_ITER = 152                                # k64_spmv 88 + k64_update 64
 CALIB_BYTES = 1 << 30                                # calibration copy for the HBM counters
 
@@ -76,7 +76,9 @@ def spmv_kernel_name(info, coef):
     return f"k_mspmv4<{f}, {cpt // 4}>" if cpt >= 4 else f"k_mspmv<{f}, {cpt}>"
 
 
-def update_kernel_name(info):
+def update_kernel_name(info, zline=False):
+    if zline:
+        return "k_mupdate_line"
     cpt = info["tile_cells"] // 256
     return f"k_mupdate4<{cpt // 4}>" if cpt >= 4 else f"k_mupdate<{cpt}>"
 
@@ -163,16 +165,21 @@ def derive_traffic(fetch_path, write_path, workload, algo):
     return rec
 
 
-def measure_workload(eafp, syss, fmt, coef, interval, label):
+def measure_workload(eafp, syss, fmt, coef, interval, label, precond=0):
     """Mixed (batched) and plain FP64 GPU solves of one workload to 1e-8:
-    solves/s, iterations, and the iteration kernels' algorithmic GB/s."""
+    solves/s, iterations, and the iteration kernels' algorithmic GB/s.  The
+    z-line preconditioner is used when asked for and every nz allows it."""
     ncells = sum(p.ncells for p in syss)
     b = eafp.Batch.from_systems(syss)
-    rec = {"workload": label, "systems": len(syss), "cells": ncells, "coef_format": coef}
+    if precond and not all(p.shape[2] % 4 == 0 and p.shape[2] <= 1024 for p in syss):
+        precond = 0
+    rec = {"workload": label, "systems": len(syss), "cells": ncells, "coef_format": coef,
+           "precond": "zline" if precond else "jacobi"}
+    b.solve_mixed(tol=1e-8, interval=interval, coef_format=fmt, max_steps=1, precond=precond)
     b.autotune(eafp.Batch.SOLVER_MIXED, fmt, reps=3)
-    b.solve_mixed(tol=1e-8, interval=interval, coef_format=fmt)
+    b.solve_mixed(tol=1e-8, interval=interval, coef_format=fmt, precond=precond)
     t = time.perf_counter()
-    st = b.solve_mixed(tol=1e-8, interval=interval, coef_format=fmt)
+    st = b.solve_mixed(tol=1e-8, interval=interval, coef_format=fmt, precond=precond)
     tm = time.perf_counter() - t
     ms_sp, ms_up = b.time_kernels(eafp.Batch.SOLVER_MIXED, fmt, reps=10)
     rec["mixed"] = {"solves_per_s": round(len(syss) / tm, 3), "seconds": round(tm, 4),
@@ -459,6 +466,8 @@ def main():
     ap.add_argument("--systems", type=int, default=64)
     ap.add_argument("--coef", choices=("fp32", "bf16", "fp16"), default="bf16")
     ap.add_argument("--interval", type=int, default=32)
+    ap.add_argument("--precond", choices=("jacobi", "zline"), default="zline",
+                    help="inner preconditioner of the mixed solver")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-reference", action="store_true", help="skip the FP64 legs and error checks")
@@ -489,7 +498,9 @@ def main():
         return 2
 
     B = args.systems
-    workload = f"batch{B}_60x120x84_{args.coef}"
+    zline = args.precond == "zline"
+    precond = 1 if zline else 0
+    workload = f"batch{B}_60x120x84_{args.coef}" + ("_zline" if zline else "")
     if args.derive_traffic:
         if args.single:
             n1 = 60 * 120 * 84
@@ -551,13 +562,15 @@ def main():
         batch.set_tiling(shape[0], shape[1])
         batch.set_columns(shape[2] if len(shape) > 2 else 0)
     elif not args.no_tune:
+        # one short solve first, so the tuner times the preconditioner's kernels
+        batch.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt, max_steps=1, precond=precond)
         tune = batch.autotune(eafp.Batch.SOLVER_MIXED, fmt, reps=5)
         log(f"autotune (mixed {args.coef}): {tune}")
     if args.kernels_only:
         calibration_copy(CALIB_BYTES)
         ms_sp, ms_up = batch.time_kernels(eafp.Batch.SOLVER_MIXED, fmt, reps=20)
         out = {"kernels_only": True, "launch_shape": batch.info(), "coef_format": args.coef,
-               "kernels": [spmv_kernel_name(batch.info(), args.coef), update_kernel_name(batch.info())],
+               "kernels": [spmv_kernel_name(batch.info(), args.coef), update_kernel_name(batch.info(), zline)],
                "k_mspmv_avg_us": round(ms_sp * 1e3, 2), "k_mupdate_avg_us": round(ms_up * 1e3, 2),
                "k_mspmv_algorithmic_bytes": BYTES_MSPMV[args.coef] * ncells,
                "k_mupdate_algorithmic_bytes": BYTES_MUPDATE * ncells,
@@ -567,13 +580,13 @@ def main():
         return 0
 
     for w in range(args.warmup):
-        st = batch.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt)
+        st = batch.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt, precond=precond)
         log(f"warmup {w}: {st[0]['seconds']:.3f}s")
     device_sync()
     t_start = time.perf_counter()
     stats = []
     for k in range(args.steps):
-        stats = batch.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt)
+        stats = batch.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt, precond=precond)
     device_sync()
     elapsed = time.perf_counter() - t_start
     value = B * args.steps / elapsed
@@ -589,7 +602,7 @@ def main():
     ms_sp, ms_up = batch.time_kernels(eafp.Batch.SOLVER_MIXED, fmt, reps=20)
     info = batch.info()
     legs = [("k_mspmv", spmv_kernel_name(info, args.coef), ms_sp, BYTES_MSPMV[args.coef] * ncells),
-            ("k_mupdate", update_kernel_name(info), ms_up, BYTES_MUPDATE * ncells)]
+            ("k_mupdate", update_kernel_name(info, zline), ms_up, BYTES_MUPDATE * ncells)]
     legs.sort(key=lambda l: -l[2])
     rl = []
     for key, name, ms, nbytes in legs:
@@ -611,7 +624,7 @@ def main():
            "config": {"workload": f"batched p' solve: {B} seeded 60x120x84 campaign-variant systems "
                                   "(electrode immersion, slag foam, jittered a_P^u), phi=0, tol 1e-8",
                       "systems": B, "mesh": "60x120x84", "cells": ncells, "tol": 1e-8,
-                      "coef_format": args.coef, "refine_interval": args.interval,
+                      "coef_format": args.coef, "refine_interval": args.interval, "precond": args.precond,
                       "launch_shape": batch.info()},
            "roofline": roofline,
            "mixed_batch": {"all_converged": conv, "inner_iters_mean": float(np.mean(inner)),
@@ -643,11 +656,12 @@ def main():
             eafp.cg_3d_mpi(*arrs, phi_h, mesh, 100000, 1e-8)
         t_drop = (time.perf_counter() - t) / reps
         eafp.release()
+        single.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt, max_steps=1, precond=precond)
         single.autotune(eafp.Batch.SOLVER_MIXED, fmt, reps=5)
-        single.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt)
+        single.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt, precond=precond)
         t = time.perf_counter()
         for _ in range(reps):
-            smx = single.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt)[0]
+            smx = single.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt, precond=precond)[0]
         tmx = (time.perf_counter() - t) / reps
         xmx = single.download(0)[2:-2, 2:-2, 2:-2].copy()
         out["fp64_single"] = {"workload": "60x120x84 production system (liquid 7000 below 0.9 m, gas 0.3)",
@@ -723,13 +737,14 @@ def main():
         syss3 = [systems.make_system(3, s) for s in range(256)]
         out["other_workloads"].append(measure_workload(
             eafp, syss3, fmt, args.coef, args.interval,
-            "256 ragged systems U{18..35}x{36..66}x{24..50}, liquid height and gas density jittered"))
+            "256 ragged systems U{18..35}x{36..66}x{24..50}, liquid height and gas density jittered", precond))
         del syss3
         log("workload 256 ragged systems done")
         ps4 = systems.make_system(4, 0)
         out["other_workloads"].append(measure_workload(
             eafp, [ps4], COEF_BF16, "bf16", args.interval,
-            "one 120x240x168 system, scrap band 0.9<z<2.5 m with face coefficients x1e-2(1-alpha_s), BF16 weights"))
+            "one 120x240x168 system, scrap band 0.9<z<2.5 m with face coefficients x1e-2(1-alpha_s), BF16 weights",
+            precond))
         del ps4
         log("workload 120x240x168 done")
     print(json.dumps(out), flush=True)

@@ -49,6 +49,11 @@ int Batch::set_tiling(int cells_per_thread, int blocks) {
     cpt = cells_per_thread;
     EAFP_TRY_RC(tiles.alloc(ntiles));
     EAFP_TRY(hipMemcpy(tiles.p, tiles_h.data(), sizeof(TileDesc) * ntiles, hipMemcpyHostToDevice));
+    if (nltiles > 0) {   // the line tiles carry the tile ranges: rebuild them too
+        const std::vector<LineTile> lt = make_line_table(geom);
+        EAFP_TRY_RC(ltiles.alloc(lt.size()));
+        EAFP_TRY(hipMemcpy(ltiles.p, lt.data(), sizeof(LineTile) * lt.size(), hipMemcpyHostToDevice));
+    }
     if (geom_d.p)
         EAFP_TRY(hipMemcpy(geom_d.p, geom.data(), sizeof(SysGeom) * nsys, hipMemcpyHostToDevice));
     if (blocks > 0) {
@@ -106,6 +111,7 @@ int Batch::init(int n, const eafp_desc* descs) {
         geom[s].nz = descs[s].nz;
     }
     ncells = layout_systems(geom);
+    nltiles = line_ranges(geom);
     quad = true;
     for (const SysGeom& g : geom) quad = quad && (g.nz % 4 == 0);
     EAFP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));

@@ -87,6 +87,8 @@ struct Batch {
     DevBuf<ColDesc> cols;
     std::vector<TileDesc> tiles_h;
     DevBuf<TileDesc> tiles;
+    int nltiles = 0;                     // line tiles (0: the batch has no z-line kernels)
+    DevBuf<LineTile> ltiles;
     DevBuf<double> coef[NCOEF];          // FP64 operand arenas, compact
     DevBuf<double> b;                    // Su plus the folded wall-halo terms of phi0
     DevBuf<double> x0;                   // initial phi interior

@@ -32,12 +32,14 @@ constexpr int MAX_COMP = 4;                // projected (singular) components pe
 constexpr unsigned char NO_COMP = 255;      // cell outside every projected component
 
 // Per-tile descriptor: the tile's system and that system's geometry in
-// one 48-byte record, so a block needs a single load to address its cells.
+// one 56-byte record, so a block needs a single load to address its cells.
 struct TileDesc {
     int sys;
     int tile_begin, tile_end;   // the system's tiles
     int n;                      // cells of the system
-    int nr, nth, nz, pad;
+    int nr, nth, nz;
+    int lt_begin, lt_end;       // the system's line tiles (z-line preconditioned kernels)
+    int pad;
     long off;                   // first cell of the system in the arenas
     float inv_nz, inv_nth;
 };
@@ -51,6 +53,8 @@ struct SysGeom {
     int tile_end;
     float inv_nz;     // 1/nz, 1/nth for the float-reciprocal division
     float inv_nth;
+    int lt_begin;     // [lt_begin, lt_end) in the line-tile table (0, 0: none)
+    int lt_end;
 };
 
 // Lays out the systems' cells in one arena: fills nr/nth/nz/n/off and the
@@ -90,10 +94,77 @@ inline std::vector<TileDesc> make_tile_table(std::vector<SysGeom>& g, int tile) 
         d.off = G.off;
         d.inv_nz = G.inv_nz;
         d.inv_nth = G.inv_nth;
+        d.lt_begin = G.lt_begin;
+        d.lt_end = G.lt_end;
         tiles.insert(tiles.end(), nt, d);
         t += nt;
     }
     return tiles;
+}
+
+// ---------------------------------------------------------------------------
+// Line tiles, the work unit of the z-line preconditioned kernels of the
+// mixed solver.  Line l = i*nth + j of a system is its z-row, the nz
+// consecutive cells l*nz .. l*nz + nz-1.  A block takes lpb = BLOCK/(nz/4)
+// whole lines of one system, nz/4 lanes per line with one quad each, so
+// thread t holds the cells line0*nz + 4t .. +3: the block's lines are one
+// contiguous run of cells, read with 16-byte accesses.  Needs every nz a
+// multiple of 4 and at most 4*BLOCK.
+// ---------------------------------------------------------------------------
+struct LineTile {
+    int sys;
+    int line0, nlines;          // lines line0 .. line0+nlines-1 of the system
+    int nz;
+    int tile_begin, tile_end;   // the system's tiles (the stencil's partials)
+    int lt_begin, lt_end;       // the system's line tiles (this kernel's partials)
+    long off;                   // first cell of the system in the arenas
+};
+
+__host__ __device__ __forceinline__ int line_lanes(int nz) { return nz / 4; }
+__host__ __device__ __forceinline__ int line_lpb(int nz) { return BLOCK / (nz / 4); }
+
+inline bool line_eligible(const std::vector<SysGeom>& g) {
+    for (const SysGeom& s : g)
+        if (s.nz < 4 || s.nz % 4 != 0 || s.nz > 4 * BLOCK) return false;
+    return true;
+}
+
+// Sets every system's line-tile range (consecutive, systems in order), or
+// clears them all when the batch is not eligible.  Returns the tile count.
+inline int line_ranges(std::vector<SysGeom>& g) {
+    const bool ok = line_eligible(g);
+    int t = 0;
+    for (SysGeom& s : g) {
+        const int nlt = ok ? (s.nr * s.nth + line_lpb(s.nz) - 1) / line_lpb(s.nz) : 0;
+        s.lt_begin = t;
+        s.lt_end = t + nlt;
+        t += nlt;
+    }
+    return t;
+}
+
+// The line-tile table (after make_tile_table has set the tile ranges).
+inline std::vector<LineTile> make_line_table(const std::vector<SysGeom>& g) {
+    std::vector<LineTile> out;
+    for (int s = 0; s < (int)g.size(); ++s) {
+        const SysGeom& G = g[s];
+        const int lines = G.nr * G.nth;
+        const int lpb = G.lt_end > G.lt_begin ? line_lpb(G.nz) : 0;
+        for (int t = G.lt_begin; t < G.lt_end; ++t) {
+            LineTile L{};
+            L.sys = s;
+            L.line0 = (t - G.lt_begin) * lpb;
+            L.nlines = lines - L.line0 < lpb ? lines - L.line0 : lpb;
+            L.nz = G.nz;
+            L.tile_begin = G.tile_begin;
+            L.tile_end = G.tile_end;
+            L.lt_begin = G.lt_begin;
+            L.lt_end = G.lt_end;
+            L.off = G.off;
+            out.push_back(L);
+        }
+    }
+    return out;
 }
 
 // ---------------------------------------------------------------------------

@@ -86,6 +86,9 @@ struct MxArgs {
     float* scale;        // FP16 only: per-cell scale (the owner's aP)
     void* e;             // diagonal excess, in the weight format (FP16: scaled like the weights)
     float* dinv;         // Jacobi inverse of the rounded operator
+    const LineTile* lt;  // line tiles (z-line preconditioner)
+    unsigned* fac;       // z-line factors per cell: FP16 l (low half), BF16 1/d (high half)
+    int line;            // 1: z-line preconditioner; the r.z partials are per line tile
     // inner FP32 vectors
     float *d, *r, *z, *q;
     float* p[2];
@@ -93,7 +96,7 @@ struct MxArgs {
     double* sumR;        // nsys * NRES: partR reduced per system (k_rsum)
     const SysGeom* geom; // per-system geometry and tile ranges
     double* partA;       // ntiles        (k_mspmv: p.q)
-    double* partB;       // ntiles * 2    (k_mreplace, k_mupdate: r.r, r.z)
+    double* partB;       // ntiles * 2, or nltiles * 2 with `line` (k_mreplace, k_mupdate: r.r, r.z)
     MxDyn* dyn[2];
     MxStat* stat[2];
     const MxConst* cst;
@@ -207,6 +210,7 @@ __global__ __launch_bounds__(BLOCK) void k_compress(MxArgs a, int tile) {
                 if (F == EAFP_COEF_FP16) a.scale[o] = s_o;
                 ((typename W::T*)a.e)[o] = ee;
                 a.dinv[o] = D > 0.f ? 1.0f / D : 0.0f;
+                if (a.line) a.q[o] = D;   // the z-line factorization reads it (q is free until the first stencil)
             }
         }
         t += nrun * w.stride;
@@ -471,7 +475,7 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv(MxArgs a, int par) {
         bool skip = P.done || P.broken;
         if (!skip) {
             double tot[2];
-            reduce_tiles<2>(a.partB, T.tile_begin, T.tile_end, tot, sh);
+            reduce_tiles<2>(a.partB, a.line ? T.lt_begin : T.tile_begin, a.line ? T.lt_end : T.tile_end, tot, sh);
             if (!(tot[1] > 0.0)) {
                 skip = true;     // r.z lost positivity: wait for the next refinement step
                 N.broken = 1;
@@ -662,7 +666,7 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv4(MxArgs a, int par) {
         bool skip = P.done || P.broken;
         if (!skip) {
             double tot[2];
-            reduce_tiles<2>(a.partB, T.tile_begin, T.tile_end, tot, sh);
+            reduce_tiles<2>(a.partB, a.line ? T.lt_begin : T.tile_begin, a.line ? T.lt_end : T.tile_end, tot, sh);
             if (!(tot[1] > 0.0)) {
                 skip = true;
                 N.broken = 1;
@@ -867,7 +871,7 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
     bool skip = P.done || P.broken;
     if (!skip) {
         double tot[2];
-        reduce_tiles<2>(a.partB, T.tile_begin, T.tile_end, tot, sh);
+        reduce_tiles<2>(a.partB, a.line ? T.lt_begin : T.tile_begin, a.line ? T.lt_end : T.tile_end, tot, sh);
         if (!(tot[1] > 0.0)) {
             skip = true;
             N.broken = 1;
@@ -1058,7 +1062,7 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_colg(MxArgs a, int par) {
     bool skip = P.done || P.broken;
     if (!skip) {
         double tot[2];
-        reduce_tiles<2>(a.partB, T.tile_begin, T.tile_end, tot, sh);
+        reduce_tiles<2>(a.partB, a.line ? T.lt_begin : T.tile_begin, a.line ? T.lt_end : T.tile_end, tot, sh);
         if (!(tot[1] > 0.0)) {
             skip = true;
             N.broken = 1;
@@ -1170,6 +1174,268 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_colg(MxArgs a, int par) {
 }
 
 // Finished systems in slot 1 (one block).
+// ---------------------------------------------------------------------------
+// z-line preconditioner.  M is the tridiagonal part along z of the rounded
+// operator (diagonal D as the Jacobi inverse sees it, off-diagonals minus
+// the +z face weights), factored M = L diag(d) L^T with L unit lower
+// bidiagonal: l_k = -w_{k-1}/d_{k-1}, d_k = D_k - w_{k-1}^2/d_{k-1}.  Any
+// positive d gives a symmetric positive definite M, so d_k is kept at or
+// above D_k/16 on nearly singular lines (an isolated Neumann column), which
+// bounds the amplification of z = M^-1 r.  Rows with D = 0 get l = 0,
+// 1/d = 0 (z = 0, as Jacobi's 1/D = 0).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned pack_fac(double l, double inv) {
+    const _Float16 lh = (_Float16)(float)l;
+    unsigned short lb;
+    __builtin_memcpy(&lb, &lh, 2);
+    return (unsigned)lb | ((unsigned)f32_to_bf16_rne((float)inv) << 16);
+}
+
+__device__ __forceinline__ void unpack_fac(float bits, float& l, float& inv) {
+    const unsigned u = __float_as_uint(bits);
+    const unsigned short lb = (unsigned short)(u & 0xFFFFu);
+    _Float16 lh;
+    __builtin_memcpy(&lh, &lb, 2);
+    l = (float)lh;
+    inv = __uint_as_float(u & 0xFFFF0000u);
+}
+
+// One thread per line, k = 0 .. nz-1 in order (once per solve, after
+// k_compress has left D in q).
+template <int F>
+__global__ __launch_bounds__(BLOCK) void k_linefac(MxArgs a) {
+    using W = WFmt<F>;
+    const LineTile L = a.lt[blockIdx.x];
+    const int li = threadIdx.x;
+    if (li >= L.nlines) return;
+    const long o0 = L.off + (long)(L.line0 + li) * L.nz;
+    double dprev = 0.0, wprev = 0.0;
+    for (int k = 0; k < L.nz; ++k) {
+        const long o = o0 + k;
+        const double Dk = a.q[o];
+        double l = 0.0, dk = Dk;
+        if (k > 0 && dprev > 0.0) {
+            l = -wprev / dprev;
+            dk = Dk - wprev * wprev / dprev;
+        }
+        dk = fmax(dk, 0.0625 * Dk);
+        wprev = W::load(a.w[2], a.scale, o);
+        const bool live = Dk > 0.0;
+        a.fac[o] = pack_fac(live ? l : 0.0, live ? 1.0 / dk : 0.0);
+        dprev = live ? dk : 0.0;
+    }
+}
+
+// z = M^-1 r on the block's lines: thread t holds 4 consecutive cells of
+// line li = t/nq (lane j = t%nq).  Forward L y = r and backward
+// L^T z = y/d are first-order recurrences; every lane composes its 4
+// cells into one affine map, the maps go through LDS, and each lane folds
+// the maps of the lanes before (forward) or after (backward) it in its line.
+// Called by the whole block.
+struct LineLds {
+    float2 fw[BLOCK];
+    float2 bw[BLOCK];
+    float lf[BLOCK];
+};
+
+__device__ __forceinline__ void line_solve(LineLds& S, bool inl, int li, int j, int nq, const float (&r)[4],
+                                           const float (&l)[4], const float (&iv)[4], float (&z)[4]) {
+    const int tid = threadIdx.x;
+    float A = 1.f, Bv = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        A = -l[k] * A;
+        Bv = r[k] - l[k] * Bv;
+    }
+    S.fw[tid] = make_float2(A, Bv);
+    S.lf[tid] = l[0];
+    __syncthreads();
+    float y = 0.f;
+    if (inl)
+        for (int m = 0; m < j; ++m) {
+            const float2 f = S.fw[li * nq + m];
+            y = f.x * y + f.y;
+        }
+    float u[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        y = r[k] - l[k] * y;
+        u[k] = y * iv[k];
+    }
+    const float ln = (inl && j + 1 < nq) ? S.lf[tid + 1] : 0.f;
+    float Ab = -ln, Bb = u[3];
+#pragma unroll
+    for (int k = 2; k >= 0; --k) {
+        Ab = -l[k + 1] * Ab;
+        Bb = u[k] - l[k + 1] * Bb;
+    }
+    S.bw[tid] = make_float2(Ab, Bb);
+    __syncthreads();
+    float zin = 0.f;
+    if (inl)
+        for (int m = nq - 1; m > j; --m) {
+            const float2 f = S.bw[li * nq + m];
+            zin = f.x * zin + f.y;
+        }
+    z[3] = u[3] - ln * zin;
+#pragma unroll
+    for (int k = 2; k >= 0; --k) z[k] = u[k] - l[k + 1] * z[k + 1];
+}
+
+// k_mupdate with the z-line preconditioner: one line tile per block.
+__global__ __launch_bounds__(BLOCK) void k_mupdate_line(MxArgs a, int par) {
+    __shared__ double sh[2 * (BLOCK / 64)];
+    __shared__ LineLds S;
+    const LineTile L = a.lt[blockIdx.x];
+    const int s = L.sys;
+    const int tid = threadIdx.x;
+    const int nq = line_lanes(L.nz);
+    const int li = tid / nq, j = tid - li * nq;
+    const bool inl = li < L.nlines;
+    const long o = L.off + (long)L.line0 * L.nz + 4 * (long)(inl ? tid : 0);
+    const F4 dv = ld4nt(a.d + o), pvv = ld4nt(a.p[par ^ 1] + o), rv = ld4nt(a.r + o), qv = ld4nt(a.q + o);
+    const F4 fv = ld4nt((const float*)a.fac + o);
+    const MxDyn P = a.dyn[0][s];
+    MxDyn N = P;
+    bool skip = P.done || P.broken;
+    if (!skip) {
+        double tot[1];
+        reduce_tiles<1>(a.partA, L.tile_begin, L.tile_end, tot, sh);
+        if (!(fabs(tot[0]) > SMALL)) {
+            skip = true;
+            N.broken = 1;
+        } else {
+            N.alpha = P.rz / tot[0];
+            N.total_inner = P.total_inner + 1;
+        }
+    }
+    if ((int)blockIdx.x == L.lt_begin && tid == 0) store_state(&a.dyn[1][s], N);
+    if (skip) return;   // uniform over the block
+    const float fa = (float)N.alpha;
+    float rn[4], l[4], iv[4], zn[4];
+    F4 dn, rr, zz;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        rn[k] = rv.v[k] - fa * qv.v[k];
+        dn.v[k] = dv.v[k] + fa * pvv.v[k];
+        unpack_fac(fv.v[k], l[k], iv[k]);
+    }
+    line_solve(S, inl, li, j, nq, rn, l, iv, zn);
+    double acc[2] = {0.0, 0.0};
+    if (inl) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            rr.v[k] = rn[k];
+            zz.v[k] = zn[k];
+            acc[0] += (double)rn[k] * (double)rn[k];
+            acc[1] += (double)rn[k] * (double)zn[k];
+        }
+        st4nt(a.d + o, dn);
+        st4nt(a.r + o, rr);
+        st4(a.z + o, zz);
+    }
+    block_sum<2>(acc, sh);
+    store_run<2>(a.partB, (int)blockIdx.x, 1, 1, acc);
+}
+
+// k_mreplace with the z-line preconditioner: one line tile per block.
+__global__ __launch_bounds__(BLOCK) void k_mreplace_line(MxArgs a) {
+    __shared__ double sh[NRES * (BLOCK / 64)];
+    __shared__ LineLds S;
+    const LineTile L = a.lt[blockIdx.x];
+    const int s = L.sys;
+    const int tid = threadIdx.x;
+    const bool writer = (int)blockIdx.x == L.lt_begin && tid == 0;
+    const MxStat P = a.stat[0][s];
+    bool done = P.done != 0;
+    bool zero_p = false;
+    double mean_r[MAX_COMP] = {};
+    if (!done) {
+        const MxConst& K = a.cst[s];
+        double tot[NRES];
+#pragma unroll
+        for (int c = 0; c < NRES; ++c) tot[c] = a.sumR[(long)s * NRES + c];
+        const int rnd = P.n_refine;
+        MxStat N = P;
+        N.bnorm = rnd == 0 ? sqrt(tot[2]) : P.bnorm;
+        N.b1 = rnd == 0 ? tot[3] : P.b1;
+        int status = EAFP_STOP_RUNNING;
+        if (N.bnorm < SMALL) {
+            N.res_l2 = 0.0;
+            N.res_l1 = 0.0;
+            status = EAFP_STOP_ZERO_RHS;
+        } else {
+            N.res_l2 = sqrt(tot[0]) / N.bnorm;
+            N.res_l1 = N.b1 > 0.0 ? tot[1] / N.b1 : tot[1];
+            if (N.res_l2 < K.tol) status = EAFP_STOP_CONVERGED;
+            else if (rnd >= K.max_steps) status = EAFP_STOP_MAX_ITER;
+        }
+        done = status != EAFP_STOP_RUNNING;
+        const MxDyn D0 = a.dyn[0][s];
+        zero_p = (rnd == 0) || K.restart || D0.broken;
+#pragma unroll
+        for (int q = 0; q < MAX_COMP; ++q) {
+            mean_r[q] = q < K.ncomp ? tot[4 + q] * K.inv_cnt[q] : 0.0;
+            N.mean_x[q] = q < K.ncomp ? tot[4 + MAX_COMP + q] * K.inv_cnt[q] : 0.0;
+        }
+        N.xbuf = (rnd + 1) & 1;
+        MxDyn D = D0;
+        if (done) {
+            N.done = 1;
+            N.status = status;
+            D.done = 1;
+        } else {
+            N.n_refine = rnd + 1;
+            D.fresh = zero_p ? 1 : 0;
+            D.broken = 0;
+        }
+        if (writer) {
+            store_state(&a.stat[1][s], N);
+            store_state(&a.dyn[1][s], D);
+        }
+    } else if (writer) {
+        const MxDyn D0 = a.dyn[0][s];
+        store_state(&a.stat[1][s], P);
+        store_state(&a.dyn[1][s], D0);
+    }
+    if (done) return;   // uniform over the block
+    const int nq = line_lanes(L.nz);
+    const int li = tid / nq, j = tid - li * nq;
+    const bool inl = li < L.nlines;
+    const long o = L.off + (long)L.line0 * L.nz + 4 * (long)(inl ? tid : 0);
+    const F4 fv = ld4nt((const float*)a.fac + o);
+    float rr[4], l[4], iv[4], zn[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int slot = a.comp[o + k];
+        const double rv = __builtin_nontemporal_load(a.r64 + o + k);
+        double mu = 0.0;
+#pragma unroll
+        for (int q = 0; q < MAX_COMP; ++q) mu = slot == q ? mean_r[q] : mu;
+        rr[k] = (float)(rv - mu);
+        unpack_fac(fv.v[k], l[k], iv[k]);
+    }
+    line_solve(S, inl, li, j, nq, rr, l, iv, zn);
+    double acc[2] = {0.0, 0.0};
+    if (inl) {
+        F4 rv4, zv4, zero;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            rv4.v[k] = rr[k];
+            zv4.v[k] = zn[k];
+            zero.v[k] = 0.0f;
+            acc[0] += (double)rr[k] * (double)rr[k];
+            acc[1] += (double)rr[k] * (double)zn[k];
+        }
+        st4nt(a.r + o, rv4);
+        st4(a.z + o, zv4);
+        st4nt(a.d + o, zero);
+        if (zero_p) st4(a.p[0] + o, zero);
+    }
+    block_sum<2>(acc, sh);
+    store_run<2>(a.partB, (int)blockIdx.x, 1, 1, acc);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_mcount(MxArgs a) {
     __shared__ double sh[BLOCK / 64];
     double v[1] = {0.0};
@@ -1213,6 +1479,7 @@ struct MixedWork {
     DevBuf<unsigned char> w[3];
     DevBuf<unsigned char> e;
     DevBuf<float> scale, dinv, d, r, z, q, p0, p1;
+    DevBuf<unsigned> fac;           // z-line factors (allocated on first use)
     DevBuf<MxDyn> dyn0, dyn1;
     DevBuf<MxStat> stat0, stat1;
     DevBuf<MxConst> cst;
@@ -1221,7 +1488,7 @@ struct MixedWork {
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t t0 = nullptr, t1 = nullptr;
     hipGraphExec_t exec = nullptr;
-    int exec_interval = 0, exec_fmt = -1, exec_gen = -1;
+    int exec_interval = 0, exec_fmt = -1, exec_gen = -1, exec_line = -1;
     MxArgs args;
 };
 
@@ -1279,7 +1546,8 @@ static int ensure_mixed(Batch& B, int fmt) {
     EAFP_TRY_RC(w->partR.alloc((size_t)B.ntiles * NRES));
     EAFP_TRY_RC(w->sumR.alloc((size_t)B.nsys * NRES));
     EAFP_TRY_RC(w->partA.alloc((size_t)B.ntiles));
-    EAFP_TRY_RC(w->partB.alloc((size_t)B.ntiles * 2));
+    EAFP_TRY_RC(w->partB.alloc((size_t)std::max(B.ntiles, B.nltiles) * 2));
+    if (B.nltiles > 0 && !w->fac.p) EAFP_TRY_RC(w->fac.alloc(B.ncells));
     MxArgs& a = w->args;
     a.tiles = B.tiles.p;
     a.ntiles = B.ntiles;
@@ -1304,6 +1572,8 @@ static int ensure_mixed(Batch& B, int fmt) {
     a.scale = fmt == EAFP_COEF_FP16 ? w->scale.p : nullptr;
     a.e = w->e.p;
     a.dinv = w->dinv.p;
+    a.lt = B.ltiles.p;
+    a.fac = w->fac.p;
     a.d = w->d.p;
     a.r = w->r.p;
     a.z = w->z.p;
@@ -1360,6 +1630,10 @@ static void launch_mspmv(Batch& B, int fmt, int par) {
 }
 
 static void launch_mupdate(Batch& B, int par) {
+    if (B.mixed->args.line) {
+        k_mupdate_line<<<B.nltiles, BLOCK, 0, B.stream>>>(B.mixed->args, par);
+        return;
+    }
     if (B.quad && B.cpt >= 4) {
         if (B.cpt == 8) k_mupdate4<2><<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, par);
         else k_mupdate4<1><<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, par);
@@ -1377,12 +1651,14 @@ static void launch_refine(Batch& B) {
     const int tile = BLOCK * B.cpt;
     k_res<<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, tile);
     k_rsum<<<B.nsys, BLOCK, 0, B.stream>>>(B.mixed->args);
-    k_mreplace<<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, tile);
+    if (B.mixed->args.line) k_mreplace_line<<<B.nltiles, BLOCK, 0, B.stream>>>(B.mixed->args);
+    else k_mreplace<<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, tile);
 }
 
 static int mixed_graph(Batch& B, int fmt, int interval) {
     MixedWork* w = B.mixed;
-    if (w->exec && w->exec_interval == interval && w->exec_fmt == fmt && w->exec_gen == B.tiling_gen)
+    if (w->exec && w->exec_interval == interval && w->exec_fmt == fmt && w->exec_gen == B.tiling_gen &&
+        w->exec_line == w->args.line)
         return EAFP_OK;
     if (w->exec) {
         (void)hipGraphExecDestroy(w->exec);
@@ -1404,6 +1680,7 @@ static int mixed_graph(Batch& B, int fmt, int interval) {
     w->exec_interval = interval;
     w->exec_fmt = fmt;
     w->exec_gen = B.tiling_gen;
+    w->exec_line = w->args.line;
     return EAFP_OK;
 }
 
@@ -1428,6 +1705,11 @@ static int mixed_start(Batch& B, const eafp_mixed_opts& o, int fmt) {
     if (fmt == EAFP_COEF_BF16) k_compress<EAFP_COEF_BF16><<<B.grid, BLOCK, 0, B.stream>>>(w->args, tile);
     else if (fmt == EAFP_COEF_FP16) k_compress<EAFP_COEF_FP16><<<B.grid, BLOCK, 0, B.stream>>>(w->args, tile);
     else k_compress<EAFP_COEF_FP32><<<B.grid, BLOCK, 0, B.stream>>>(w->args, tile);
+    if (w->args.line) {
+        if (fmt == EAFP_COEF_BF16) k_linefac<EAFP_COEF_BF16><<<B.nltiles, BLOCK, 0, B.stream>>>(w->args);
+        else if (fmt == EAFP_COEF_FP16) k_linefac<EAFP_COEF_FP16><<<B.nltiles, BLOCK, 0, B.stream>>>(w->args);
+        else k_linefac<EAFP_COEF_FP32><<<B.nltiles, BLOCK, 0, B.stream>>>(w->args);
+    }
     launch_refine(B);    // x = x0, first residual, first inner start
     EAFP_TRY(hipGetLastError());
     return EAFP_OK;
@@ -1440,6 +1722,8 @@ static int check_mixed(Batch& B, const eafp_mixed_opts& o) {
     }
     if (!(o.tol > 0.0) || o.max_steps < 1 || o.interval < 0) return EAFP_ERR_ARG;
     if (o.coef_format < EAFP_COEF_FP32 || o.coef_format > EAFP_COEF_FP16) return EAFP_ERR_ARG;
+    if (o.precond != EAFP_PRECOND_JACOBI && o.precond != EAFP_PRECOND_ZLINE) return EAFP_ERR_ARG;
+    if (o.precond == EAFP_PRECOND_ZLINE && B.nltiles == 0) return EAFP_ERR_UNSUPPORTED;
     return EAFP_OK;
 }
 
@@ -1448,6 +1732,7 @@ int solve_mixed(Batch& B, const eafp_mixed_opts& o, eafp_stats* stats) {
     const int fmt = o.coef_format;
     EAFP_TRY_RC(ensure_mixed(B, fmt));
     MixedWork* w = B.mixed;
+    w->args.line = o.precond == EAFP_PRECOND_ZLINE ? 1 : 0;
     int interval = o.interval > 0 ? o.interval : 32;
     interval = (interval + 1) & ~1;    // even: the p buffer parity restarts every step
     EAFP_TRY_RC(mixed_graph(B, fmt, interval));

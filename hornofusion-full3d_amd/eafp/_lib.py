@@ -22,6 +22,7 @@ EAFP_ERR_STATE = -5
 STOP_NAMES = {0: "running", 1: "converged", 2: "breakdown", 3: "max_iter", 4: "zero_rhs"}
 
 COEF_FP32, COEF_BF16, COEF_FP16 = 0, 1, 2
+PRECOND_JACOBI, PRECOND_ZLINE = 0, 1
 
 # Every symbol declared in include/eafp.h.
 EXPORTS = (
@@ -101,7 +102,7 @@ class Stats(ctypes.Structure):
 
 class MixedOpts(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("interval", ctypes.c_int), ("max_steps", ctypes.c_int),
-                ("coef_format", ctypes.c_int), ("restart", ctypes.c_int)]
+                ("coef_format", ctypes.c_int), ("restart", ctypes.c_int), ("precond", ctypes.c_int)]
 
 
 STEP_FP64, STEP_MIXED = 0, 1

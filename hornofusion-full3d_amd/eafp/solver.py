@@ -171,7 +171,8 @@ def pp_step(F, cfg, m, alpha_p, solver="fp64", max_iter=1000, tol=1e-8, mixed=No
         else:
             out[n] = np.array(F[n], dtype=np.float64, order="F")
     io = _lib.PPUpdate(**{n: out[n].ctypes.data for n in _lib.PP_UPDATE})
-    mo = dict(tol=1e-8, interval=32, max_steps=2000, coef_format=_lib.COEF_BF16, restart=0)
+    mo = dict(tol=1e-8, interval=32, max_steps=2000, coef_format=_lib.COEF_BF16, restart=0,
+              precond=_lib.PRECOND_JACOBI)
     mo.update(mixed or {})
     if solver not in ("fp64", "mixed"):
         raise ValueError(f"solver must be 'fp64' or 'mixed', not {solver!r}")
@@ -255,9 +256,11 @@ class Batch:
         check(self._L.eafp_batch_solve_fp64(self._h, int(max_iter), float(tol), st), "eafp_batch_solve_fp64")
         return [s.as_dict() for s in st]
 
-    def solve_mixed(self, tol=1e-8, interval=32, max_steps=2000, coef_format=_lib.COEF_FP32, restart=False):
-        """FP32 inner CG + FP64 refinement every `interval` inner iterations."""
-        o = MixedOpts(float(tol), int(interval), int(max_steps), int(coef_format), int(bool(restart)))
+    def solve_mixed(self, tol=1e-8, interval=32, max_steps=2000, coef_format=_lib.COEF_FP32, restart=False,
+                    precond=_lib.PRECOND_JACOBI):
+        """FP32 inner CG + FP64 refinement every `interval` inner iterations;
+        ``precond`` the inner preconditioner (PRECOND_JACOBI or PRECOND_ZLINE)."""
+        o = MixedOpts(float(tol), int(interval), int(max_steps), int(coef_format), int(bool(restart)), int(precond))
         st = (Stats * len(self))()
         check(self._L.eafp_batch_solve_mixed(self._h, ctypes.byref(o), st), "eafp_batch_solve_mixed")
         return [s.as_dict() for s in st]

@@ -79,7 +79,17 @@ typedef struct eafp_mixed_opts {
     int restart;          /* 0: the inner CG keeps its search direction across
                              refinement steps (residual replacement); 1: it restarts
                              at every step (classical iterative refinement) */
+    int precond;          /* EAFP_PRECOND_*: preconditioner of the inner CG */
 } eafp_mixed_opts;
+
+/* Inner preconditioners of the mixed solver.  JACOBI: the diagonal of the
+ * rounded operator (what cg_3d_mpi uses).  ZLINE: its tridiagonal part along
+ * every z-line, M = L D L^T (l in FP16, 1/d in BF16: 4 bytes per cell, as
+ * Jacobi's 1/D), applied by two scans per line; fewer inner iterations on
+ * the furnace's p' systems.  ZLINE needs every nz a multiple of 4 and at
+ * most 1024 (EAFP_ERR_UNSUPPORTED otherwise). */
+#define EAFP_PRECOND_JACOBI 0
+#define EAFP_PRECOND_ZLINE 1
 
 /* ---------------------------------------------------------------------------
  * Drop-in: cg_3d_mpi(aW,aE,aS,aN,aB,aT,aP,Su,phi,m,max_iter,tol,residual,n_iter)

@@ -30,10 +30,11 @@ def test_header_symbols_exported():
 
 
 def test_struct_layouts_match_header():
-    # eafp_desc: 7 ints; eafp_stats: double, 4 ints, 3 doubles; eafp_mixed_opts: double, 4 ints
+    # eafp_desc: 7 ints; eafp_stats: double, 4 ints, 3 doubles; eafp_mixed_opts: double, 5 ints (+4 padding)
     assert ctypes.sizeof(_lib.Desc) == 28
     assert ctypes.sizeof(_lib.Stats) == 48
-    assert ctypes.sizeof(_lib.MixedOpts) == 24
+    assert ctypes.sizeof(_lib.MixedOpts) == 32
+    assert _lib.MixedOpts.precond.offset == 24
 
 
 def _arrays(nr=4, nth=6, nz=5):
@@ -179,8 +180,8 @@ def test_tdma_rejects_bad_calls():
 
 
 def test_pp_step_struct_layout():
-    # eafp_pp_step_opts: 2 ints, 2 doubles, then an eafp_mixed_opts (24 bytes) at offset 24
-    assert ctypes.sizeof(_lib.PPStepOpts) == 48
+    # eafp_pp_step_opts: 2 ints, 2 doubles, then an eafp_mixed_opts (32 bytes) at offset 24
+    assert ctypes.sizeof(_lib.PPStepOpts) == 56
     assert _lib.PPStepOpts.mixed.offset == 24
     assert _lib.PPStepOpts.alpha_p.offset == 16
 

@@ -1,0 +1,158 @@
+"""GPU parity of the mixed solver with the z-line inner preconditioner
+(EAFP_PRECOND_ZLINE): the same bars as the Jacobi form (tests/
+test_gpu_mixed.py), checked against the FP64 oracle of cg_3d_mpi on the
+same systems:
+  - true relative residual ||b - A x||/||b|| < 1e-8 with the caller's FP64
+    operator (independent scipy operator on the host, 1.2e-8 for its
+    rounding), agreeing with the solver's own FP64 residual;
+  - relative p' error against the oracle (tol 1e-12) after the same gauge
+    fix below 1e-5; zero mean on every singular component; inactive cells 0;
+  - every tiling and stencil form gives a converged answer, a zero-RHS
+    system stops at once and a NaN system never converges without holding
+    the others back; a repeated solve repeats bit for bit;
+  - on the production meshes it needs fewer inner iterations than Jacobi
+    (the point of the preconditioner).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import eafp
+import oracle
+from eafp import systems
+from eafp._lib import COEF_BF16, COEF_FP16, COEF_FP32, PRECOND_JACOBI, PRECOND_ZLINE, EAFP_ERR_UNSUPPORTED
+
+pytestmark = pytest.mark.gpu
+
+FORMATS = {"fp32": COEF_FP32, "bf16": COEF_BF16, "fp16": COEF_FP16}
+
+
+def host_true_residual(ps, phi):
+    A = oracle.csr(ps)
+    x = oracle.interior(phi).ravel()
+    b = ps.interior("Su").ravel()
+    return np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+
+
+_ref = {}
+
+
+def oracle_solution(ps, key):
+    if key not in _ref:
+        phi, _, nit, _ = oracle.cg(ps, max_iter=200000, tol=1e-12)
+        _ref[key] = (oracle.interior(phi), nit)
+    return _ref[key]
+
+
+def check_answer(ps, phi, st, key):
+    x_ref, _ = oracle_solution(ps, key)
+    err = oracle.gauge_error(oracle.interior(phi), x_ref, ps.labels, ps.ncomp)
+    tr = host_true_residual(ps, phi)
+    print(f"  status={st['status']} inner={st['n_iter']} refine={st['n_refine']} res={st['residual']:.2e} "
+          f"host_res={tr:.2e} err={err:.2e}", flush=True)
+    assert st["status"] == "converged"
+    assert st["residual"] < 1e-8 and tr < 1.2e-8
+    assert abs(tr - st["residual"]) < 1e-3 * st["residual"] + 2e-12
+    assert err < 1e-5
+    x = oracle.interior(phi)
+    for c in range(ps.ncomp):
+        m = ps.labels == c
+        assert abs(x[m].mean()) < 1e-12 * np.abs(x).max() + 1e-300
+    assert not x[~ps.active].any()
+
+
+@pytest.mark.parametrize("fmt", list(FORMATS))
+@pytest.mark.parametrize("config,shape", [(1, None), (2, (20, 36, 28)), (3, (26, 54, 28))])
+def test_zline_single(config, shape, fmt):
+    ps = systems.make_system(config, 0, shape)
+    print(f"case zline config={config} shape={ps.shape} fmt={fmt}", flush=True)
+    with eafp.Batch.from_systems([ps]) as b:
+        st = b.solve_mixed(tol=1e-8, coef_format=FORMATS[fmt], precond=PRECOND_ZLINE)[0]
+        phi = b.download(0)
+    check_answer(ps, phi, st, (config, shape))
+
+
+@pytest.mark.parametrize("cpt,cols", [(2, 0), (8, 0), (4, 2)])
+def test_zline_production_fewer_iterations(cpt, cols):
+    """60x120x84 under every stencil form; fewer inner iterations than Jacobi."""
+    ps = systems.make_system(2)
+    print(f"case zline production cpt={cpt} columns={cols}", flush=True)
+    with eafp.Batch.from_systems([ps]) as b:
+        b.set_tiling(cpt, 0)
+        b.set_columns(cols)
+        stj = b.solve_mixed(tol=1e-8, coef_format=COEF_BF16, precond=PRECOND_JACOBI)[0]
+        st = b.solve_mixed(tol=1e-8, coef_format=COEF_BF16, precond=PRECOND_ZLINE)[0]
+        phi = b.download(0)
+        stj2 = b.solve_mixed(tol=1e-8, coef_format=COEF_BF16, precond=PRECOND_JACOBI)[0]
+    tr = host_true_residual(ps, phi)
+    print(f"  jacobi inner={stj['n_iter']} | zline inner={st['n_iter']} refine={st['n_refine']} "
+          f"res={st['residual']:.2e} host_res={tr:.2e}", flush=True)
+    assert st["status"] == "converged" and tr < 1.2e-8
+    assert st["n_iter"] < 0.9 * stj["n_iter"]
+    # switching back restores the Jacobi run bit for bit
+    assert stj2["n_iter"] == stj["n_iter"] and stj2["residual"] == stj["residual"]
+
+
+def test_zline_batch_ragged_with_failures():
+    """Uneven meshes (every nz a multiple of 4) in one launch sequence; a
+    zero-RHS system returns at once and a NaN system never converges without
+    stalling the others; a second solve repeats bit for bit."""
+    syss = [systems.make_system(1, 0), systems.make_system(2, 0, (20, 36, 28)),
+            systems.make_system(3, 0, (26, 54, 28)), systems.make_system(1, 1)]
+    z = systems.make_system(1, 2)
+    z.Su[...] = 0.0
+    bad = systems.make_system(1, 3)
+    bad.Su[6, 7, 8] = np.nan
+    syss += [z, bad]
+    with eafp.Batch.from_systems(syss) as b:
+        st = b.solve_mixed(tol=1e-8, max_steps=100, coef_format=COEF_BF16, precond=PRECOND_ZLINE)
+        for s, ps in enumerate(syss):
+            print(f"case zline batch system={s} shape={ps.shape} -> {st[s]}", flush=True)
+            phi = b.download(s)
+            if s == 4:
+                assert st[s]["status"] == "zero_rhs"
+                continue
+            if s == 5:
+                assert st[s]["status"] == "max_iter" and math.isnan(st[s]["residual"])
+                continue
+            check_answer(ps, phi, st[s], ("zb", s))
+        st2 = b.solve_mixed(tol=1e-8, max_steps=100, coef_format=COEF_BF16, precond=PRECOND_ZLINE)
+        for a1, a2 in zip(st, st2):
+            assert a1["n_iter"] == a2["n_iter"]
+            assert (a1["residual"] == a2["residual"]) or (math.isnan(a1["residual"]) and math.isnan(a2["residual"]))
+
+
+def test_zline_refused_when_ineligible():
+    """nz not a multiple of 4: EAFP_ERR_UNSUPPORTED before any solve work."""
+    ps = systems.make_system(3, 0)   # 26x54x28 with the generator's nz
+    ps2 = systems.make_system(3, 1)  # nz = 40
+    odd = systems.make_system(1, 0, (12, 24, 18))
+    with eafp.Batch.from_systems([ps, ps2, odd]) as b:
+        with pytest.raises(eafp.EafpError) as e:
+            b.solve_mixed(tol=1e-8, coef_format=COEF_BF16, precond=PRECOND_ZLINE)
+        assert e.value.rc == EAFP_ERR_UNSUPPORTED
+        st = b.solve_mixed(tol=1e-8, coef_format=COEF_BF16)   # Jacobi still works
+        assert all(s["status"] == "converged" for s in st)
+
+
+def test_zline_pp_step():
+    """The fused p' step with the z-line mixed solver meets the residual target."""
+    from eafp import fields
+    shape = (20, 36, 28)
+    F, cfg = fields.make_fields(*shape, seed=1, masks=True)
+    m = eafp.Mesh(*shape)
+    out, pp, res, st = eafp.pp_step(F, cfg, m, 0.3, solver="mixed", mixed=dict(tol=1e-8, precond=PRECOND_ZLINE))
+    outj, ppj, resj, stj = eafp.pp_step(F, cfg, m, 0.3, solver="mixed", mixed=dict(tol=1e-8))
+    ref, _, _ = oracle.pp_assemble(F, cfg, shape)
+
+    class S:
+        pass
+    ps = S()
+    ps.shape = shape
+    ps.padded = lambda n: ref[n]
+    ps.interior = lambda n: oracle.interior(ref[n])
+    tr = host_true_residual(ps, pp)
+    print(f"case zline pp_step: inner {st['n_iter']} (jacobi {stj['n_iter']}) host_res={tr:.2e}", flush=True)
+    assert st["status"] == "converged" and tr < 1.2e-8
+    assert res == resj

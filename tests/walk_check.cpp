@@ -22,6 +22,7 @@
 // LDS slab indices, the partial slots) and checks that every global access
 // stays inside its system, every LDS index inside its array, every cell is
 // computed exactly once and every partial slot written exactly once.
+// It replays the z-line preconditioned kernels too (check_lines).
 // Prints "ok <cases>" and exits 0, or the first violation and exits 1.
 #include <cstdio>
 #include <cstdlib>
@@ -289,6 +290,66 @@ static void check_columns_generic(std::vector<SysGeom> g, int cpt, int splits) {
     ++g_cases;
 }
 
+// The z-line preconditioned kernels (k_linefac, k_mupdate_line,
+// k_mreplace_line): one line tile per block, thread t of tile L holds the
+// cells off + line0*nz + 4t .. +3 when t/nq < L.nlines (nq = nz/4), the
+// block's first cell otherwise; LDS maps fw/bw/lf indexed by t, folds over
+// li*nq + m inside the thread's own line; k_linefac's thread li walks line
+// line0+li.  Checks every access stays inside its system and array, quads
+// are 16-byte aligned, every cell is covered exactly once by both forms,
+// every partial slot written once, and the ranges the stencils reduce.
+static void check_lines(std::vector<SysGeom> g, int cpt) {
+    layout_systems(g);
+    const int nlt = line_ranges(g);
+    const std::vector<TileDesc> tiles = make_tile_table(g, BLOCK * cpt);
+    const std::vector<LineTile> lt = make_line_table(g);
+    CHECK((int)lt.size() == nlt, "line tiles %zu != %d", lt.size(), nlt);
+    long total = 0;
+    for (const SysGeom& G : g) total = G.off + G.n > total ? G.off + G.n : total;
+    std::vector<int> cover(total, 0), fac_cover(total, 0), slot(nlt, 0);
+    for (const TileDesc& T : tiles) {
+        CHECK(T.lt_begin == g[T.sys].lt_begin && T.lt_end == g[T.sys].lt_end, "tile range of system %d", T.sys);
+    }
+    for (int b = 0; b < nlt; ++b) {
+        const LineTile L = lt[b];
+        const SysGeom& G = g[L.sys];
+        CHECK(b >= G.lt_begin && b < G.lt_end, "tile %d outside its system's range", b);
+        CHECK(L.tile_begin == G.tile_begin && L.tile_end == G.tile_end, "tile range %d", b);
+        CHECK(L.nlines >= 1 && L.nlines <= line_lpb(L.nz), "nlines %d", L.nlines);
+        CHECK(L.line0 + L.nlines <= G.nr * G.nth, "lines past the system");
+        const int nq = line_lanes(L.nz);
+        CHECK(nq >= 1 && nq <= BLOCK, "nq %d", nq);
+        slot[b] += 1;   // store_run<2>(partB, blockIdx.x, 1, 1, .)
+        for (int tid = 0; tid < BLOCK; ++tid) {
+            const int li = tid / nq, j = tid - li * nq;
+            const bool inl = li < L.nlines;
+            const long o = L.off + (long)L.line0 * L.nz + 4 * (long)(inl ? tid : 0);
+            CHECK(o % 4 == 0, "quad alignment");
+            CHECK(o >= G.off && o + 4 <= G.off + G.n, "quad outside system (tile %d tid %d)", b, tid);
+            if (inl) {
+                const long c = o - G.off;
+                CHECK(c / L.nz == L.line0 + li && c % L.nz == 4 * j, "thread's cells not on its line");
+                for (int k = 0; k < 4; ++k) cover[o + k] += 1;
+                for (int m = 0; m < j; ++m) CHECK(li * nq + m < BLOCK, "fw index");
+                for (int m = nq - 1; m > j; --m) CHECK(li * nq + m < BLOCK, "bw index");
+                if (j + 1 < nq) CHECK(tid + 1 < BLOCK, "lf index");
+            }
+            if (tid < L.nlines) {   // k_linefac
+                const long o0 = L.off + (long)(L.line0 + tid) * L.nz;
+                CHECK(o0 >= G.off && o0 + L.nz <= G.off + G.n, "factor line outside system");
+                for (int k = 0; k < L.nz; ++k) fac_cover[o0 + k] += 1;
+            }
+        }
+    }
+    for (const SysGeom& G : g)
+        for (long c = G.off; c < G.off + G.n; ++c) {
+            CHECK(cover[c] == 1, "cell %ld covered %d times", c, cover[c]);
+            CHECK(fac_cover[c] == 1, "cell %ld factored %d times", c, fac_cover[c]);
+        }
+    for (int b = 0; b < nlt; ++b) CHECK(slot[b] == 1, "partial slot %d written %d times", b, slot[b]);
+    ++g_cases;
+}
+
 static SysGeom sys(int nr, int nth, int nz) {
     SysGeom s{};
     s.nr = nr;
@@ -308,6 +369,8 @@ int main() {
         {sys(120, 240, 168)},                                         // the large single system
         {sys(60, 120, 84), sys(3, 1, 4), sys(1, 7, 8), sys(20, 36, 28)},   // nz multiples of 4, ragged
         {sys(26, 54, 29), sys(18, 36, 50), sys(35, 66, 24), sys(9, 12, 256), sys(4, 5, 1)},   // any nz
+        {sys(3, 2, 1024), sys(5, 7, 4), sys(2, 3, 1020), sys(60, 120, 84)},   // z-lines of 1 .. 256 lanes
+        {sys(12, 24, 16), sys(20, 36, 28), sys(16, 30, 24), sys(24, 48, 32)},   // the assembled p' meshes
     };
     const int cpts[] = {1, 2, 4, 8};
     for (auto& bt : batches) {
@@ -326,6 +389,11 @@ int main() {
                 if (G > 16 * 256 && G != rounded) continue;
                 check_shape(bt, cpt, G);
                 if (quad && cpt >= 4) check_quads(bt, cpt, G);
+            }
+            {
+                std::vector<SysGeom> gl = bt;
+                layout_systems(gl);
+                if (line_eligible(gl)) check_lines(bt, cpt);
             }
             for (int splits : {1, 2, 3, 4, 8}) {
                 if (quad) check_columns(bt, cpt, splits);
