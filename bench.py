@@ -175,13 +175,13 @@ def measure_workload(eafp, syss, fmt, coef, interval, label, precond=0):
         precond = 0
     rec = {"workload": label, "systems": len(syss), "cells": ncells, "coef_format": coef,
            "precond": "zline" if precond else "jacobi"}
-    b.solve_mixed(tol=1e-8, interval=interval, coef_format=fmt, max_steps=1, precond=precond)
-    b.autotune(eafp.Batch.SOLVER_MIXED, fmt, reps=3)
+    msolver = eafp.Batch.SOLVER_MIXED_ZLINE if precond else eafp.Batch.SOLVER_MIXED
+    b.autotune(msolver, fmt, reps=3)
     b.solve_mixed(tol=1e-8, interval=interval, coef_format=fmt, precond=precond)
     t = time.perf_counter()
     st = b.solve_mixed(tol=1e-8, interval=interval, coef_format=fmt, precond=precond)
     tm = time.perf_counter() - t
-    ms_sp, ms_up = b.time_kernels(eafp.Batch.SOLVER_MIXED, fmt, reps=10)
+    ms_sp, ms_up = b.time_kernels(msolver, fmt, reps=10)
     rec["mixed"] = {"solves_per_s": round(len(syss) / tm, 3), "seconds": round(tm, 4),
                     "all_converged": all(x["status"] == "converged" for x in st),
                     "inner_iters_max": max(x["n_iter"] for x in st),
@@ -557,18 +557,17 @@ def main():
     batch = eafp.Batch.from_systems(syss)
     log(f"uploaded; launch shape {batch.info()}")
     tune = None
+    msolver = eafp.Batch.SOLVER_MIXED_ZLINE if zline else eafp.Batch.SOLVER_MIXED
     if args.tiling:
         shape = [int(v) for v in args.tiling.split(",")]
         batch.set_tiling(shape[0], shape[1])
         batch.set_columns(shape[2] if len(shape) > 2 else 0)
     elif not args.no_tune:
-        # one short solve first, so the tuner times the preconditioner's kernels
-        batch.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt, max_steps=1, precond=precond)
-        tune = batch.autotune(eafp.Batch.SOLVER_MIXED, fmt, reps=5)
+        tune = batch.autotune(msolver, fmt, reps=5)
         log(f"autotune (mixed {args.coef}): {tune}")
     if args.kernels_only:
         calibration_copy(CALIB_BYTES)
-        ms_sp, ms_up = batch.time_kernels(eafp.Batch.SOLVER_MIXED, fmt, reps=20)
+        ms_sp, ms_up = batch.time_kernels(msolver, fmt, reps=20)
         out = {"kernels_only": True, "launch_shape": batch.info(), "coef_format": args.coef,
                "kernels": [spmv_kernel_name(batch.info(), args.coef), update_kernel_name(batch.info(), zline)],
                "k_mspmv_avg_us": round(ms_sp * 1e3, 2), "k_mupdate_avg_us": round(ms_up * 1e3, 2),
@@ -599,7 +598,7 @@ def main():
 
     # the two inner-iteration kernels, timed with HIP events on the library
     # stream; the roofline entry is the one with the longer launch
-    ms_sp, ms_up = batch.time_kernels(eafp.Batch.SOLVER_MIXED, fmt, reps=20)
+    ms_sp, ms_up = batch.time_kernels(msolver, fmt, reps=20)
     info = batch.info()
     legs = [("k_mspmv", spmv_kernel_name(info, args.coef), ms_sp, BYTES_MSPMV[args.coef] * ncells),
             ("k_mupdate", update_kernel_name(info, zline), ms_up, BYTES_MUPDATE * ncells)]
@@ -656,8 +655,7 @@ def main():
             eafp.cg_3d_mpi(*arrs, phi_h, mesh, 100000, 1e-8)
         t_drop = (time.perf_counter() - t) / reps
         eafp.release()
-        single.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt, max_steps=1, precond=precond)
-        single.autotune(eafp.Batch.SOLVER_MIXED, fmt, reps=5)
+        single.autotune(msolver, fmt, reps=5)
         single.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt, precond=precond)
         t = time.perf_counter()
         for _ in range(reps):

@@ -14,11 +14,15 @@
 namespace eafp {
 
 int time_fp64_kernels(Batch& B, int reps, float* ms_spmv, float* ms_update);
-int time_mixed_kernels(Batch& B, int fmt, int reps, float* ms_spmv, float* ms_update);
+int time_mixed_kernels(Batch& B, int fmt, int precond, int reps, float* ms_spmv, float* ms_update);
 
+// solver 0: FP64 CG; 1: mixed with the Jacobi inner preconditioner; 2:
+// mixed with the z-line inner preconditioner.
 int time_kernels(Batch& B, int solver, int fmt, int reps, float* ms_spmv, float* ms_update) {
     if (solver == 0) return time_fp64_kernels(B, reps, ms_spmv, ms_update);
-    if (solver == 1) return time_mixed_kernels(B, fmt, reps, ms_spmv, ms_update);
+    if (solver == 1 || solver == 2)
+        return time_mixed_kernels(B, fmt, solver == 2 ? EAFP_PRECOND_ZLINE : EAFP_PRECOND_JACOBI, reps, ms_spmv,
+                                  ms_update);
     return EAFP_ERR_ARG;
 }
 
@@ -48,7 +52,7 @@ int autotune(Batch& B, int solver, int fmt, int reps, int* best_cpt, int* best_b
     }
     EAFP_TRY_RC(B.set_columns(0));
     EAFP_TRY_RC(B.set_tiling(bc, bb));
-    if (solver == 1) {
+    if (solver == 1 || solver == 2) {
         // the column-marching stencil, where every system qualifies
         int bs = 0;
         for (int k : {1, 2, 4, 8, 16}) {

@@ -1791,15 +1791,17 @@ int solve_mixed(Batch& B, const eafp_mixed_opts& o, eafp_stats* stats) {
 // Average durations of the two inner-iteration kernels over `reps`
 // iterations (systems kept running: the tolerance is out of reach), each
 // launch bracketed by HIP events on the library stream.
-int time_mixed_kernels(Batch& B, int fmt, int reps, float* ms_spmv, float* ms_update) {
+int time_mixed_kernels(Batch& B, int fmt, int precond, int reps, float* ms_spmv, float* ms_update) {
     eafp_mixed_opts o{};
     o.tol = 1e-300;
     o.interval = 2;
     o.max_steps = 1 << 20;
     o.coef_format = fmt;
+    o.precond = precond;
     EAFP_TRY_RC(check_mixed(B, o));
     if (reps < 1) return EAFP_ERR_ARG;
     EAFP_TRY_RC(ensure_mixed(B, fmt));
+    B.mixed->args.line = precond == EAFP_PRECOND_ZLINE ? 1 : 0;
     EAFP_TRY_RC(mixed_start(B, o, fmt));
     hipEvent_t e[3];
     for (auto& x : e) EAFP_TRY(hipEventCreate(&x));

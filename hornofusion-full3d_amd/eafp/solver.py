@@ -265,7 +265,7 @@ class Batch:
         check(self._L.eafp_batch_solve_mixed(self._h, ctypes.byref(o), st), "eafp_batch_solve_mixed")
         return [s.as_dict() for s in st]
 
-    SOLVER_FP64, SOLVER_MIXED = 0, 1
+    SOLVER_FP64, SOLVER_MIXED, SOLVER_MIXED_ZLINE = 0, 1, 2   # the last: mixed with the z-line preconditioner
 
     def set_tiling(self, cells_per_thread, blocks=0):
         check(self._L.eafp_batch_set_tiling(self._h, int(cells_per_thread), int(blocks)), "eafp_batch_set_tiling")

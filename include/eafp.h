@@ -307,7 +307,8 @@ int eafp_batch_get_columns(const eafp_batch* b, int* r_splits);
 
 /* Measurement: average duration (ms) of one launch of each of the two
  * iteration kernels of a solver (solver 0: FP64 CG, k64_spmv/k64_update;
- * solver 1: mixed, k_mspmv/k_mupdate with coef_format) over `reps`
+ * solver 1: mixed, k_mspmv/k_mupdate with coef_format; solver 2: mixed
+ * with the z-line inner preconditioner, k_mspmv/k_mupdate_line) over `reps`
  * iterations on the uploaded systems, timed with HIP events on the
  * library's stream.  Leaves the batch's results undefined (solve again).
  * No reference counterpart. */
@@ -315,8 +316,9 @@ int eafp_batch_time_kernels(eafp_batch* b, int solver, int coef_format, int reps
                             float* ms_update);
 
 /* Autotuner: times every tile shape (cells per thread x blocks per CU) for
- * the solver's iteration kernels and keeps the fastest; for the mixed
- * solver it then times the column-marching stencil at 1, 2, 4, 8 and 16
+ * the solver's iteration kernels (solver as for eafp_batch_time_kernels)
+ * and keeps the fastest; for the mixed solver it then times the
+ * column-marching stencil at 1, 2, 4, 8 and 16
  * r-segments and keeps it if it is faster (eafp_batch_get_columns tells).
  * Returns the tile shape and the iteration time (ms).  No reference
  * counterpart. */

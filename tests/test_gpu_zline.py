@@ -156,3 +156,17 @@ def test_zline_pp_step():
     print(f"case zline pp_step: inner {st['n_iter']} (jacobi {stj['n_iter']}) host_res={tr:.2e}", flush=True)
     assert st["status"] == "converged" and tr < 1.2e-8
     assert res == resj
+
+
+def test_zline_autotune_and_kernel_timer():
+    """Solver code 2 of the tuner and the kernel timer runs the z-line kernels;
+    the batch solves correctly afterwards."""
+    ps = systems.make_system(2, 0, (20, 36, 28))
+    with eafp.Batch.from_systems([ps]) as b:
+        tune = b.autotune(eafp.Batch.SOLVER_MIXED_ZLINE, COEF_BF16, reps=2)
+        ms_sp, ms_up = b.time_kernels(eafp.Batch.SOLVER_MIXED_ZLINE, COEF_BF16, reps=3)
+        print(f"case zline autotune {tune} spmv {ms_sp * 1e3:.1f} us update {ms_up * 1e3:.1f} us", flush=True)
+        assert ms_sp > 0 and ms_up > 0
+        st = b.solve_mixed(tol=1e-8, coef_format=COEF_BF16, precond=PRECOND_ZLINE)[0]
+        phi = b.download(0)
+    check_answer(ps, phi, st, (2, (20, 36, 28)))
