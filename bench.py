@@ -296,7 +296,7 @@ def measure_step(eafp, shape=(60, 120, 84), reps=5):
     post-solve steps) on two-phase fields at 60x120x84, device-resident:
     with the reference's defaults (cfg%max_inner_pres = 50 FP64 CG
     iterations, tol 1e-8), and solved to 1e-8 by the FP64 CG and by the
-    mixed solver (BF16).  Beside it the same 50-iteration step from host
+    mixed solver (BF16; Jacobi and z-line inner preconditioners).  Beside it the same 50-iteration step from host
     arrays, fused and as the three unfused calls, and the oracle's three
     steps on one host core."""
     import ctypes
@@ -330,12 +330,14 @@ def measure_step(eafp, shape=(60, 120, 84), reps=5):
     pp_d = dev(nbytes=8 * npad)
     L = _lib.load()
     d = eafp.solver.desc_of(m)
-    mixed = _lib.MixedOpts(1e-8, 32, 2000, _lib.COEF_BF16, 0)
+    mixed = _lib.MixedOpts(1e-8, 32, 2000, _lib.COEF_BF16, 0, _lib.PRECOND_JACOBI)
+    mixed_zl = _lib.MixedOpts(1e-8, 32, 2000, _lib.COEF_BF16, 0, _lib.PRECOND_ZLINE)
     rec = {"workload": "one p' step on two-phase 60x120x84 fields: assembly, solve from pp = 0, fluxes, velocities, p",
            "device_resident": {}}
     for label, o in (("fp64_50_iterations", _lib.PPStepOpts(_lib.STEP_FP64, 50, 1e-8, 0.3, mixed)),
                      ("fp64_to_1e-8", _lib.PPStepOpts(_lib.STEP_FP64, 100000, 1e-8, 0.3, mixed)),
-                     ("mixed_bf16_to_1e-8", _lib.PPStepOpts(_lib.STEP_MIXED, 0, 0.0, 0.3, mixed))):
+                     ("mixed_bf16_to_1e-8", _lib.PPStepOpts(_lib.STEP_MIXED, 0, 0.0, 0.3, mixed)),
+                     ("mixed_bf16_zline_to_1e-8", _lib.PPStepOpts(_lib.STEP_MIXED, 0, 0.0, 0.3, mixed_zl))):
         res = ctypes.c_double()
         st = _lib.Stats()
         call = lambda: _lib.check(L.eafp_pp_step(ctypes.byref(d), ctypes.byref(f), ctypes.byref(c), ctypes.byref(o),
