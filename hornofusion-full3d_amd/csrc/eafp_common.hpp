@@ -105,11 +105,12 @@ inline std::vector<TileDesc> make_tile_table(std::vector<SysGeom>& g, int tile) 
 // ---------------------------------------------------------------------------
 // Line tiles, the work unit of the z-line preconditioned kernels of the
 // mixed solver.  Line l = i*nth + j of a system is its z-row, the nz
-// consecutive cells l*nz .. l*nz + nz-1.  A block takes lpb = BLOCK/(nz/4)
-// whole lines of one system, nz/4 lanes per line with one quad each, so
-// thread t holds the cells line0*nz + 4t .. +3: the block's lines are one
-// contiguous run of cells, read with 16-byte accesses.  Needs every nz a
-// multiple of 4 and at most 4*BLOCK.
+// consecutive cells l*nz .. l*nz + nz-1.  A block takes lpb = BLOCK/nq whole
+// lines of one system, nq = ceil(nz/4) lanes per line; lane j of line li
+// holds the line's cells 4j .. 4j+3 (those below nz), so the block's lines
+// are one contiguous run of cells.  When every nz of the batch is a multiple
+// of 4, thread t holds the cells line0*nz + 4t .. +3 and reads them with
+// 16-byte accesses.  Needs every nz at most 4*BLOCK.
 // ---------------------------------------------------------------------------
 struct LineTile {
     int sys;
@@ -120,12 +121,12 @@ struct LineTile {
     long off;                   // first cell of the system in the arenas
 };
 
-__host__ __device__ __forceinline__ int line_lanes(int nz) { return nz / 4; }
-__host__ __device__ __forceinline__ int line_lpb(int nz) { return BLOCK / (nz / 4); }
+__host__ __device__ __forceinline__ int line_lanes(int nz) { return (nz + 3) / 4; }
+__host__ __device__ __forceinline__ int line_lpb(int nz) { return BLOCK / line_lanes(nz); }
 
 inline bool line_eligible(const std::vector<SysGeom>& g) {
     for (const SysGeom& s : g)
-        if (s.nz < 4 || s.nz % 4 != 0 || s.nz > 4 * BLOCK) return false;
+        if (s.nz < 1 || s.nz > 4 * BLOCK) return false;
     return true;
 }
 

@@ -1282,25 +1282,97 @@ __device__ __forceinline__ void line_solve(LineLds& S, bool inl, int li, int j, 
     for (int k = 2; k >= 0; --k) z[k] = u[k] - l[k + 1] * z[k + 1];
 }
 
+// The cells of a line-tile thread.  Quad form (every nz of the batch a
+// multiple of 4): cells o .. o+3, one 16-byte access each.  Generic form:
+// the block's lines are one run of n = nlines*nz cells starting at
+// `run`; the block moves every array through LDS (stage/unstage: lane-
+// consecutive, coalesced accesses), and the thread takes its cells
+// li*nz + 4j .. +nv-1 of the run from there (nv < 4 only in a line's last
+// lane; slots past nv repeat its first cell).  Threads past the tile's
+// lines take the run's first cell.
+constexpr int LINE_RUN = 4 * BLOCK;   // cells of a line tile at most
+
+template <bool Q>
+struct LineCells {
+    long o;      // quad form: first cell
+    long run;    // generic form: first cell of the block's run
+    int n;       // generic form: cells in the run
+    int c0;      // generic form: the thread's first cell within the run
+    int nv;
+    bool inl;
+    int li, j, nq;
+    __device__ LineCells(const LineTile& L) {
+        const int tid = threadIdx.x;
+        nq = line_lanes(L.nz);
+        li = tid / nq;
+        j = tid - li * nq;
+        inl = li < L.nlines;
+        run = L.off + (long)L.line0 * L.nz;
+        n = L.nlines * L.nz;
+        if (Q) {
+            o = run + 4 * (long)(inl ? tid : 0);
+            nv = 4;
+            c0 = 0;
+        } else {
+            o = run;
+            c0 = inl ? li * L.nz + 4 * j : 0;
+            nv = inl ? min(4, L.nz - 4 * j) : 1;
+        }
+    }
+    // generic form: the run of p into buf (the caller syncs before reading)
+    __device__ void stage(const float* p, float* buf) const {
+        for (int c = threadIdx.x; c < n; c += BLOCK) buf[c] = __builtin_nontemporal_load(p + run + c);
+    }
+    __device__ F4 from(const float* buf) const {
+        F4 r;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r.v[k] = buf[c0 + (k < nv ? k : 0)];
+        return r;
+    }
+    __device__ void to(float* buf, const F4& v) const {
+        if (!inl) return;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k < nv) buf[c0 + k] = v.v[k];
+    }
+    // generic form: buf back to the run of p (the caller syncs before)
+    __device__ void unstage(float* p, const float* buf, bool nt) const {
+        for (int c = threadIdx.x; c < n; c += BLOCK) {
+            if (nt) __builtin_nontemporal_store(buf[c], p + run + c);
+            else p[run + c] = buf[c];
+        }
+    }
+};
+
 // k_mupdate with the z-line preconditioner: one line tile per block.
+template <bool Q>
 __global__ __launch_bounds__(BLOCK) void k_mupdate_line(MxArgs a, int par) {
     __shared__ double sh[2 * (BLOCK / 64)];
     __shared__ LineLds S;
+    __shared__ float G[Q ? 1 : 5][Q ? 1 : LINE_RUN];   // generic form: d, p, r, q, factors of the run
     const LineTile L = a.lt[blockIdx.x];
     const int s = L.sys;
-    const int tid = threadIdx.x;
-    const int nq = line_lanes(L.nz);
-    const int li = tid / nq, j = tid - li * nq;
-    const bool inl = li < L.nlines;
-    const long o = L.off + (long)L.line0 * L.nz + 4 * (long)(inl ? tid : 0);
-    const F4 dv = ld4nt(a.d + o), pvv = ld4nt(a.p[par ^ 1] + o), rv = ld4nt(a.r + o), qv = ld4nt(a.q + o);
-    const F4 fv = ld4nt((const float*)a.fac + o);
+    const LineCells<Q> C(L);
+    F4 dv, pvv, rv, qv, fv;
+    if (Q) {
+        dv = ld4nt(a.d + C.o);
+        pvv = ld4nt(a.p[par ^ 1] + C.o);
+        rv = ld4nt(a.r + C.o);
+        qv = ld4nt(a.q + C.o);
+        fv = ld4nt((const float*)a.fac + C.o);
+    } else {
+        C.stage(a.d, G[0]);
+        C.stage(a.p[par ^ 1], G[1]);
+        C.stage(a.r, G[2]);
+        C.stage(a.q, G[3]);
+        C.stage((const float*)a.fac, G[4]);
+    }
     const MxDyn P = a.dyn[0][s];
     MxDyn N = P;
     bool skip = P.done || P.broken;
     if (!skip) {
         double tot[1];
-        reduce_tiles<1>(a.partA, L.tile_begin, L.tile_end, tot, sh);
+        reduce_tiles<1>(a.partA, L.tile_begin, L.tile_end, tot, sh);   // syncs the block
         if (!(fabs(tot[0]) > SMALL)) {
             skip = true;
             N.broken = 1;
@@ -1309,43 +1381,66 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate_line(MxArgs a, int par) {
             N.total_inner = P.total_inner + 1;
         }
     }
-    if ((int)blockIdx.x == L.lt_begin && tid == 0) store_state(&a.dyn[1][s], N);
+    if ((int)blockIdx.x == L.lt_begin && threadIdx.x == 0) store_state(&a.dyn[1][s], N);
     if (skip) return;   // uniform over the block
+    if (!Q) {
+        dv = C.from(G[0]);
+        pvv = C.from(G[1]);
+        rv = C.from(G[2]);
+        qv = C.from(G[3]);
+        fv = C.from(G[4]);
+    }
     const float fa = (float)N.alpha;
     float rn[4], l[4], iv[4], zn[4];
     F4 dn, rr, zz;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        rn[k] = rv.v[k] - fa * qv.v[k];
+        const bool v = k < C.nv;
+        rn[k] = v ? rv.v[k] - fa * qv.v[k] : 0.f;
         dn.v[k] = dv.v[k] + fa * pvv.v[k];
         unpack_fac(fv.v[k], l[k], iv[k]);
+        l[k] = v ? l[k] : 0.f;
+        iv[k] = v ? iv[k] : 0.f;
     }
-    line_solve(S, inl, li, j, nq, rn, l, iv, zn);
+    line_solve(S, C.inl, C.li, C.j, C.nq, rn, l, iv, zn);
     double acc[2] = {0.0, 0.0};
-    if (inl) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            rr.v[k] = rn[k];
-            zz.v[k] = zn[k];
+    for (int k = 0; k < 4; ++k) {
+        rr.v[k] = rn[k];
+        zz.v[k] = zn[k];
+        if (C.inl) {
             acc[0] += (double)rn[k] * (double)rn[k];
             acc[1] += (double)rn[k] * (double)zn[k];
         }
-        st4nt(a.d + o, dn);
-        st4nt(a.r + o, rr);
-        st4(a.z + o, zz);
+    }
+    if (Q) {
+        if (C.inl) {
+            st4nt(a.d + C.o, dn);
+            st4nt(a.r + C.o, rr);
+            st4(a.z + C.o, zz);
+        }
+    } else {
+        // every thread has read its cells of G (line_solve synced since): reuse G[0..2]
+        C.to(G[0], dn);
+        C.to(G[1], rr);
+        C.to(G[2], zz);
+        __syncthreads();
+        C.unstage(a.d, G[0], true);
+        C.unstage(a.r, G[1], true);
+        C.unstage(a.z, G[2], false);
     }
     block_sum<2>(acc, sh);
     store_run<2>(a.partB, (int)blockIdx.x, 1, 1, acc);
 }
 
 // k_mreplace with the z-line preconditioner: one line tile per block.
+template <bool Q>
 __global__ __launch_bounds__(BLOCK) void k_mreplace_line(MxArgs a) {
     __shared__ double sh[NRES * (BLOCK / 64)];
     __shared__ LineLds S;
     const LineTile L = a.lt[blockIdx.x];
     const int s = L.sys;
-    const int tid = threadIdx.x;
-    const bool writer = (int)blockIdx.x == L.lt_begin && tid == 0;
+    const bool writer = (int)blockIdx.x == L.lt_begin && threadIdx.x == 0;
     const MxStat P = a.stat[0][s];
     bool done = P.done != 0;
     bool zero_p = false;
@@ -1399,25 +1494,34 @@ __global__ __launch_bounds__(BLOCK) void k_mreplace_line(MxArgs a) {
         store_state(&a.dyn[1][s], D0);
     }
     if (done) return;   // uniform over the block
-    const int nq = line_lanes(L.nz);
-    const int li = tid / nq, j = tid - li * nq;
-    const bool inl = li < L.nlines;
-    const long o = L.off + (long)L.line0 * L.nz + 4 * (long)(inl ? tid : 0);
-    const F4 fv = ld4nt((const float*)a.fac + o);
+    // once per refinement step: the generic form reads and writes its cells directly
+    const LineCells<Q> C(L);
+    const long oc0 = Q ? C.o : C.run + C.c0;
+    F4 fv;
+    if (Q) {
+        fv = ld4nt((const float*)a.fac + oc0);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) fv.v[k] = ((const float*)a.fac)[oc0 + (k < C.nv ? k : 0)];
+    }
     float rr[4], l[4], iv[4], zn[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const int slot = a.comp[o + k];
-        const double rv = __builtin_nontemporal_load(a.r64 + o + k);
+        const bool v = k < C.nv;
+        const long oc = oc0 + (v ? k : 0);
+        const int slot = a.comp[oc];
+        const double rv = __builtin_nontemporal_load(a.r64 + oc);
         double mu = 0.0;
 #pragma unroll
         for (int q = 0; q < MAX_COMP; ++q) mu = slot == q ? mean_r[q] : mu;
-        rr[k] = (float)(rv - mu);
+        rr[k] = v ? (float)(rv - mu) : 0.f;
         unpack_fac(fv.v[k], l[k], iv[k]);
+        l[k] = v ? l[k] : 0.f;
+        iv[k] = v ? iv[k] : 0.f;
     }
-    line_solve(S, inl, li, j, nq, rr, l, iv, zn);
+    line_solve(S, C.inl, C.li, C.j, C.nq, rr, l, iv, zn);
     double acc[2] = {0.0, 0.0};
-    if (inl) {
+    if (C.inl) {
         F4 rv4, zv4, zero;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -1427,10 +1531,21 @@ __global__ __launch_bounds__(BLOCK) void k_mreplace_line(MxArgs a) {
             acc[0] += (double)rr[k] * (double)rr[k];
             acc[1] += (double)rr[k] * (double)zn[k];
         }
-        st4nt(a.r + o, rv4);
-        st4(a.z + o, zv4);
-        st4nt(a.d + o, zero);
-        if (zero_p) st4(a.p[0] + o, zero);
+        if (Q) {
+            st4nt(a.r + oc0, rv4);
+            st4(a.z + oc0, zv4);
+            st4nt(a.d + oc0, zero);
+            if (zero_p) st4(a.p[0] + oc0, zero);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (k < C.nv) {
+                    __builtin_nontemporal_store(rr[k], a.r + oc0 + k);
+                    a.z[oc0 + k] = zn[k];
+                    __builtin_nontemporal_store(0.0f, a.d + oc0 + k);
+                    if (zero_p) a.p[0][oc0 + k] = 0.0f;
+                }
+        }
     }
     block_sum<2>(acc, sh);
     store_run<2>(a.partB, (int)blockIdx.x, 1, 1, acc);
@@ -1631,7 +1746,8 @@ static void launch_mspmv(Batch& B, int fmt, int par) {
 
 static void launch_mupdate(Batch& B, int par) {
     if (B.mixed->args.line) {
-        k_mupdate_line<<<B.nltiles, BLOCK, 0, B.stream>>>(B.mixed->args, par);
+        if (B.quad) k_mupdate_line<true><<<B.nltiles, BLOCK, 0, B.stream>>>(B.mixed->args, par);
+        else k_mupdate_line<false><<<B.nltiles, BLOCK, 0, B.stream>>>(B.mixed->args, par);
         return;
     }
     if (B.quad && B.cpt >= 4) {
@@ -1651,7 +1767,8 @@ static void launch_refine(Batch& B) {
     const int tile = BLOCK * B.cpt;
     k_res<<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, tile);
     k_rsum<<<B.nsys, BLOCK, 0, B.stream>>>(B.mixed->args);
-    if (B.mixed->args.line) k_mreplace_line<<<B.nltiles, BLOCK, 0, B.stream>>>(B.mixed->args);
+    if (B.mixed->args.line && B.quad) k_mreplace_line<true><<<B.nltiles, BLOCK, 0, B.stream>>>(B.mixed->args);
+    else if (B.mixed->args.line) k_mreplace_line<false><<<B.nltiles, BLOCK, 0, B.stream>>>(B.mixed->args);
     else k_mreplace<<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, tile);
 }
 

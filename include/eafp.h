@@ -86,8 +86,8 @@ typedef struct eafp_mixed_opts {
  * rounded operator (what cg_3d_mpi uses).  ZLINE: its tridiagonal part along
  * every z-line, M = L D L^T (l in FP16, 1/d in BF16: 4 bytes per cell, as
  * Jacobi's 1/D), applied by two scans per line; fewer inner iterations on
- * the furnace's p' systems.  ZLINE needs every nz a multiple of 4 and at
- * most 1024 (EAFP_ERR_UNSUPPORTED otherwise). */
+ * the furnace's p' systems.  ZLINE needs every nz at most 1024
+ * (EAFP_ERR_UNSUPPORTED otherwise). */
 #define EAFP_PRECOND_JACOBI 0
 #define EAFP_PRECOND_ZLINE 1
 

@@ -291,14 +291,17 @@ static void check_columns_generic(std::vector<SysGeom> g, int cpt, int splits) {
 }
 
 // The z-line preconditioned kernels (k_linefac, k_mupdate_line,
-// k_mreplace_line): one line tile per block, thread t of tile L holds the
-// cells off + line0*nz + 4t .. +3 when t/nq < L.nlines (nq = nz/4), the
-// block's first cell otherwise; LDS maps fw/bw/lf indexed by t, folds over
-// li*nq + m inside the thread's own line; k_linefac's thread li walks line
-// line0+li.  Checks every access stays inside its system and array, quads
-// are 16-byte aligned, every cell is covered exactly once by both forms,
-// every partial slot written once, and the ranges the stencils reduce.
-static void check_lines(std::vector<SysGeom> g, int cpt) {
+// k_mreplace_line): one line tile per block; thread t of tile L is lane
+// j = t%nq of line li = t/nq (nq = ceil(nz/4)) and holds the line's cells
+// 4j .. 4j+nv-1 (LineCells: the quad form, every nz a multiple of 4, reads
+// off + line0*nz + 4t .. +3 with 16-byte accesses; the generic form reads
+// each cell, slots past nv at the lane's first cell); threads past the
+// tile's lines take the tile's first cell.  LDS maps fw/bw/lf indexed by t,
+// folds over li*nq + m inside the thread's own line; k_linefac's thread li
+// walks line line0+li.  Checks every access stays inside its system, quads
+// are 16-byte aligned, every cell is covered exactly once, every partial
+// slot written once, and the ranges the stencils reduce.
+static void check_lines(std::vector<SysGeom> g, int cpt, bool quad) {
     layout_systems(g);
     const int nlt = line_ranges(g);
     const std::vector<TileDesc> tiles = make_tile_table(g, BLOCK * cpt);
@@ -323,13 +326,37 @@ static void check_lines(std::vector<SysGeom> g, int cpt) {
         for (int tid = 0; tid < BLOCK; ++tid) {
             const int li = tid / nq, j = tid - li * nq;
             const bool inl = li < L.nlines;
-            const long o = L.off + (long)L.line0 * L.nz + 4 * (long)(inl ? tid : 0);
-            CHECK(o % 4 == 0, "quad alignment");
-            CHECK(o >= G.off && o + 4 <= G.off + G.n, "quad outside system (tile %d tid %d)", b, tid);
+            long o;
+            int nv;
+            if (quad) {
+                o = L.off + (long)L.line0 * L.nz + 4 * (long)(inl ? tid : 0);
+                nv = 4;
+                CHECK(o % 4 == 0, "quad alignment");
+                CHECK(o >= G.off && o + 4 <= G.off + G.n, "quad outside system (tile %d tid %d)", b, tid);
+            } else {
+                // the update stages the block's run of cells through LDS (buffers of 4*BLOCK)
+                const long run = L.off + (long)L.line0 * L.nz;
+                const int n = L.nlines * L.nz;
+                CHECK(n <= 4 * BLOCK, "run of %d cells exceeds the LDS buffer", n);
+                CHECK(run >= G.off && run + n <= G.off + G.n, "run outside system (tile %d)", b);
+                const int c0 = inl ? li * L.nz + 4 * j : 0;
+                CHECK(c0 >= 0 && c0 < n, "LDS index %d of %d", c0, n);
+                o = L.off + (long)(L.line0 + (inl ? li : 0)) * L.nz + 4 * (inl ? j : 0);
+                CHECK(o == run + c0, "direct and staged cell differ");
+                nv = inl ? (L.nz - 4 * j < 4 ? L.nz - 4 * j : 4) : 1;
+                CHECK(nv >= 1, "empty lane");
+                for (int k = 0; k < 4; ++k) {
+                    const long a = o + (k < nv ? k : 0);
+                    CHECK(a >= G.off && a < G.off + G.n, "cell outside system (tile %d tid %d)", b, tid);
+                }
+                if (inl && j + 1 < nq) CHECK(nv == 4, "partial lane before the line's last");
+                if (inl) CHECK(c0 + nv <= n, "LDS cells past the run");
+            }
             if (inl) {
                 const long c = o - G.off;
                 CHECK(c / L.nz == L.line0 + li && c % L.nz == 4 * j, "thread's cells not on its line");
-                for (int k = 0; k < 4; ++k) cover[o + k] += 1;
+                CHECK(4 * j + nv <= L.nz, "cells past the line");
+                for (int k = 0; k < nv; ++k) cover[o + k] += 1;
                 for (int m = 0; m < j; ++m) CHECK(li * nq + m < BLOCK, "fw index");
                 for (int m = nq - 1; m > j; --m) CHECK(li * nq + m < BLOCK, "bw index");
                 if (j + 1 < nq) CHECK(tid + 1 < BLOCK, "lf index");
@@ -370,6 +397,7 @@ int main() {
         {sys(60, 120, 84), sys(3, 1, 4), sys(1, 7, 8), sys(20, 36, 28)},   // nz multiples of 4, ragged
         {sys(26, 54, 29), sys(18, 36, 50), sys(35, 66, 24), sys(9, 12, 256), sys(4, 5, 1)},   // any nz
         {sys(3, 2, 1024), sys(5, 7, 4), sys(2, 3, 1020), sys(60, 120, 84)},   // z-lines of 1 .. 256 lanes
+        {sys(3, 2, 1023), sys(4, 9, 1), sys(2, 5, 2), sys(3, 3, 3), sys(7, 11, 1021)},   // partial last lanes
         {sys(12, 24, 16), sys(20, 36, 28), sys(16, 30, 24), sys(24, 48, 32)},   // the assembled p' meshes
     };
     const int cpts[] = {1, 2, 4, 8};
@@ -393,7 +421,10 @@ int main() {
             {
                 std::vector<SysGeom> gl = bt;
                 layout_systems(gl);
-                if (line_eligible(gl)) check_lines(bt, cpt);
+                if (line_eligible(gl)) {
+                    if (quad) check_lines(bt, cpt, true);
+                    check_lines(bt, cpt, false);
+                }
             }
             for (int splits : {1, 2, 3, 4, 8}) {
                 if (quad) check_columns(bt, cpt, splits);
