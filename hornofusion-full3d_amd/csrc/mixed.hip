@@ -1250,12 +1250,24 @@ __device__ __forceinline__ void line_solve(LineLds& S, bool inl, int li, int j, 
     S.fw[tid] = make_float2(A, Bv);
     S.lf[tid] = l[0];
     __syncthreads();
+    // the folds read 8 maps ahead of the multiply-add chain (same order of
+    // operations as one map at a time)
     float y = 0.f;
-    if (inl)
-        for (int m = 0; m < j; ++m) {
-            const float2 f = S.fw[li * nq + m];
-            y = f.x * y + f.y;
+    if (inl) {
+        const float2* f = &S.fw[li * nq];
+        int m = 0;
+        for (; m + 8 <= j; m += 8) {
+            float2 t[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) t[u] = f[m + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) y = t[u].x * y + t[u].y;
         }
+        for (; m < j; ++m) {
+            const float2 t = f[m];
+            y = t.x * y + t.y;
+        }
+    }
     float u[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -1272,11 +1284,21 @@ __device__ __forceinline__ void line_solve(LineLds& S, bool inl, int li, int j, 
     S.bw[tid] = make_float2(Ab, Bb);
     __syncthreads();
     float zin = 0.f;
-    if (inl)
-        for (int m = nq - 1; m > j; --m) {
-            const float2 f = S.bw[li * nq + m];
-            zin = f.x * zin + f.y;
+    if (inl) {
+        const float2* f = &S.bw[li * nq];
+        int m = nq - 1;
+        for (; m - 8 >= j; m -= 8) {
+            float2 t[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) t[u] = f[m - u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) zin = t[u].x * zin + t[u].y;
         }
+        for (; m > j; --m) {
+            const float2 t = f[m];
+            zin = t.x * zin + t.y;
+        }
+    }
     z[3] = u[3] - ln * zin;
 #pragma unroll
     for (int k = 2; k >= 0; --k) z[k] = u[k] - l[k + 1] * z[k + 1];
