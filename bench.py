@@ -634,6 +634,20 @@ def main():
                            "inner_iters_max": int(max(inner)), "refine_steps_mean": float(np.mean(refine)),
                            "max_true_rel_residual": float(max(res)), "algorithmic_GBps": round(solve_gbs, 1),
                            "autotune": tune}}
+    if zline:
+        # the same batch with the Jacobi inner preconditioner, timed the same way (not retuned)
+        batch.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt)
+        device_sync()
+        t = time.perf_counter()
+        for k in range(args.steps):
+            stj = batch.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt)
+        device_sync()
+        tj = (time.perf_counter() - t) / args.steps
+        out["mixed_batch"]["jacobi_inner"] = {
+            "solves_per_s": round(B / tj, 3), "inner_iters_mean": float(np.mean([x["n_iter"] for x in stj])),
+            "all_converged": all(x["status"] == "converged" for x in stj),
+            "max_true_rel_residual": float(max(x["residual"] for x in stj))}
+        log(f"jacobi inner preconditioner: {B / tj:.2f} solves/s")
 
     if not args.no_reference:
         # the plain FP64 GPU CG (drop-in algorithm) on the single production system
