@@ -6,7 +6,7 @@ import pytest
 
 import oracle
 from eafp import systems
-from fortran_io import run_driver
+from fortran_io import STEP_OUT, run_driver, run_step_driver
 
 pytestmark = pytest.mark.gpu
 
@@ -56,3 +56,27 @@ def test_fortran_tdma_refuses_parallel_mesh():
     coef["aP"] = np.full(pad, 7.0, order="F")
     status, _ = run_tdma_driver(coef, np.zeros(pad, order="F"), shape, 1, is_parallel=True)
     assert status is None
+
+
+@pytest.mark.parametrize("shape,kw,solver,precond", [((20, 36, 28), dict(seed=1, masks=True), 0, 0),
+                                                     ((12, 24, 16), dict(seed=0), 0, 0),
+                                                     ((16, 30, 24), dict(seed=2, two_phase=False), 1, 1),
+                                                     ((60, 120, 84), dict(seed=5, masks=True), 0, 0)])
+def test_fortran_pressure_step(shape, kw, solver, precond):
+    """eafp_pressure_step from Fortran, velocities and p updated in place,
+    bit for bit the C-ABI step (eafp_pp_step through ctypes) on the same
+    fields; the FP64 form is itself bit-identical to the three separate
+    entry points (tests/test_gpu_step.py)."""
+    import eafp
+    from eafp import fields
+    print(f"case fortran pressure step shape={shape} {kw} solver={solver} precond={precond}", flush=True)
+    F, cfg = fields.make_fields(*shape, **kw)
+    st, nit, res, cg_res, out = run_step_driver(F, cfg, shape, 0.3, 50, 1e-8, solver=solver, precond=precond)
+    assert st == 0
+    ref, pp, res_ref, sref = eafp.pp_step(F, cfg, eafp.Mesh(*shape), 0.3, solver="mixed" if solver else "fp64",
+                                          max_iter=50, tol=1e-8,
+                                          mixed=dict(tol=1e-8, interval=32, max_steps=2000, precond=precond))
+    assert res == res_ref and cg_res == sref["residual"] and nit == sref["n_iter"]
+    np.testing.assert_array_equal(out["pp"], pp)
+    for n in STEP_OUT[1:]:
+        np.testing.assert_array_equal(out[n], ref[n], err_msg=n)
