@@ -48,14 +48,6 @@ int Batch::set_tiling(int cells_per_thread, int blocks) {
     ntiles = (int)tiles_h.size();
     cpt = cells_per_thread;
     EAFP_TRY_RC(tiles.alloc(ntiles));
-    EAFP_TRY(hipMemcpy(tiles.p, tiles_h.data(), sizeof(TileDesc) * ntiles, hipMemcpyHostToDevice));
-    if (nltiles > 0) {   // the line tiles carry the tile ranges: rebuild them too
-        const std::vector<LineTile> lt = make_line_table(geom);
-        EAFP_TRY_RC(ltiles.alloc(lt.size()));
-        EAFP_TRY(hipMemcpy(ltiles.p, lt.data(), sizeof(LineTile) * lt.size(), hipMemcpyHostToDevice));
-    }
-    if (geom_d.p)
-        EAFP_TRY(hipMemcpy(geom_d.p, geom.data(), sizeof(SysGeom) * nsys, hipMemcpyHostToDevice));
     if (blocks > 0) {
         grid = std::max(NXCD, (blocks + NXCD - 1) / NXCD * NXCD);
     } else {
@@ -63,9 +55,11 @@ int Batch::set_tiling(int cells_per_thread, int blocks) {
     }
     ++tiling_gen;
     // the column table refers to the tile ranges: rebuild it (or drop it
-    // when the new tiles leave a system fewer slots than segments)
-    if (col_split > 0 && set_columns(col_split) != EAFP_OK) col_split = 0;
-    return EAFP_OK;
+    // when the new tiles leave a system fewer slots than segments); the
+    // tables go up once the stencil form is settled
+    if (col_split > 0 && set_columns(col_split) == EAFP_OK) return EAFP_OK;
+    col_split = 0;
+    return upload_tables();
 }
 
 // Column-marching stencil for the mixed solver: r_splits > 0 cuts every
@@ -77,7 +71,7 @@ int Batch::set_columns(int r_splits) {
     if (r_splits == 0) {
         col_split = 0;
         ++tiling_gen;
-        return EAFP_OK;
+        return upload_tables();
     }
     std::vector<ColDesc> t;
     if (!make_col_table(geom, r_splits, t)) return EAFP_ERR_UNSUPPORTED;
@@ -86,6 +80,21 @@ int Batch::set_columns(int r_splits) {
     EAFP_TRY(hipMemcpy(cols.p, cols_h.data(), sizeof(ColDesc) * cols_h.size(), hipMemcpyHostToDevice));
     col_split = r_splits;
     ++tiling_gen;
+    return upload_tables();
+}
+
+// The tile table, the line tiles and the geometry on the device, with the
+// p.q partial-slot counts of the current stencil form (col_split).
+int Batch::upload_tables() {
+    set_partial_counts(geom, tiles_h, col_split);
+    EAFP_TRY(hipMemcpy(tiles.p, tiles_h.data(), sizeof(TileDesc) * ntiles, hipMemcpyHostToDevice));
+    if (nltiles > 0) {   // the line tiles carry the tile ranges and counts: rebuild them too
+        const std::vector<LineTile> lt = make_line_table(geom);
+        EAFP_TRY_RC(ltiles.alloc(lt.size()));
+        EAFP_TRY(hipMemcpy(ltiles.p, lt.data(), sizeof(LineTile) * lt.size(), hipMemcpyHostToDevice));
+    }
+    if (geom_d.p)
+        EAFP_TRY(hipMemcpy(geom_d.p, geom.data(), sizeof(SysGeom) * nsys, hipMemcpyHostToDevice));
     return EAFP_OK;
 }
 

@@ -109,6 +109,7 @@ struct Batch {
     int init(int n, const eafp_desc* descs);
     int set_tiling(int cells_per_thread, int blocks);
     int set_columns(int r_splits);
+    int upload_tables();
     // components = false skips the singular-component analysis, which only
     // the mixed solver needs (it reads the coefficients on the host).
     int upload(int s, const double* const host[8], const double* phi0, int arrays_on_device,

@@ -39,7 +39,7 @@ struct TileDesc {
     int n;                      // cells of the system
     int nr, nth, nz;
     int lt_begin, lt_end;       // the system's line tiles (z-line preconditioned kernels)
-    int pad;
+    int pa_n;                   // p.q partial slots the stencil writes: tile_begin .. tile_begin+pa_n-1
     long off;                   // first cell of the system in the arenas
     float inv_nz, inv_nth;
 };
@@ -55,6 +55,7 @@ struct SysGeom {
     float inv_nth;
     int lt_begin;     // [lt_begin, lt_end) in the line-tile table (0, 0: none)
     int lt_end;
+    int pa_n;         // p.q partial slots of the current stencil form (tiles, or column segments)
 };
 
 // Lays out the systems' cells in one arena: fills nr/nth/nz/n/off and the
@@ -96,6 +97,8 @@ inline std::vector<TileDesc> make_tile_table(std::vector<SysGeom>& g, int tile) 
         d.inv_nth = G.inv_nth;
         d.lt_begin = G.lt_begin;
         d.lt_end = G.lt_end;
+        G.pa_n = nt;
+        d.pa_n = nt;
         tiles.insert(tiles.end(), nt, d);
         t += nt;
     }
@@ -118,6 +121,8 @@ struct LineTile {
     int nz;
     int tile_begin, tile_end;   // the system's tiles (the stencil's partials)
     int lt_begin, lt_end;       // the system's line tiles (this kernel's partials)
+    int pa_n;                   // the stencil's p.q partial slots (as TileDesc::pa_n)
+    int pad;
     long off;                   // first cell of the system in the arenas
 };
 
@@ -161,6 +166,7 @@ inline std::vector<LineTile> make_line_table(const std::vector<SysGeom>& g) {
             L.tile_end = G.tile_end;
             L.lt_begin = G.lt_begin;
             L.lt_end = G.lt_end;
+            L.pa_n = G.pa_n;
             L.off = G.off;
             out.push_back(L);
         }
@@ -198,6 +204,24 @@ __host__ __device__ __forceinline__ int col_rows(int nz) { return COL_CELLS / nz
 // with empty entries to a multiple of NXCD.  Returns false (and no table)
 // when a system does not qualify: nz above COL_MAX_NZ, or fewer tiles than
 // segments (the partial slots).
+// Segments of one system at `splits` r-segments per column.
+inline int col_segments(const SysGeom& G, int splits) {
+    const int rows = col_rows(G.nz);
+    return ((G.nth + rows - 1) / rows) * (splits < G.nr ? splits : G.nr);
+}
+
+// The p.q partial slots each system's stencil writes: one per column
+// segment in the column form (the segment's index), one per tile otherwise.
+// Sets SysGeom::pa_n and every tile's copy of it.
+inline void set_partial_counts(std::vector<SysGeom>& g, std::vector<TileDesc>& tiles, int col_splits) {
+    for (SysGeom& G : g) {
+        const int nt = G.tile_end - G.tile_begin;
+        const int ns = col_splits > 0 ? col_segments(G, col_splits) : nt;
+        G.pa_n = ns < nt ? ns : nt;   // the column form exists only when every system has nseg <= its tiles
+    }
+    for (TileDesc& T : tiles) T.pa_n = g[T.sys].pa_n;
+}
+
 inline bool make_col_table(const std::vector<SysGeom>& g, int splits, std::vector<ColDesc>& out) {
     out.clear();
     for (const SysGeom& G : g) {

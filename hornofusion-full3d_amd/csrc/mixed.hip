@@ -562,7 +562,7 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate(MxArgs a, int par) {
         bool skip = P.done || P.broken;
         if (!skip) {
             double tot[1];
-            reduce_tiles<1>(a.partA, T.tile_begin, T.tile_end, tot, sh);
+            reduce_tiles<1>(a.partA, T.tile_begin, T.tile_begin + T.pa_n, tot, sh);
             if (!(fabs(tot[0]) > SMALL)) {
                 skip = true;
                 N.broken = 1;
@@ -764,7 +764,7 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate4(MxArgs a, int par) {
         bool skip = P.done || P.broken;
         if (!skip) {
             double tot[1];
-            reduce_tiles<1>(a.partA, T.tile_begin, T.tile_end, tot, sh);
+            reduce_tiles<1>(a.partA, T.tile_begin, T.tile_begin + T.pa_n, tot, sh);
             if (!(fabs(tot[0]) > SMALL)) {
                 skip = true;
                 N.broken = 1;
@@ -970,9 +970,7 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
         }
     }
     block_sum<1>(acc, sh);
-    if (tid == 0) a.partA[(long)T.tile_begin + C.seg] = acc[0];
-    for (long t = (long)T.tile_begin + C.seg + (long)(1 + tid) * C.nseg; t < T.tile_end; t += (long)BLOCK * C.nseg)
-        a.partA[t] = 0.0;
+    if (tid == 0) a.partA[(long)T.tile_begin + C.seg] = acc[0];   // the consumers read nseg slots
 }
 
 // ---------------------------------------------------------------------------
@@ -1168,9 +1166,7 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_colg(MxArgs a, int par) {
         }
     }
     block_sum<1>(acc, sh);
-    if (tid == 0) a.partA[(long)T.tile_begin + C.seg] = acc[0];
-    for (long t = (long)T.tile_begin + C.seg + (long)(1 + tid) * C.nseg; t < T.tile_end; t += (long)BLOCK * C.nseg)
-        a.partA[t] = 0.0;
+    if (tid == 0) a.partA[(long)T.tile_begin + C.seg] = acc[0];   // the consumers read nseg slots
 }
 
 // Finished systems in slot 1 (one block).
@@ -1394,7 +1390,7 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate_line(MxArgs a, int par) {
     bool skip = P.done || P.broken;
     if (!skip) {
         double tot[1];
-        reduce_tiles<1>(a.partA, L.tile_begin, L.tile_end, tot, sh);   // syncs the block
+        reduce_tiles<1>(a.partA, L.tile_begin, L.tile_begin + L.pa_n, tot, sh);   // syncs the block
         if (!(fabs(tot[0]) > SMALL)) {
             skip = true;
             N.broken = 1;

@@ -218,15 +218,22 @@ static void check_columns(std::vector<SysGeom> g, int cpt, int splits) {
                 if (hal) CHECK((hrow + 1) * T.nz + hk + 3 < SP, "halo Sp index");
             }
         }
-        CHECK(T.tile_begin + C.seg < T.tile_end, "segment slot");
-        for (long t = T.tile_begin + C.seg; t < T.tile_end; t += C.nseg) {
-            CHECK(part[t] == 0, "column partial slot %ld written twice", t);
-            part[t] = 1;
-        }
+        // one p.q partial per segment, at the system's first tile + seg
+        CHECK(C.seg >= 0 && C.seg < C.nseg && T.tile_begin + C.seg < T.tile_end, "segment slot");
+        const long ts = T.tile_begin + C.seg;
+        CHECK(part[ts] == 0, "column partial slot %ld written twice", ts);
+        part[ts] = 1;
     }
     for (const SysGeom& S : g)
         for (long c = 0; c < S.n; ++c) CHECK(hit[S.off + c] == 1, "column cell %ld of system at %ld missed", c, S.off);
-    for (int t = 0; t < ntiles; ++t) CHECK(part[t] == 1, "column partial slot %d never written", t);
+    // the update kernels read slots tile_begin .. tile_begin + pa_n - 1: exactly the written ones
+    set_partial_counts(g, tiles, splits);
+    std::vector<unsigned char> read(ntiles, 0);
+    for (const TileDesc& T : tiles) {
+        CHECK(T.pa_n == col_segments(g[T.sys], splits), "pa_n of system %d", T.sys);
+        for (int u = 0; u < T.pa_n; ++u) read[T.tile_begin + u] = 1;
+    }
+    for (int t = 0; t < ntiles; ++t) CHECK(part[t] == read[t], "partial slot %d written %d, read %d", t, part[t], read[t]);
     ++g_cases;
 }
 
@@ -318,6 +325,7 @@ static void check_lines(std::vector<SysGeom> g, int cpt, bool quad) {
         const SysGeom& G = g[L.sys];
         CHECK(b >= G.lt_begin && b < G.lt_end, "tile %d outside its system's range", b);
         CHECK(L.tile_begin == G.tile_begin && L.tile_end == G.tile_end, "tile range %d", b);
+        CHECK(L.pa_n >= 1 && L.pa_n <= L.tile_end - L.tile_begin && L.pa_n == G.pa_n, "partial count of tile %d", b);
         CHECK(L.nlines >= 1 && L.nlines <= line_lpb(L.nz), "nlines %d", L.nlines);
         CHECK(L.line0 + L.nlines <= G.nr * G.nth, "lines past the system");
         const int nq = line_lanes(L.nz);
