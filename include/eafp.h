@@ -260,7 +260,10 @@ typedef struct eafp_batch eafp_batch;
 int eafp_batch_create(int nsys, const eafp_desc* descs, eafp_batch** out);
 
 /* Uploads system s.  Arrays are padded Fortran layout; arrays_on_device = 0
- * for host pointers, 1 for device pointers.  phi0 may be NULL (zero). */
+ * for host pointers, 1 for device pointers (packed where they lie).  phi0
+ * may be NULL (zero).  The singular components the mixed solver projects
+ * are found here: a GPU pass proves most systems free of them; otherwise
+ * the coefficients are analysed on the host. */
 int eafp_batch_upload(eafp_batch* b, int s,
                       const double* aW, const double* aE, const double* aS, const double* aN,
                       const double* aB, const double* aT, const double* aP, const double* Su,
@@ -271,8 +274,9 @@ int eafp_batch_upload(eafp_batch* b, int s,
  * masks.  stats: nsys entries (may be NULL). */
 int eafp_batch_solve_fp64(eafp_batch* b, int max_iter, double tol, eafp_stats* stats);
 
-/* Mixed precision: FP32 inner Jacobi-PCG on low-precision symmetric face
- * coefficients, wrapped in FP64 iterative refinement: every `interval`
+/* Mixed precision: FP32 inner PCG (Jacobi or z-line preconditioner,
+ * opts->precond) on low-precision symmetric face coefficients, wrapped in
+ * FP64 iterative refinement: every `interval`
  * inner iterations the FP64 correction is folded into x, the true residual
  * b - A x is recomputed with the caller's FP64 operator and replaces the
  * inner residual, projected to zero mean on every singular Neumann
