@@ -168,12 +168,10 @@ def derive_traffic(fetch_path, write_path, workload, algo):
 def measure_workload(eafp, syss, fmt, coef, interval, label, precond=0):
     """Mixed (batched) and plain FP64 GPU solves of one workload to 1e-8:
     solves/s, iterations, and the iteration kernels' algorithmic GB/s.  The
-    z-line preconditioner is used when asked for and every nz is a multiple
-    of 4 (its quad kernels); batches with other nz run Jacobi, which is the
-    faster of the two there (the generic line kernels are latency-bound)."""
+    z-line preconditioner is used when asked for and every nz allows it."""
     ncells = sum(p.ncells for p in syss)
     b = eafp.Batch.from_systems(syss)
-    if precond and not all(p.shape[2] % 4 == 0 and p.shape[2] <= 1024 for p in syss):
+    if precond and not all(p.shape[2] <= 1024 for p in syss):
         precond = 0
     rec = {"workload": label, "systems": len(syss), "cells": ncells, "coef_format": coef,
            "precond": "zline" if precond else "jacobi"}

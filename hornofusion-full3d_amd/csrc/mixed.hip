@@ -1302,13 +1302,16 @@ __device__ __forceinline__ void line_solve(LineLds& S, bool inl, int li, int j, 
 
 // The cells of a line-tile thread.  Quad form (every nz of the batch a
 // multiple of 4): cells o .. o+3, one 16-byte access each.  Generic form:
-// the block's lines are one run of n = nlines*nz cells starting at
-// `run`; the block moves every array through LDS (stage/unstage: lane-
-// consecutive, coalesced accesses), and the thread takes its cells
-// li*nz + 4j .. +nv-1 of the run from there (nv < 4 only in a line's last
-// lane; slots past nv repeat its first cell).  Threads past the tile's
+// the block's lines are one run of n = nlines*nz cells starting at `run`;
+// the block moves every array through LDS in 16-byte chunks aligned to the
+// arena (stage/unstage; the chunks cover [run & ~3, run + n) rounded up to
+// 4, inside the system's 32-cell-aligned, padded slot of the arena), and
+// the thread takes its cells li*nz + 4j .. +nv-1 of the run from there
+// (nv < 4 only in a line's last lane; slots past nv repeat its first
+// cell).  Unstaging writes only the run's cells.  Threads past the tile's
 // lines take the run's first cell.
 constexpr int LINE_RUN = 4 * BLOCK;   // cells of a line tile at most
+constexpr int LINE_BUF = LINE_RUN + 4; // LDS floats per staged array (alignment slack)
 
 template <bool Q>
 struct LineCells {
@@ -1337,27 +1340,51 @@ struct LineCells {
             nv = inl ? min(4, L.nz - 4 * j) : 1;
         }
     }
-    // generic form: the run of p into buf (the caller syncs before reading)
+    // generic form: buf[m] holds cell (run & ~3) + m; the run's cell c is buf[sh + c]
+    __device__ int shift() const { return (int)(run & 3); }
+    __device__ int chunks() const { return (shift() + n + 3) >> 2; }
+    // the run of p into buf (the caller syncs before reading)
     __device__ void stage(const float* p, float* buf) const {
-        for (int c = threadIdx.x; c < n; c += BLOCK) buf[c] = __builtin_nontemporal_load(p + run + c);
+        const float* g = p + (run - shift());
+        for (int q = threadIdx.x; q < chunks(); q += BLOCK) {
+            const F4 v = ld4nt(g + 4 * q);
+            __builtin_memcpy(buf + 4 * q, &v, 16);
+        }
     }
     __device__ F4 from(const float* buf) const {
         F4 r;
+        const int b = shift() + c0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) r.v[k] = buf[c0 + (k < nv ? k : 0)];
+        for (int k = 0; k < 4; ++k) r.v[k] = buf[b + (k < nv ? k : 0)];
         return r;
     }
     __device__ void to(float* buf, const F4& v) const {
         if (!inl) return;
+        const int b = shift() + c0;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            if (k < nv) buf[c0 + k] = v.v[k];
+            if (k < nv) buf[b + k] = v.v[k];
     }
-    // generic form: buf back to the run of p (the caller syncs before)
+    // buf back to the run's cells of p (the caller syncs before): whole
+    // chunks with one 16-byte store, the run's partial first and last chunks
+    // cell by cell
     __device__ void unstage(float* p, const float* buf, bool nt) const {
-        for (int c = threadIdx.x; c < n; c += BLOCK) {
-            if (nt) __builtin_nontemporal_store(buf[c], p + run + c);
-            else p[run + c] = buf[c];
+        const int sh = shift(), nc = chunks();
+        float* g = p + (run - sh);
+        for (int q = threadIdx.x; q < nc; q += BLOCK) {
+            const int m0 = 4 * q;
+            if (m0 >= sh && m0 + 4 <= sh + n) {
+                F4 v;
+                __builtin_memcpy(&v, buf + m0, 16);
+                if (nt) st4nt(g + m0, v);
+                else st4(g + m0, v);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int m = m0 + k;
+                    if (m >= sh && m < sh + n) g[m] = buf[m];
+                }
+            }
         }
     }
 };
@@ -1367,7 +1394,7 @@ template <bool Q>
 __global__ __launch_bounds__(BLOCK) void k_mupdate_line(MxArgs a, int par) {
     __shared__ double sh[2 * (BLOCK / 64)];
     __shared__ LineLds S;
-    __shared__ float G[Q ? 1 : 5][Q ? 1 : LINE_RUN];   // generic form: d, p, r, q, factors of the run
+    __shared__ __align__(16) float G[Q ? 1 : 5][Q ? 4 : LINE_BUF];   // generic form: d, p, r, q, factors of the run
     const LineTile L = a.lt[blockIdx.x];
     const int s = L.sys;
     const LineCells<Q> C(L);

@@ -349,6 +349,14 @@ static void check_lines(std::vector<SysGeom> g, int cpt, bool quad) {
                 CHECK(run >= G.off && run + n <= G.off + G.n, "run outside system (tile %d)", b);
                 const int c0 = inl ? li * L.nz + 4 * j : 0;
                 CHECK(c0 >= 0 && c0 < n, "LDS index %d of %d", c0, n);
+                // 16-byte chunks aligned to the arena: [run & ~3, run + n) rounded up to 4
+                const int sh = (int)(run & 3), nch = (sh + n + 3) / 4;
+                const long gstart = run - sh;
+                const long slot_end = G.off + (G.n + 31) / 32 * 32;
+                CHECK(G.off % 4 == 0 && gstart >= G.off && gstart + 4L * nch <= slot_end,
+                      "staged chunks outside the system's slot (tile %d)", b);
+                CHECK(4 * nch <= 4 * BLOCK + 4, "staged chunks exceed the LDS buffer");
+                CHECK(sh + c0 + 3 < 4 * BLOCK + 4, "LDS read index");
                 o = L.off + (long)(L.line0 + (inl ? li : 0)) * L.nz + 4 * (inl ? j : 0);
                 CHECK(o == run + c0, "direct and staged cell differ");
                 nv = inl ? (L.nz - 4 * j < 4 ? L.nz - 4 * j : 4) : 1;
