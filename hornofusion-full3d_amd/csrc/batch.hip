@@ -125,7 +125,16 @@ int Batch::init(int n, const eafp_desc* descs) {
     for (const SysGeom& g : geom) quad = quad && (g.nz % 4 == 0);
     EAFP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     EAFP_TRY_RC(geom_d.alloc(n));
-    EAFP_TRY_RC(set_tiling(2, 0));
+    // default launch shape: for one system (the drop-in's case) what the
+    // autotuner picks for the FP64 CG on the 60x120x84 production mesh, one
+    // cell per thread and four blocks per CU (at most one per tile); for
+    // batches, two cells per thread and one block per tile up to 8 per CU
+    if (n == 1) {
+        const int nt1 = (geom[0].n + BLOCK - 1) / BLOCK;
+        EAFP_TRY_RC(set_tiling(1, std::min(4 * device_cus(), nt1)));
+    } else {
+        EAFP_TRY_RC(set_tiling(2, 0));
+    }
     for (int c = 0; c < NCOEF; ++c) EAFP_TRY_RC(coef[c].alloc(ncells));
     EAFP_TRY_RC(b.alloc(ncells));
     EAFP_TRY_RC(x0.alloc(ncells));
