@@ -1863,6 +1863,8 @@ static int mixed_start(Batch& B, const eafp_mixed_opts& o, int fmt) {
     EAFP_TRY(hipMemcpyAsync(w->cst.p, cst.data(), sizeof(MxConst) * B.nsys, hipMemcpyHostToDevice, B.stream));
     EAFP_TRY(hipMemcpyAsync(w->stat1.p, st.data(), sizeof(MxStat) * B.nsys, hipMemcpyHostToDevice, B.stream));
     EAFP_TRY(hipMemcpyAsync(w->dyn1.p, dy.data(), sizeof(MxDyn) * B.nsys, hipMemcpyHostToDevice, B.stream));
+    // the sources are this function's vectors: do not leave the copies queued behind them
+    EAFP_TRY(hipStreamSynchronize(B.stream));
     const int tile = BLOCK * B.cpt;
     if (fmt == EAFP_COEF_BF16) k_compress<EAFP_COEF_BF16><<<B.grid, BLOCK, 0, B.stream>>>(w->args, tile);
     else if (fmt == EAFP_COEF_FP16) k_compress<EAFP_COEF_FP16><<<B.grid, BLOCK, 0, B.stream>>>(w->args, tile);
