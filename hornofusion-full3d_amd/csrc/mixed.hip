@@ -89,6 +89,8 @@ struct MxArgs {
     const LineTile* lt;  // line tiles (z-line preconditioner)
     unsigned* fac;       // z-line factors per cell: FP16 l (low half), BF16 1/d (high half)
     int line;            // 1: z-line preconditioner; the r.z partials are per line tile
+    int state_first;     // 1 (batches): read the system's state before the cell loads, so the
+                         // blocks of converged systems load nothing; 0 (one system): overlap them
     // inner FP32 vectors
     float *d, *r, *z, *q;
     float* p[2];
@@ -469,8 +471,14 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv(MxArgs a, int par) {
             }
         };
         // the first batch of cell loads is in flight while the partials are reduced
-        load(t);
-        const MxDyn P = a.dyn[1][s];
+        MxDyn P;
+        if (a.state_first) {   // a batch: converged systems load nothing
+            P = a.dyn[1][s];
+            if (!(P.done || P.broken)) load(t);
+        } else {
+            load(t);
+            P = a.dyn[1][s];
+        }
         MxDyn N = P;
         bool skip = P.done || P.broken;
         if (!skip) {
@@ -556,8 +564,14 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate(MxArgs a, int par) {
             }
         };
         // the first batch of cell loads is in flight while the partials are reduced
-        load(t);
-        const MxDyn P = a.dyn[0][s];
+        MxDyn P;
+        if (a.state_first) {   // a batch: converged systems load nothing
+            P = a.dyn[0][s];
+            if (!(P.done || P.broken)) load(t);
+        } else {
+            load(t);
+            P = a.dyn[0][s];
+        }
         MxDyn N = P;
         bool skip = P.done || P.broken;
         if (!skip) {
@@ -660,8 +674,14 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv4(MxArgs a, int par) {
                 ev[m] = W::load4(a.e, a.scale, o);
             }
         };
-        load(t);
-        const MxDyn P = a.dyn[1][s];
+        MxDyn P;
+        if (a.state_first) {   // a batch: converged systems load nothing
+            P = a.dyn[1][s];
+            if (!(P.done || P.broken)) load(t);
+        } else {
+            load(t);
+            P = a.dyn[1][s];
+        }
         MxDyn N = P;
         bool skip = P.done || P.broken;
         if (!skip) {
@@ -758,8 +778,14 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate4(MxArgs a, int par) {
                 iv[m] = ld4nt(dinv + o);
             }
         };
-        load(t);
-        const MxDyn P = a.dyn[0][s];
+        MxDyn P;
+        if (a.state_first) {   // a batch: converged systems load nothing
+            P = a.dyn[0][s];
+            if (!(P.done || P.broken)) load(t);
+        } else {
+            load(t);
+            P = a.dyn[0][s];
+        }
         MxDyn N = P;
         bool skip = P.done || P.broken;
         if (!skip) {
@@ -854,19 +880,33 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
     const int i0 = C.i0, i1 = C.i1;
 
     // planes i0-1 (W), i0 and i0+1 (E) of z and p_old, the weights of i0,
-    // the halos of i0: in flight while the partials are reduced
-    F4 zW = ld4(z + base(i0 > 0 ? i0 - 1 : i0) + qoff), oW = ld4(pold + base(i0 > 0 ? i0 - 1 : i0) + qoff);
-    F4 w0W = W::load4(a.w[0], a.scale, base(i0 > 0 ? i0 - 1 : i0) + qoff);
-    F4 zC = ld4(z + base(i0) + qoff), oC = ld4(pold + base(i0) + qoff);
-    F4 zE = ld4(z + base(i0 + 1) + qoff), oE = ld4(pold + base(i0 + 1) + qoff);
-    F4 w0 = W::load4(a.w[0], a.scale, base(i0) + qoff);
-    F4 w1 = W::load4(a.w[1], a.scale, base(i0) + qoff);
-    F4 w2 = W::load4(a.w[2], a.scale, base(i0) + qoff);
-    F4 ev = W::load4(a.e, a.scale, base(i0) + qoff);
-    F4 zH = ld4(z + base(i0) + hoff), oH = ld4(pold + base(i0) + hoff);
-    F4 w1H = W::load4(a.w[1], a.scale, base(i0) + hoff);
-
-    const MxDyn P = a.dyn[1][s];
+    // the halos of i0: in flight while the partials are reduced (in a batch,
+    // only once the system's state shows it still running)
+    F4 zW, oW, w0W, zC, oC, zE, oE, w0, w1, w2, ev, zH, oH, w1H;
+    auto prologue = [&]() {
+        zW = ld4(z + base(i0 > 0 ? i0 - 1 : i0) + qoff);
+        oW = ld4(pold + base(i0 > 0 ? i0 - 1 : i0) + qoff);
+        w0W = W::load4(a.w[0], a.scale, base(i0 > 0 ? i0 - 1 : i0) + qoff);
+        zC = ld4(z + base(i0) + qoff);
+        oC = ld4(pold + base(i0) + qoff);
+        zE = ld4(z + base(i0 + 1) + qoff);
+        oE = ld4(pold + base(i0 + 1) + qoff);
+        w0 = W::load4(a.w[0], a.scale, base(i0) + qoff);
+        w1 = W::load4(a.w[1], a.scale, base(i0) + qoff);
+        w2 = W::load4(a.w[2], a.scale, base(i0) + qoff);
+        ev = W::load4(a.e, a.scale, base(i0) + qoff);
+        zH = ld4(z + base(i0) + hoff);
+        oH = ld4(pold + base(i0) + hoff);
+        w1H = W::load4(a.w[1], a.scale, base(i0) + hoff);
+    };
+    MxDyn P;
+    if (a.state_first) {
+        P = a.dyn[1][s];
+        if (!(P.done || P.broken)) prologue();
+    } else {
+        prologue();
+        P = a.dyn[1][s];
+    }
     MxDyn N = P;
     bool skip = P.done || P.broken;
     if (!skip) {
@@ -1033,29 +1073,37 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_colg(MxArgs a, int par) {
     const int i0 = C.i0, i1 = C.i1;
     const long bW = base(i0 > 0 ? i0 - 1 : i0), b0 = base(i0), b1 = base(i0 + 1);
     float zW[M], oW[M], w0W[M], zC[M], oC[M], zE[M], oE[M], w0[M], w1[M], w2[M], ev[M];
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-        zW[m] = z[bW + coff[m]];
-        oW[m] = pold[bW + coff[m]];
-        w0W[m] = W::load(a.w[0], a.scale, bW + coff[m]);
-        zC[m] = z[b0 + coff[m]];
-        oC[m] = pold[b0 + coff[m]];
-        zE[m] = z[b1 + coff[m]];
-        oE[m] = pold[b1 + coff[m]];
-        w0[m] = W::load(a.w[0], a.scale, b0 + coff[m]);
-        w1[m] = W::load(a.w[1], a.scale, b0 + coff[m]);
-        w2[m] = W::load(a.w[2], a.scale, b0 + coff[m]);
-        ev[m] = W::load(a.e, a.scale, b0 + coff[m]);
-    }
     float zH[MH], oH[MH], w1H[MH];
+    auto prologue = [&]() {
 #pragma unroll
-    for (int m = 0; m < MH; ++m) {
-        zH[m] = z[b0 + hoff[m]];
-        oH[m] = pold[b0 + hoff[m]];
-        w1H[m] = W::load(a.w[1], a.scale, b0 + hoff[m]);
+        for (int m = 0; m < M; ++m) {
+            zW[m] = z[bW + coff[m]];
+            oW[m] = pold[bW + coff[m]];
+            w0W[m] = W::load(a.w[0], a.scale, bW + coff[m]);
+            zC[m] = z[b0 + coff[m]];
+            oC[m] = pold[b0 + coff[m]];
+            zE[m] = z[b1 + coff[m]];
+            oE[m] = pold[b1 + coff[m]];
+            w0[m] = W::load(a.w[0], a.scale, b0 + coff[m]);
+            w1[m] = W::load(a.w[1], a.scale, b0 + coff[m]);
+            w2[m] = W::load(a.w[2], a.scale, b0 + coff[m]);
+            ev[m] = W::load(a.e, a.scale, b0 + coff[m]);
+        }
+#pragma unroll
+        for (int m = 0; m < MH; ++m) {
+            zH[m] = z[b0 + hoff[m]];
+            oH[m] = pold[b0 + hoff[m]];
+            w1H[m] = W::load(a.w[1], a.scale, b0 + hoff[m]);
+        }
+    };
+    MxDyn P;
+    if (a.state_first) {   // a batch: converged systems load nothing
+        P = a.dyn[1][s];
+        if (!(P.done || P.broken)) prologue();
+    } else {
+        prologue();
+        P = a.dyn[1][s];
     }
-
-    const MxDyn P = a.dyn[1][s];
     MxDyn N = P;
     bool skip = P.done || P.broken;
     if (!skip) {
@@ -1399,20 +1447,29 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate_line(MxArgs a, int par) {
     const int s = L.sys;
     const LineCells<Q> C(L);
     F4 dv, pvv, rv, qv, fv;
-    if (Q) {
-        dv = ld4nt(a.d + C.o);
-        pvv = ld4nt(a.p[par ^ 1] + C.o);
-        rv = ld4nt(a.r + C.o);
-        qv = ld4nt(a.q + C.o);
-        fv = ld4nt((const float*)a.fac + C.o);
+    auto load_cells = [&]() {
+        if (Q) {
+            dv = ld4nt(a.d + C.o);
+            pvv = ld4nt(a.p[par ^ 1] + C.o);
+            rv = ld4nt(a.r + C.o);
+            qv = ld4nt(a.q + C.o);
+            fv = ld4nt((const float*)a.fac + C.o);
+        } else {
+            C.stage(a.d, G[0]);
+            C.stage(a.p[par ^ 1], G[1]);
+            C.stage(a.r, G[2]);
+            C.stage(a.q, G[3]);
+            C.stage((const float*)a.fac, G[4]);
+        }
+    };
+    MxDyn P;
+    if (a.state_first) {
+        P = a.dyn[0][s];
+        if (!(P.done || P.broken)) load_cells();
     } else {
-        C.stage(a.d, G[0]);
-        C.stage(a.p[par ^ 1], G[1]);
-        C.stage(a.r, G[2]);
-        C.stage(a.q, G[3]);
-        C.stage((const float*)a.fac, G[4]);
+        load_cells();
+        P = a.dyn[0][s];
     }
-    const MxDyn P = a.dyn[0][s];
     MxDyn N = P;
     bool skip = P.done || P.broken;
     if (!skip) {
@@ -1734,6 +1791,7 @@ static int ensure_mixed(Batch& B, int fmt) {
     a.dinv = w->dinv.p;
     a.lt = B.ltiles.p;
     a.fac = w->fac.p;
+    a.state_first = B.nsys > 1 ? 1 : 0;
     a.d = w->d.p;
     a.r = w->r.p;
     a.z = w->z.p;
