@@ -60,6 +60,7 @@ struct Cg64Args {
     CgSlot* slot[2];
     const CgConst* cst;
     int* ndone;          // finished systems, counted after each chunk
+    int state_first;     // 1 (batches): read a system's state before its cell loads; 0: overlap them
 };
 
 struct Cg64Work {
@@ -196,9 +197,16 @@ __global__ __launch_bounds__(BLOCK) void k64_spmv(Cg64Args a, int par) {
                 cf[m][6] = cT[o];
             }
         };
-        // the first batch of cell loads is in flight while the partials are reduced
-        load(t);
-        const CgSlot P = a.slot[1][s];
+        // one system: the first batch of cell loads is in flight while the
+        // partials are reduced; a batch: converged systems load nothing
+        CgSlot P;
+        if (a.state_first) {
+            P = a.slot[1][s];
+            if (!P.done) load(t);
+        } else {
+            load(t);
+            P = a.slot[1][s];
+        }
         CgSlot N = P;
         if (!P.done) {
             const CgConst K = a.cst[s];
@@ -303,9 +311,16 @@ __global__ __launch_bounds__(BLOCK) void k64_update(Cg64Args a, int par) {
                 apv[m] = aP[o];
             }
         };
-        // the first batch of cell loads is in flight while the partials are reduced
-        load(t);
-        const CgSlot P = a.slot[0][s];
+        // one system: the first batch of cell loads is in flight while the
+        // partials are reduced; a batch: converged systems load nothing
+        CgSlot P;
+        if (a.state_first) {
+            P = a.slot[0][s];
+            if (!P.done) load(t);
+        } else {
+            load(t);
+            P = a.slot[0][s];
+        }
         CgSlot N = P;
         if (!P.done) {
             double tot[1];
@@ -402,6 +417,7 @@ static int ensure_cg64(Batch& B) {
     a.partB = w->partB.p;
     a.slot[0] = w->slot0.p;
     a.slot[1] = w->slot1.p;
+    a.state_first = B.nsys > 1 ? 1 : 0;
     a.cst = w->cst.p;
     a.ndone = w->ndone.p;
     return EAFP_OK;
