@@ -5,6 +5,7 @@
 // host round trip), and the post-solve kernels read p' straight from the
 // solver's padded iterate.  Host-array callers pay one upload of the fields
 // and one download of the results instead of the three steps' transfers.
+#include <algorithm>
 #include <memory>
 #include <mutex>
 
@@ -68,6 +69,8 @@ int ensure_step(const eafp_desc* d, bool host_mode) {
     }
     return EAFP_OK;
 }
+
+long n_interior(const eafp_desc* d) { return (long)d->nr * d->ntheta * d->nz; }
 
 bool valid_mixed(const eafp_mixed_opts& o) {
     return o.tol > 0.0 && o.max_steps >= 1 && o.interval >= 0 && o.coef_format >= EAFP_COEF_FP32 &&
@@ -182,6 +185,14 @@ int pp_step(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config* c
     for (int q = 0; q < 8; ++q) csys[q] = sys[q];
     const bool mixed = o->solver == EAFP_STEP_MIXED;
     EAFP_TRY_RC(B.upload(0, csys, nullptr, 1, mixed));
+    // launch shape by solver, as tuned on the 60x120x84 production mesh: the
+    // FP64 CG at one cell per thread, the mixed solver at two; four blocks
+    // per CU, at most one per tile
+    const int cpt = mixed ? 2 : 1;
+    if (B.cpt != cpt) {
+        const int tiles = (int)((n_interior(d) + (long)BLOCK * cpt - 1) / ((long)BLOCK * cpt));
+        EAFP_TRY_RC(B.set_tiling(cpt, std::min(4 * device_cus(), tiles)));
+    }
     eafp_stats st{};
     if (mixed)
         EAFP_TRY_RC(solve_mixed(B, o->mixed, &st));
