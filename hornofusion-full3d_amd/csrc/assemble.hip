@@ -569,6 +569,7 @@ int ensure_pp(const eafp_desc* d) {
     if (w.nr == d->nr && w.nth == d->ntheta && w.nz == d->nz) return EAFP_OK;
     const long n = (long)(d->nr + 4) * (d->ntheta + 4) * (d->nz + 4);
     if (n > (1L << 31) - 8L * BLOCK) return EAFP_ERR_UNSUPPORTED;   // 32-bit cell positions
+    w.nr = w.nth = w.nz = -1;   // a failed allocation below leaves the workspace invalid, not stale
     EAFP_TRY_RC(w.coord.alloc((size_t)(d->nr + d->ntheta + d->nz + 12)));
     EAFP_TRY_RC(w.f3.alloc((size_t)n * NF3));
     EAFP_TRY_RC(w.grads.alloc((size_t)n * 4));

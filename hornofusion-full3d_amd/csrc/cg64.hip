@@ -368,10 +368,10 @@ __global__ __launch_bounds__(BLOCK) void k64_count(Cg64Args a) {
     if (threadIdx.x == 0) *a.ndone = (int)v[0];
 }
 
-static int ensure_cg64(Batch& B) {
-    if (!B.cg64) {
-        Cg64Work* w = new Cg64Work();
-        B.cg64 = w;
+// First-time buffers of the FP64 solver; the caller publishes w only when
+// every one of them exists.
+static int init_cg64(Batch& B, Cg64Work* w) {
+    {
         const size_t n = B.ncells;
         EAFP_TRY_RC(w->x.alloc(n));
         EAFP_TRY_RC(w->r.alloc(n));
@@ -389,6 +389,19 @@ static int ensure_cg64(Batch& B) {
         EAFP_TRY(hipEventCreate(&w->t1));
         EAFP_TRY(hipMemset(w->p0.p, 0, n * sizeof(double)));
         EAFP_TRY(hipMemset(w->p1.p, 0, n * sizeof(double)));
+    }
+    return EAFP_OK;
+}
+
+static int ensure_cg64(Batch& B) {
+    if (!B.cg64) {
+        Cg64Work* nw = new Cg64Work();
+        const int rc = init_cg64(B, nw);
+        if (rc != EAFP_OK) {
+            free_cg64(nw);
+            return rc;
+        }
+        B.cg64 = nw;
     }
     Cg64Work* w = B.cg64;
     EAFP_TRY_RC(w->partA.alloc((size_t)B.ntiles));

@@ -1722,10 +1722,10 @@ void free_mixed(MixedWork* w) {
 
 int download_x(Batch& B, int s, const double* x_compact);
 
-static int ensure_mixed(Batch& B, int fmt) {
-    if (!B.mixed) {
-        MixedWork* w = new MixedWork();
-        B.mixed = w;
+// First-time buffers of the mixed solver; the caller publishes w only when
+// every one of them exists.
+static int init_mixed(Batch& B, MixedWork* w) {
+    {
         const size_t n = B.ncells;
         EAFP_TRY_RC(w->X0.alloc(n));
         EAFP_TRY_RC(w->X1.alloc(n));
@@ -1752,9 +1752,23 @@ static int ensure_mixed(Batch& B, int fmt) {
         EAFP_TRY(hipEventCreate(&w->t0));
         EAFP_TRY(hipEventCreate(&w->t1));
     }
+    return EAFP_OK;
+}
+
+static int ensure_mixed(Batch& B, int fmt) {
+    if (!B.mixed) {
+        MixedWork* nw = new MixedWork();
+        const int rc = init_mixed(B, nw);
+        if (rc != EAFP_OK) {
+            free_mixed(nw);
+            return rc;
+        }
+        B.mixed = nw;
+    }
     MixedWork* w = B.mixed;
     if (w->fmt != fmt) {
         const size_t n = B.ncells;
+        w->fmt = -1;   // a failed allocation below leaves the weights invalid, not stale
         for (auto& wb : w->w) EAFP_TRY_RC(wb.alloc(n * wbytes(fmt)));
         EAFP_TRY_RC(w->e.alloc(n * wbytes(fmt)));
         if (fmt == EAFP_COEF_FP16) EAFP_TRY_RC(w->scale.alloc(n));

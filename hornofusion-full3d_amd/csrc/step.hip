@@ -43,7 +43,8 @@ int ensure_step(const eafp_desc* d, bool host_mode) {
     StepWork& w = *g_step;
     const long n = (long)(d->nr + 4) * (d->ntheta + 4) * (d->nz + 4);
     const long n2 = (long)(d->nr + 4) * (d->ntheta + 4);
-    if (w.nr != d->nr || w.nth != d->ntheta || w.nz != d->nz) {
+    if (!w.B || w.nr != d->nr || w.nth != d->ntheta || w.nz != d->nz) {
+        w.nr = w.nth = w.nz = -1;   // a failure below leaves the workspace invalid, not stale
         w.B.reset();
         auto nb = std::make_unique<Batch>();
         EAFP_TRY_RC(nb->init(1, d));
