@@ -187,3 +187,15 @@ def test_zline_autotune_and_kernel_timer():
         st = b.solve_mixed(tol=1e-8, coef_format=COEF_BF16, precond=PRECOND_ZLINE)[0]
         phi = b.download(0)
     check_answer(ps, phi, st, (2, (20, 36, 28)))
+
+
+def test_zline_invalid_precond_refused():
+    """An unknown preconditioner code is EAFP_ERR_ARG; the batch still solves afterwards."""
+    from eafp._lib import EAFP_ERR_ARG
+    ps = systems.make_system(1, 0)
+    with eafp.Batch.from_systems([ps]) as b:
+        with pytest.raises(eafp.EafpError) as e:
+            b.solve_mixed(tol=1e-8, coef_format=COEF_BF16, precond=7)
+        assert e.value.rc == EAFP_ERR_ARG
+        st = b.solve_mixed(tol=1e-8, coef_format=COEF_BF16, precond=PRECOND_ZLINE)[0]
+        assert st["status"] == "converged"
