@@ -55,7 +55,7 @@ int autotune(Batch& B, int solver, int fmt, int reps, int* best_cpt, int* best_b
     if (solver == 1 || solver == 2) {
         // the column-marching stencil, where every system qualifies
         int bs = 0;
-        for (int k : {1, 2, 4, 8, 16}) {
+        for (int k : {1, 2, 4, 6, 8, 10, 12, 16}) {
             if (B.set_columns(k) != EAFP_OK) continue;
             float a = 0.f, b = 0.f;
             EAFP_TRY_RC(time_kernels(B, solver, fmt, 2, &a, &b));

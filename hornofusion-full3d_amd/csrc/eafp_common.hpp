@@ -193,7 +193,7 @@ struct ColDesc {
     int i0, i1;    // r planes of the segment
     int seg;       // index among the system's segments (0 writes the system's state)
     int nseg;      // segments of the system
-    int pad;
+    int dir;       // +1: the quad-form march goes up in r, -1: down (odd segments of a column)
 };
 
 // Rows per column for a given nz (every thread holds one quad of a row).
@@ -241,12 +241,14 @@ inline bool make_col_table(const std::vector<SysGeom>& g, int splits, std::vecto
                 d.i1 = (int)(((long)G.nr * (sp + 1)) / nsp);
                 d.seg = c * nsp + sp;
                 d.nseg = nseg;
+                d.dir = (sp & 1) ? -1 : 1;
                 out.push_back(d);
             }
     }
     while (out.size() % NXCD) {
         ColDesc e{};
         e.tile = -1;
+        e.dir = 1;
         out.push_back(e);
     }
     return true;

@@ -844,7 +844,12 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate4(MxArgs a, int par) {
 // the two theta halo rows, loaded by the first 2*nz/4 threads), from which
 // the theta and z neighbours are read: one barrier per plane.  Per cell it
 // reads z, p_old, the 3 weights and e once, plus the halo rows (z and p_old
-// of 2 rows, w1 of one, per nj rows), and writes p and q.
+// of 2 rows, w1 of one, per nj rows), and writes p and q.  A segment marches
+// up in r (C.dir = +1) or down (-1): the segments of a column alternate, so
+// the two segments on either side of a boundary read the planes there (each
+// one's extra plane beyond its range) at about the same time, and the
+// second read finds them in L2.  Registers keep W/E by position in r, so
+// the arithmetic per cell is the same in both directions.
 // ---------------------------------------------------------------------------
 template <int F>
 __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
@@ -876,17 +881,22 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
     const int hk = hal ? 4 * (tid < Qr ? tid : tid - Qr) : 0;
     const long hoff = (long)(((C.j0 + hrow) % nth + nth) % nth) * nz + hk;
     const long sr = (long)nth * nz;
-    auto base = [&](int i) { return T.off + (long)(i < nr ? i : nr - 1) * sr; };
-    const int i0 = C.i0, i1 = C.i1;
+    // a plane outside 0 .. nr-1 is clamped to the nearest one: its loads stay
+    // in the system, and as a neighbour it equals the cell's own p (face term 0)
+    auto base = [&](int i) { return T.off + (long)(i < 0 ? 0 : i < nr ? i : nr - 1) * sr; };
+    const int dir = C.dir;
+    const int i0 = dir > 0 ? C.i0 : C.i1 - 1;   // first plane of the march
+    const int np = C.i1 - C.i0;
 
-    // planes i0-1 (W), i0 and i0+1 (E) of z and p_old, the weights of i0,
-    // the halos of i0: in flight while the partials are reduced (in a batch,
-    // only once the system's state shows it still running)
+    // planes i0-1 (W), i0 and i0+1 (E) of z and p_old, the weights of i0
+    // (w0 is the face to i0+1, w0W the face to i0-1), the halos of i0: in
+    // flight while the partials are reduced (in a batch, only once the
+    // system's state shows it still running)
     F4 zW, oW, w0W, zC, oC, zE, oE, w0, w1, w2, ev, zH, oH, w1H;
     auto prologue = [&]() {
-        zW = ld4(z + base(i0 > 0 ? i0 - 1 : i0) + qoff);
-        oW = ld4(pold + base(i0 > 0 ? i0 - 1 : i0) + qoff);
-        w0W = W::load4(a.w[0], a.scale, base(i0 > 0 ? i0 - 1 : i0) + qoff);
+        zW = ld4(z + base(i0 - 1) + qoff);
+        oW = ld4(pold + base(i0 - 1) + qoff);
+        w0W = W::load4(a.w[0], a.scale, base(i0 - 1) + qoff);
         zC = ld4(z + base(i0) + qoff);
         oC = ld4(pold + base(i0) + qoff);
         zE = ld4(z + base(i0 + 1) + qoff);
@@ -932,21 +942,25 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
     };
     F4 pW = pv(zW, oW), pC = pv(zC, oC), pE = pv(zE, oE), pH = pv(zH, oH);
     double acc[1] = {0.0};
-    for (int i = i0; i < i1; ++i) {
+    for (int t = 0; t < np; ++t) {
+        const int i = i0 + dir * t;
         const int b = i & 1;
-        // loads for the next plane: z/p_old of i+2 (its E), weights and halos of i+1
-        const bool more = i + 1 < i1;
+        // loads for the next plane i+dir: z/p_old of i+2*dir (its outer
+        // neighbour), its weights and halos, and its outer face (w0 of i+1
+        // up, w0 of i-2 down)
+        const bool more = t + 1 < np;
         F4 zN2, oN2, w0n, w1n, w2n, evn, zHn, oHn, w1Hn;
         if (more) {
-            zN2 = ld4(z + base(i + 2) + qoff);
-            oN2 = ld4(pold + base(i + 2) + qoff);
-            w0n = W::load4(a.w[0], a.scale, base(i + 1) + qoff);
-            w1n = W::load4(a.w[1], a.scale, base(i + 1) + qoff);
-            w2n = W::load4(a.w[2], a.scale, base(i + 1) + qoff);
-            evn = W::load4(a.e, a.scale, base(i + 1) + qoff);
-            zHn = ld4(z + base(i + 1) + hoff);
-            oHn = ld4(pold + base(i + 1) + hoff);
-            w1Hn = W::load4(a.w[1], a.scale, base(i + 1) + hoff);
+            const int in = i + dir;
+            zN2 = ld4(z + base(i + 2 * dir) + qoff);
+            oN2 = ld4(pold + base(i + 2 * dir) + qoff);
+            w0n = W::load4(a.w[0], a.scale, base(dir > 0 ? in : in - 1) + qoff);
+            w1n = W::load4(a.w[1], a.scale, base(in) + qoff);
+            w2n = W::load4(a.w[2], a.scale, base(in) + qoff);
+            evn = W::load4(a.e, a.scale, base(in) + qoff);
+            zHn = ld4(z + base(in) + hoff);
+            oHn = ld4(pold + base(in) + hoff);
+            w1Hn = W::load4(a.w[1], a.scale, base(in) + hoff);
         }
         // plane i into the slab: p (rows -1 .. nj), w1 (rows -1 .. nj-1), w2 of each quad's last cell
         if (act) {
@@ -997,11 +1011,20 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
             for (int q = 0; q < 4; ++q) acc[0] += (double)pC.v[q] * (double)y.v[q];
         }
         if (more) {
-            pW = pC;
-            pC = pE;
-            pE = pv(zN2, oN2);
-            w0W = w0;
-            w0 = w0n;
+            const F4 pn = pv(zN2, oN2);
+            if (dir > 0) {
+                pW = pC;
+                pC = pE;
+                pE = pn;
+                w0W = w0;
+                w0 = w0n;
+            } else {
+                pE = pC;
+                pC = pW;
+                pW = pn;
+                w0 = w0W;
+                w0W = w0n;
+            }
             w1 = w1n;
             w2 = w2n;
             ev = evn;

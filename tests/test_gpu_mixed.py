@@ -179,13 +179,15 @@ def test_mixed_quad_kernels(cpt, fmt):
 
 
 @pytest.mark.parametrize("fmt", list(FORMATS))
-@pytest.mark.parametrize("splits", [1, 3])
+@pytest.mark.parametrize("splits", [1, 2, 3])
 def test_mixed_column_kernel(splits, fmt):
     """The column-marching stencil (theta-row slabs marched through r, one
     quad per thread, LDS theta/z neighbours): same bar as the tile kernels.
     Shapes: a slab narrower than nth with a partial last column, a single
     column whose halo rows wrap onto itself (nth*nz <= 1024), nz = 32 and 28,
-    r-segments that split the planes unevenly."""
+    r-segments that split the planes unevenly.  Odd segments march down in
+    r: with 2 splits one ends at the outer wall, with 3 one lies between two
+    upward ones."""
     syss = [systems.make_system(2, 0, (20, 36, 28)), systems.make_system(1, 0),
             systems.make_system(5, 1, (24, 48, 32)), systems.make_system(2, 3, (7, 50, 20))]
     with eafp.Batch.from_systems(syss) as b:

@@ -182,7 +182,9 @@ static void check_columns(std::vector<SysGeom> g, int cpt, int splits) {
         const int Qr = T.nz / 4;
         CHECK(C.nj >= 1 && C.nj * T.nz <= COL_CELLS && C.j0 + C.nj <= T.nth && C.i0 < C.i1 && C.i1 <= T.nr,
               "column shape");
-        auto base = [&](int i) { return T.off + (long)(i < T.nr ? i : T.nr - 1) * sr; };
+        CHECK(C.dir == 1 || C.dir == -1, "march direction");
+        auto base = [&](int i) { return T.off + (long)(i < 0 ? 0 : i < T.nr ? i : T.nr - 1) * sr; };
+        const int ib = C.dir > 0 ? C.i0 : C.i1 - 1, np = C.i1 - C.i0;
         for (int tid = 0; tid < BLOCK; ++tid) {
             const int row = tid / Qr, kq = tid - row * Qr;
             const bool act = row < C.nj;
@@ -192,15 +194,19 @@ static void check_columns(std::vector<SysGeom> g, int cpt, int splits) {
             const int hk = hal ? 4 * (tid < Qr ? tid : tid - Qr) : 0;
             const long hoff = (long)(((C.j0 + hrow) % T.nth + T.nth) % T.nth) * T.nz + hk;
             auto vec = [&](long a) { CHECK(a % 4 == 0 && a >= lo && a + 4 <= hi, "column vector %ld", a); };
-            vec(base(C.i0 > 0 ? C.i0 - 1 : C.i0) + qoff);
-            vec(base(C.i0) + qoff);
-            vec(base(C.i0 + 1) + qoff);
-            vec(base(C.i0) + hoff);
-            for (int i = C.i0; i < C.i1; ++i) {
-                if (i + 1 < C.i1) {
-                    vec(base(i + 2) + qoff);
-                    vec(base(i + 1) + qoff);
-                    vec(base(i + 1) + hoff);
+            vec(base(ib - 1) + qoff);
+            vec(base(ib) + qoff);
+            vec(base(ib + 1) + qoff);
+            vec(base(ib) + hoff);
+            for (int t = 0; t < np; ++t) {
+                const int i = ib + C.dir * t;
+                CHECK(i >= C.i0 && i < C.i1, "march plane");
+                if (t + 1 < np) {
+                    const int in = i + C.dir;
+                    vec(base(i + 2 * C.dir) + qoff);
+                    vec(base(C.dir > 0 ? in : in - 1) + qoff);
+                    vec(base(in) + qoff);
+                    vec(base(in) + hoff);
                 }
                 if (act) {
                     for (int q = 0; q < 4; ++q) {
