@@ -193,7 +193,7 @@ struct ColDesc {
     int i0, i1;    // r planes of the segment
     int seg;       // index among the system's segments (0 writes the system's state)
     int nseg;      // segments of the system
-    int dir;       // +1: the quad-form march goes up in r, -1: down (odd segments of a column)
+    int dir;       // +1: the march goes up in r, -1: down (odd segments of a column)
 };
 
 // Rows per column for a given nz (every thread holds one quad of a row).

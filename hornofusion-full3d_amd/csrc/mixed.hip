@@ -1038,7 +1038,8 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
 
 // ---------------------------------------------------------------------------
 // Column-marching stencil for any nz (at most COL_MAX_NZ): as k_mspmv_col,
-// but a thread takes the slab cells tid + m*BLOCK (m < 4) of each plane with
+// alternating march directions included, but a thread takes the slab cells
+// tid + m*BLOCK (m < 4) of each plane with
 // 4-byte accesses, and the z neighbours of every cell come from the LDS
 // slab too (w+z of the whole slab is staged for the -z faces).
 // ---------------------------------------------------------------------------
@@ -1092,9 +1093,11 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_colg(MxArgs a, int par) {
         hoff[m] = (long)(((C.j0 + hrow) % nth + nth) % nth) * nz + hk;
     }
     const long sr = (long)nth * nz;
-    auto base = [&](int i) { return T.off + (long)(i < nr ? i : nr - 1) * sr; };
-    const int i0 = C.i0, i1 = C.i1;
-    const long bW = base(i0 > 0 ? i0 - 1 : i0), b0 = base(i0), b1 = base(i0 + 1);
+    auto base = [&](int i) { return T.off + (long)(i < 0 ? 0 : i < nr ? i : nr - 1) * sr; };
+    const int dir = C.dir;                      // as in k_mspmv_col
+    const int i0 = dir > 0 ? C.i0 : C.i1 - 1;   // first plane of the march
+    const int np = C.i1 - C.i0;
+    const long bW = base(i0 - 1), b0 = base(i0), b1 = base(i0 + 1);
     float zW[M], oW[M], w0W[M], zC[M], oC[M], zE[M], oE[M], w0[M], w1[M], w2[M], ev[M];
     float zH[MH], oH[MH], w1H[MH];
     auto prologue = [&]() {
@@ -1154,17 +1157,19 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_colg(MxArgs a, int par) {
 #pragma unroll
     for (int m = 0; m < MH; ++m) pH[m] = zH[m] + fb * oH[m];
     double acc[1] = {0.0};
-    for (int i = i0; i < i1; ++i) {
+    for (int t = 0; t < np; ++t) {
+        const int i = i0 + dir * t;
         const int b = i & 1;
-        const bool more = i + 1 < i1;
+        const bool more = t + 1 < np;
         float zN2[M], oN2[M], w0n[M], w1n[M], w2n[M], evn[M], zHn[MH], oHn[MH], w1Hn[MH];
         if (more) {
-            const long b2 = base(i + 2), bn = base(i + 1);
+            const int in = i + dir;
+            const long b2 = base(i + 2 * dir), bn = base(in), bf = base(dir > 0 ? in : in - 1);
 #pragma unroll
             for (int m = 0; m < M; ++m) {
                 zN2[m] = z[b2 + coff[m]];
                 oN2[m] = pold[b2 + coff[m]];
-                w0n[m] = W::load(a.w[0], a.scale, bn + coff[m]);
+                w0n[m] = W::load(a.w[0], a.scale, bf + coff[m]);
                 w1n[m] = W::load(a.w[1], a.scale, bn + coff[m]);
                 w2n[m] = W::load(a.w[2], a.scale, bn + coff[m]);
                 evn[m] = W::load(a.e, a.scale, bn + coff[m]);
@@ -1220,11 +1225,20 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_colg(MxArgs a, int par) {
         if (more) {
 #pragma unroll
             for (int m = 0; m < M; ++m) {
-                pW[m] = pC[m];
-                pC[m] = pE[m];
-                pE[m] = zN2[m] + fb * oN2[m];
-                w0W[m] = w0[m];
-                w0[m] = w0n[m];
+                const float pn = zN2[m] + fb * oN2[m];
+                if (dir > 0) {
+                    pW[m] = pC[m];
+                    pC[m] = pE[m];
+                    pE[m] = pn;
+                    w0W[m] = w0[m];
+                    w0[m] = w0n[m];
+                } else {
+                    pE[m] = pC[m];
+                    pC[m] = pW[m];
+                    pW[m] = pn;
+                    w0[m] = w0W[m];
+                    w0W[m] = w0n[m];
+                }
                 w1[m] = w1n[m];
                 w2[m] = w2n[m];
                 ev[m] = evn[m];

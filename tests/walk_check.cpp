@@ -257,7 +257,9 @@ static void check_columns_generic(std::vector<SysGeom> g, int cpt, int splits) {
         const TileDesc T = tiles[C.tile];
         const long lo = T.off, hi = T.off + T.n, sr = (long)T.nth * T.nz;
         const int nz = T.nz;
-        auto base = [&](int i) { return T.off + (long)(i < T.nr ? i : T.nr - 1) * sr; };
+        auto base = [&](int i) { return T.off + (long)(i < 0 ? 0 : i < T.nr ? i : T.nr - 1) * sr; };
+        const int ib = C.dir > 0 ? C.i0 : C.i1 - 1, np = C.i1 - C.i0;
+        CHECK(C.dir == 1 || C.dir == -1, "generic march direction");
         auto in = [&](long a) { CHECK(a >= lo && a < hi, "generic column access %ld outside [%ld,%ld)", a, lo, hi); };
         for (int tid = 0; tid < BLOCK; ++tid) {
             for (int m = 0; m < M; ++m) {
@@ -267,13 +269,17 @@ static void check_columns_generic(std::vector<SysGeom> g, int cpt, int splits) {
                 const int row = fdiv(cc, nz, T.inv_nz), k = cc - row * nz;
                 CHECK(row >= 0 && row < C.nj && k >= 0 && k < nz, "generic slab cell %d", cc);
                 const long coff = (long)(C.j0 + row) * nz + k;
-                in(base(C.i0 > 0 ? C.i0 - 1 : C.i0) + coff);
-                in(base(C.i0) + coff);
-                in(base(C.i0 + 1) + coff);
-                for (int i = C.i0; i < C.i1; ++i) {
-                    if (i + 1 < C.i1) {
-                        in(base(i + 2) + coff);
-                        in(base(i + 1) + coff);
+                in(base(ib - 1) + coff);
+                in(base(ib) + coff);
+                in(base(ib + 1) + coff);
+                for (int t = 0; t < np; ++t) {
+                    const int i = ib + C.dir * t;
+                    CHECK(i >= C.i0 && i < C.i1, "generic march plane");
+                    if (t + 1 < np) {
+                        const int nx = i + C.dir;
+                        in(base(i + 2 * C.dir) + coff);
+                        in(base(nx) + coff);
+                        in(base(C.dir > 0 ? nx : nx - 1) + coff);
                     }
                     if (!act) continue;
                     const int sl = (row + 1) * nz + k;
@@ -294,7 +300,7 @@ static void check_columns_generic(std::vector<SysGeom> g, int cpt, int splits) {
                 const int hslot = (hrow + 1) * nz + hk;
                 CHECK(hslot >= 0 && hslot < SP && (hslot >= nz || hslot < SW), "generic halo slot");
                 const long hoff = (long)(((C.j0 + hrow) % T.nth + T.nth) % T.nth) * nz + hk;
-                for (int i = C.i0; i <= C.i1; ++i) in(base(i) + hoff);
+                for (int i = C.i0 - 1; i <= C.i1; ++i) in(base(i) + hoff);
             }
         }
     }
