@@ -300,8 +300,9 @@ int eafp_batch_set_tiling(eafp_batch* b, int cells_per_thread, int blocks);
 
 /* The mixed solver's stencil as a column march: r_splits > 0 cuts every
  * system into columns of theta-rows (about 1024 cells per r-plane) marched
- * through the r-planes, each column in r_splits segments; 0 returns to the
- * tile kernels.  EAFP_ERR_UNSUPPORTED when a system's nz exceeds 256 or a
+ * through the r-planes, each column in r_splits segments (neighbouring
+ * segments march in opposite directions, so the planes at a boundary are
+ * mostly read from HBM once); 0 returns to the tile kernels.  EAFP_ERR_UNSUPPORTED when a system's nz exceeds 256 or a
  * system has fewer tiles than segments.  Kept across eafp_batch_set_tiling
  * while the new tiles allow it.  No reference counterpart. */
 int eafp_batch_set_columns(eafp_batch* b, int r_splits);
@@ -322,8 +323,8 @@ int eafp_batch_time_kernels(eafp_batch* b, int solver, int coef_format, int reps
 /* Autotuner: times every tile shape (cells per thread x blocks per CU) for
  * the solver's iteration kernels (solver as for eafp_batch_time_kernels)
  * and keeps the fastest; for the mixed solver it then times the
- * column-marching stencil at 1, 2, 4, 8 and 16
- * r-segments and keeps it if it is faster (eafp_batch_get_columns tells).
+ * column-marching stencil at 1, 2, 4, 6, 8, 10, 12 and 16 r-segments and
+ * keeps it if it is faster (eafp_batch_get_columns tells).
  * Returns the tile shape and the iteration time (ms).  No reference
  * counterpart. */
 int eafp_batch_autotune(eafp_batch* b, int solver, int coef_format, int reps, int* cells_per_thread,
