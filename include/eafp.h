@@ -302,9 +302,10 @@ int eafp_batch_set_tiling(eafp_batch* b, int cells_per_thread, int blocks);
  * system into columns of theta-rows (about 1024 cells per r-plane) marched
  * through the r-planes, each column in r_splits segments (neighbouring
  * segments march in opposite directions, so the planes at a boundary are
- * mostly read from HBM once); 0 returns to the tile kernels.  EAFP_ERR_UNSUPPORTED when a system's nz exceeds 256 or a
- * system has fewer tiles than segments.  Kept across eafp_batch_set_tiling
- * while the new tiles allow it.  No reference counterpart. */
+ * mostly read from HBM once); 0 returns to the tile kernels.
+ * EAFP_ERR_UNSUPPORTED when a system's nz exceeds 256 or a system has fewer
+ * tiles than segments.  Kept across eafp_batch_set_tiling while the new
+ * tiles allow it.  No reference counterpart. */
 int eafp_batch_set_columns(eafp_batch* b, int r_splits);
 
 /* Column setting in use (0: tile kernels). */
