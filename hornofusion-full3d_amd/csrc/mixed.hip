@@ -1474,29 +1474,13 @@ struct LineCells {
     }
 };
 
-// Line tile of a k_mupdate_line block.  The grid is nlt rounded up to a
-// multiple of NXCD; the blocks of XCD x (blockIdx % NXCD == x) take the x-th
-// contiguous chunk of the line tiles from its end down.  The column stencil
-// walks its chunks of the same systems upwards, so each kernel begins on
-// what the other wrote last, which the memory-side cache may still hold.
-// -1: a padding block.
-__host__ __device__ __forceinline__ int line_grid(int nlt) { return (nlt + NXCD - 1) / NXCD * NXCD; }
-__device__ __forceinline__ int line_tile_down(int nlt) {
-    const int per = (int)gridDim.x / NXCD;
-    const int x = (int)blockIdx.x % NXCD, k = (int)blockIdx.x / NXCD;
-    const int lt = (x + 1) * per - 1 - k;
-    return lt < nlt ? lt : -1;
-}
-
 // k_mupdate with the z-line preconditioner: one line tile per block.
 template <bool Q>
-__global__ __launch_bounds__(BLOCK) void k_mupdate_line(MxArgs a, int par, int nlt) {
+__global__ __launch_bounds__(BLOCK) void k_mupdate_line(MxArgs a, int par) {
     __shared__ double sh[2 * (BLOCK / 64)];
     __shared__ LineLds S;
     __shared__ __align__(16) float G[Q ? 1 : 5][Q ? 4 : LINE_BUF];   // generic form: d, p, r, q, factors of the run
-    const int lt = line_tile_down(nlt);
-    if (lt < 0) return;
-    const LineTile L = a.lt[lt];
+    const LineTile L = a.lt[blockIdx.x];
     const int s = L.sys;
     const LineCells<Q> C(L);
     F4 dv, pvv, rv, qv, fv;
@@ -1536,7 +1520,7 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate_line(MxArgs a, int par, int n
             N.total_inner = P.total_inner + 1;
         }
     }
-    if (lt == L.lt_begin && threadIdx.x == 0) store_state(&a.dyn[1][s], N);
+    if ((int)blockIdx.x == L.lt_begin && threadIdx.x == 0) store_state(&a.dyn[1][s], N);
     if (skip) return;   // uniform over the block
     if (!Q) {
         dv = C.from(G[0]);
@@ -1585,7 +1569,7 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate_line(MxArgs a, int par, int n
         C.unstage(a.z, G[2], false);
     }
     block_sum<2>(acc, sh);
-    store_run<2>(a.partB, lt, 1, 1, acc);
+    store_run<2>(a.partB, (int)blockIdx.x, 1, 1, acc);
 }
 
 // k_mreplace with the z-line preconditioner: one line tile per block.
@@ -1916,9 +1900,8 @@ static void launch_mspmv(Batch& B, int fmt, int par) {
 
 static void launch_mupdate(Batch& B, int par) {
     if (B.mixed->args.line) {
-        const int g = line_grid(B.nltiles);
-        if (B.quad) k_mupdate_line<true><<<g, BLOCK, 0, B.stream>>>(B.mixed->args, par, B.nltiles);
-        else k_mupdate_line<false><<<g, BLOCK, 0, B.stream>>>(B.mixed->args, par, B.nltiles);
+        if (B.quad) k_mupdate_line<true><<<B.nltiles, BLOCK, 0, B.stream>>>(B.mixed->args, par);
+        else k_mupdate_line<false><<<B.nltiles, BLOCK, 0, B.stream>>>(B.mixed->args, par);
         return;
     }
     if (B.quad && B.cpt >= 4) {

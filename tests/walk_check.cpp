@@ -413,24 +413,7 @@ static SysGeom sys(int nr, int nth, int nz) {
     return s;
 }
 
-// k_mupdate_line's block -> line tile map (line_grid, line_tile_down in
-// mixed.hip): the padded grid covers every line tile exactly once.
-static void check_line_order(int nlt) {
-    const int G = (nlt + NXCD - 1) / NXCD * NXCD, per = G / NXCD;
-    std::vector<int> seen(nlt, 0);
-    for (int b = 0; b < G; ++b) {
-        const int x = b % NXCD, k = b / NXCD;
-        const int lt = (x + 1) * per - 1 - k;
-        CHECK(lt >= x * per && lt < (x + 1) * per, "line tile %d outside its XCD chunk", lt);
-        if (lt < nlt) ++seen[lt];
-    }
-    for (int t = 0; t < nlt; ++t) CHECK(seen[t] == 1, "line tile %d taken %d times", t, seen[t]);
-    ++g_cases;
-}
-
 int main() {
-    for (int nlt = 1; nlt <= 200; ++nlt) check_line_order(nlt);
-    for (int nlt : {4800, 38400, 38401, 38407}) check_line_order(nlt);
     std::vector<std::vector<SysGeom>> batches = {
         {sys(60, 120, 84)},                                           // the production p' system
         {sys(1, 1, 1)},                                               // one cell
