@@ -31,10 +31,12 @@ iteration-by-iteration history is "parity unpinned" beyond these checks.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import math
 import os
 import subprocess
+import tempfile
 
 import numpy as np
 
@@ -58,34 +60,63 @@ def lib():
     global _lib
     if _lib is None:
         build()
-        L = ctypes.CDLL(_LIB_PATH)
-        i = ctypes.c_int
-        L.oracle_cg.argtypes = [i, i, i] + [_dp] * 8 + [_dp, i, ctypes.c_double, _dp,
-                                                         ctypes.POINTER(ctypes.c_int), _dp]
-        L.oracle_cg.restype = ctypes.c_int
-        L.oracle_matvec.argtypes = [i, i, i] + [_dp] * 7 + [_dp, _dp]
-        L.oracle_matvec.restype = None
-        L.oracle_theta_halo.argtypes = [i, i, i, _dp]
-        L.oracle_theta_halo.restype = None
-        L.oracle_dot.argtypes = [i, i, i, _dp, _dp]
-        L.oracle_dot.restype = ctypes.c_double
-        L.oracle_true_residual.argtypes = [i, i, i] + [_dp] * 9 + [_dp, _dp]
-        L.oracle_true_residual.restype = None
-        L.oracle_pp_assemble.argtypes = [i, i, i, ctypes.c_void_p, ctypes.c_void_p] + [_dp] * 8 + [_dp, _dp]
-        L.oracle_pp_assemble.restype = ctypes.c_int
-        L.oracle_pp_assemble_ex.argtypes = [i, i, i, ctypes.c_void_p, ctypes.c_void_p] + [_dp] * 8 + \
-            [_dp, _dp, ctypes.c_void_p]
-        L.oracle_pp_assemble_ex.restype = ctypes.c_int
-        ip = ctypes.POINTER(ctypes.c_int)
-        L.oracle_pp_correct.argtypes = [i, i, i, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, _dp,
-                                        ctypes.c_void_p, _dp, _dp, _dp, ip, ip, ip, ctypes.POINTER(_dp), _dp]
-        L.oracle_pp_correct.restype = ctypes.c_int
-        L.oracle_tdma.argtypes = [_dp] * 5 + [i]
-        L.oracle_tdma.restype = ctypes.c_int
-        L.oracle_tdma_3d.argtypes = [i, i, i] + [_dp] * 9 + [i]
-        L.oracle_tdma_3d.restype = ctypes.c_int
-        _lib = L
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+def _bind(L):
+    """Declares the argument and result types of every oracle function."""
+    i = ctypes.c_int
+    L.oracle_cg.argtypes = [i, i, i] + [_dp] * 8 + [_dp, i, ctypes.c_double, _dp,
+                                                     ctypes.POINTER(ctypes.c_int), _dp]
+    L.oracle_cg.restype = ctypes.c_int
+    L.oracle_matvec.argtypes = [i, i, i] + [_dp] * 7 + [_dp, _dp]
+    L.oracle_matvec.restype = None
+    L.oracle_theta_halo.argtypes = [i, i, i, _dp]
+    L.oracle_theta_halo.restype = None
+    L.oracle_dot.argtypes = [i, i, i, _dp, _dp]
+    L.oracle_dot.restype = ctypes.c_double
+    L.oracle_true_residual.argtypes = [i, i, i] + [_dp] * 9 + [_dp, _dp]
+    L.oracle_true_residual.restype = None
+    L.oracle_pp_assemble.argtypes = [i, i, i, ctypes.c_void_p, ctypes.c_void_p] + [_dp] * 8 + [_dp, _dp]
+    L.oracle_pp_assemble.restype = ctypes.c_int
+    L.oracle_pp_assemble_ex.argtypes = [i, i, i, ctypes.c_void_p, ctypes.c_void_p] + [_dp] * 8 + \
+        [_dp, _dp, ctypes.c_void_p]
+    L.oracle_pp_assemble_ex.restype = ctypes.c_int
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.oracle_pp_correct.argtypes = [i, i, i, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, _dp,
+                                    ctypes.c_void_p, _dp, _dp, _dp, ip, ip, ip, ctypes.POINTER(_dp), _dp]
+    L.oracle_pp_correct.restype = ctypes.c_int
+    L.oracle_tdma.argtypes = [_dp] * 5 + [i]
+    L.oracle_tdma.restype = ctypes.c_int
+    L.oracle_tdma_3d.argtypes = [i, i, i] + [_dp] * 9 + [i]
+    L.oracle_tdma_3d.restype = ctypes.c_int
+    return L
+
+
+_PLANT_DIR = None
+
+
+@contextlib.contextmanager
+def planted(n: int):
+    """Within the block, every oracle call goes to the C sources rebuilt
+    with ``-DORACLE_PLANT=n``: mistake ``n`` planted (the hooks are listed in
+    ``pp_assemble.c`` and ``cg_oracle.c``).  The pinning tests use it to show
+    that each of their checks fails on the mistake it is meant to catch."""
+    global _lib, _PLANT_DIR
+    if _PLANT_DIR is None:
+        _PLANT_DIR = tempfile.mkdtemp(prefix="oracle_plant_")
+    path = os.path.join(_PLANT_DIR, f"liboracle_plant{int(n)}.so")
+    if not os.path.exists(path):
+        srcs = [os.path.join(_HERE, f) for f in ("cg_oracle.c", "pp_assemble.c", "tdma.c")]
+        subprocess.run(["gcc", "-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-shared",
+                        f"-DORACLE_PLANT={int(n)}", "-o", path] + srcs + ["-lm"], check=True)
+    saved = lib()
+    _lib = _bind(ctypes.CDLL(path))
+    try:
+        yield
+    finally:
+        _lib = saved
 
 
 def lib_omp():

@@ -22,13 +22,18 @@
  *                      breakdown exit, recurrence-residual stop, final theta
  *                      halo copy of phi)
  *   oracle_true_residual  no reference counterpart: ||b - A x||_2/||b||_2 and
- *                      the L1 form sum|b - A x| / sum|b| for reporting.
+ *                      the L1 form sum|b - A x| / sum|b| (the form of
+ *                      res_cont, src/mod_pressure_3d.f90:293-306) for
+ *                      reporting, with A x as matvec forms it.
  *
  * Build with -ffp-contract=off so that every multiply and add rounds on its
  * own (no FMA), which gives the plain sequential evaluation order.
  * Parity of this file is pinned by tests/test_oracle.py (scipy direct solve,
  * manufactured solutions, an independent numpy CSR operator, the closed
- * form A*1 = aP - sum(a_nb), invariants) -- see the test docstrings.
+ * form A*1 = aP - sum(a_nb), invariants) -- see the test docstrings -- and,
+ * for oracle_true_residual, by tests/test_oracle_pins.py (a scipy CSR
+ * residual with the wall-link halo terms, stale theta halos, the L1 form,
+ * and two planted mistakes that must fail).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -203,8 +208,23 @@ int oracle_cg(int nr, int nth, int nz,
     return 0;
 }
 
-/* ||b - A x||_2 / ||b||_2 and sum|b - A x| / sum|b| over the interior.
- * x's halos are used as they are (no theta copy is made here). */
+/* Mutation hooks of the pinning tests (tests/test_oracle_pins.py rebuilds
+ * this file with -DORACLE_PLANT=n and shows its checks fail there); every
+ * other build has ORACLE_PLANT 0, the function as written.
+ *   9: the theta neighbours of j = 1 and nth read x's stored theta halos
+ *  10: the r and z halos of x are taken as 0 */
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
+
+/* ||b - A x||_2 / ||b||_2 and sum|b - A x| / sum|b| over the interior, with
+ * A x formed as the reference's matvec forms it (src/mod_solver_3d.f90:
+ * 588-605, after the halo exchange of x):
+ *  - the theta neighbours of j = 1 and j = nth are the periodic cells j = nth
+ *    and j = 1, whatever x's theta halo planes hold (the serial halo copy,
+ *    src/mod_mpi_topology.f90:303-310, that matvec makes before its sweep);
+ *  - the r and z halos of x are used as they are, each weighted by its wall
+ *    link (aW at i = 1, aE at i = nr, aB at k = 1, aT at k = nz). */
 void oracle_true_residual(int nr, int nth, int nz,
                           const double* aW, const double* aE, const double* aS, const double* aN,
                           const double* aB, const double* aT, const double* aP, const double* Su,
@@ -215,10 +235,22 @@ void oracle_true_residual(int nr, int nth, int nz,
         for (int j = 1; j <= nth; ++j)
             for (int i = 1; i <= nr; ++i) {
                 long o = off(&g, i, j, k);
+                int js = j - 1, jn = j + 1;
+                if (ORACLE_PLANT != 9) {
+                    if (js < 1) js = nth;
+                    if (jn > nth) jn = 1;
+                }
+                double xw = x[o - g.sx], xe = x[o + g.sx], xb = x[o - g.sz], xt = x[o + g.sz];
+                if (ORACLE_PLANT == 10) {
+                    if (i == 1) xw = 0.0;
+                    if (i == nr) xe = 0.0;
+                    if (k == 1) xb = 0.0;
+                    if (k == nz) xt = 0.0;
+                }
                 double ax = aP[o] * x[o]
-                          - aW[o] * x[o - g.sx] - aE[o] * x[o + g.sx]
-                          - aS[o] * x[o - g.sy] - aN[o] * x[o + g.sy]
-                          - aB[o] * x[o - g.sz] - aT[o] * x[o + g.sz];
+                          - aW[o] * xw - aE[o] * xe
+                          - aS[o] * x[off(&g, i, js, k)] - aN[o] * x[off(&g, i, jn, k)]
+                          - aB[o] * xb - aT[o] * xt;
                 double rr = Su[o] - ax;
                 s2 += rr * rr; b2 += Su[o] * Su[o];
                 s1 += fabs(rr); b1 += fabs(Su[o]);

@@ -39,11 +39,25 @@
  * The expressions keep the reference's operand order (left to right, as
  * written); built with -ffp-contract=off.
  *
- * Parity of this file is pinned by tests/test_pp_oracle.py: face
- * coefficient symmetry, telescoping of the face fluxes (sum of Su over a
- * closed domain), the rest state (zero velocity with a pressure gradient the
- * cell and face differences agree on gives Su = 0), the closed form of a
- * uniform single-phase mesh, the diagonal formula, and the boundary rows.
+ * Parity of this file is pinned by two test modules:
+ *   tests/test_pp_oracle.py   face coefficients equal from both sides,
+ *                             telescoping face fluxes (sum of Su over a
+ *                             closed domain), the rest state (zero velocity
+ *                             with a pressure gradient the cell and face
+ *                             differences agree on gives Su = 0), the
+ *                             diagonal bound and the boundary rows; for the
+ *                             post-solve steps, balanced conservative fluxes,
+ *                             the closed-form velocity change of a linear p'
+ *                             and the clamped update / sealed cells;
+ *   tests/test_oracle_pins.py the face coefficients of a single-phase mesh
+ *                             against their closed form from the cell
+ *                             geometry, the exact diagonal (compliance, gas
+ *                             and liquid acoustic terms), and the gas
+ *                             compressibility source of a uniform heating or
+ *                             cooling step with its cap and T_MIN_GAS floor;
+ *                             each check is also run on this file rebuilt
+ *                             with a planted mistake (ORACLE_PLANT below) and
+ *                             must fail there.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -59,6 +73,32 @@
 #define C_SOUND_LIQ 4000.0            /* src/mod_pressure_3d.f90:67 */
 #define ROOM_EPS 1.0e-4               /* src/mod_pressure_3d.f90:663 */
 #define P_HYDRO_CAP 2.0e6             /* src/mod_pressure_3d.f90:74 */
+
+/* Mutation hooks of the pinning tests: tests/test_oracle_pins.py rebuilds
+ * this file with -DORACLE_PLANT=n to show that each of its checks fails on
+ * a plausible mistake.  Every other build leaves ORACLE_PLANT at 0, where
+ * each hook is the restatement as written (same operations, same bits). */
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
+/* 1: harmonic instead of arithmetic mean of alpha V / aP_u on the r faces */
+#if ORACLE_PLANT == 1
+#define D_MEAN_R(a, b) (2.0 * (a) * (b) / ((a) + (b)))
+#else
+#define D_MEAN_R(a, b) (0.5 * ((a) + (b)))
+#endif
+/* 2: the own cell's alpha instead of the face average on the z faces */
+#define ALPHA_MEAN_Z(a, b) (ORACLE_PLANT == 2 ? (a) : 0.5 * ((a) + (b)))
+/* 3: the own cell's r area on the W face */
+#define AREA_W(own, nb) (ORACLE_PLANT == 3 ? (own) : (nb))
+/* 4: no compliance factor; 5: the gas acoustic term without 1/dt;
+ * 6: c instead of c^2 in the liquid acoustic term */
+#define COMPLIANCE (ORACLE_PLANT == 4 ? 0.0 : PP_COMPLIANCE)
+#define ACOUSTIC_DT(dt) (ORACLE_PLANT == 5 ? 1.0 : (dt))
+#define C2_LIQ (ORACLE_PLANT == 6 ? C_SOUND_LIQ : C_SOUND_LIQ * C_SOUND_LIQ)
+/* 7: the compressibility source without its cap; 8: without the T_MIN_GAS floor */
+#define COMP_CLAMP(cap, src) (ORACLE_PLANT == 7 ? (src) : fmax(-(cap), fmin((cap), (src))))
+#define T_FLOOR (ORACLE_PLANT == 8 ? 0.0 : T_MIN_GAS)
 
 /* Same layout as eafp_pp_fields / eafp_pp_config in include/eafp.h. */
 typedef struct {
@@ -188,18 +228,18 @@ static void add_phase(const ctx_t* x, const phase_t* ph, double* aW, double* aE,
                 const long oB = O(g, i, j, k - 1), oT = O(g, i, j, k + 1);
                 double d_f, af, u_f, delta;
                 if (link(x, ph->act, oW)) {
-                    d_f = 0.5 * (aef[o] * f->vol[o] / fmax(ph->aPr[o], SMALL) + aef[oW] * f->vol[oW] / fmax(ph->aPr[oW], SMALL));
+                    d_f = D_MEAN_R(aef[o] * f->vol[o] / fmax(ph->aPr[o], SMALL), aef[oW] * f->vol[oW] / fmax(ph->aPr[oW], SMALL));
                     af = 0.5 * (aef[o] + aef[oW]);
                     delta = f->r[i + 1] - f->r[i];
                     u_f = 0.5 * (ph->ur[oW] + ph->ur[o]) + d_f * (0.5 * (gr[oW] + gr[o]) - (pf[o] - pf[oW]) / delta);
                     if (face_open(x, ph->isg, u_f, oW, o)) {
-                        aW[o] = aW[o] + af * d_f * f->Ar[oW] / delta;
+                        aW[o] = aW[o] + af * d_f * f->Ar[AREA_W(o, oW)] / delta;
                         Su[o] = Su[o] + af * u_f * f->Ar[oW];
                         *flux_ref = *flux_ref + fabs(af * u_f * f->Ar[oW]);
                     }
                 }
                 if (link(x, ph->act, oE)) {
-                    d_f = 0.5 * (aef[o] * f->vol[o] / fmax(ph->aPr[o], SMALL) + aef[oE] * f->vol[oE] / fmax(ph->aPr[oE], SMALL));
+                    d_f = D_MEAN_R(aef[o] * f->vol[o] / fmax(ph->aPr[o], SMALL), aef[oE] * f->vol[oE] / fmax(ph->aPr[oE], SMALL));
                     af = 0.5 * (aef[o] + aef[oE]);
                     delta = f->r[i + 2] - f->r[i + 1];
                     u_f = 0.5 * (ph->ur[o] + ph->ur[oE]) + d_f * (0.5 * (gr[o] + gr[oE]) - (pf[oE] - pf[o]) / delta);
@@ -243,7 +283,7 @@ static void add_phase(const ctx_t* x, const phase_t* ph, double* aW, double* aE,
                 }
                 if (link(x, ph->act, oB)) {
                     d_f = 0.5 * (aef[o] * f->vol[o] / fmax(ph->aPz[o], SMALL) + aef[oB] * f->vol[oB] / fmax(ph->aPz[oB], SMALL));
-                    af = 0.5 * (aef[o] + aef[oB]);
+                    af = ALPHA_MEAN_Z(aef[o], aef[oB]);
                     delta = f->z[k + 1] - f->z[k];
                     u_f = 0.5 * (ph->uz[oB] + ph->uz[o]) +
                           d_f * (0.5 * (gz[oB] + gz[o]) - (pf[o] - pf[oB]) / delta -
@@ -256,7 +296,7 @@ static void add_phase(const ctx_t* x, const phase_t* ph, double* aW, double* aE,
                 }
                 if (link(x, ph->act, oT)) {
                     d_f = 0.5 * (aef[o] * f->vol[o] / fmax(ph->aPz[o], SMALL) + aef[oT] * f->vol[oT] / fmax(ph->aPz[oT], SMALL));
-                    af = 0.5 * (aef[o] + aef[oT]);
+                    af = ALPHA_MEAN_Z(aef[o], aef[oT]);
                     delta = f->z[k + 2] - f->z[k + 1];
                     u_f = 0.5 * (ph->uz[o] + ph->uz[oT]) +
                           d_f * (0.5 * (gz[o] + gz[oT]) - (pf[oT] - pf[o]) / delta -
@@ -339,10 +379,10 @@ int oracle_pp_assemble_ex(int nr, int nth, int nz, const pp_fields* f, const pp_
                     const long o = O(&g, i, j, k);
                     if (f->cell_type[o] == 0 || !x.act_g[o]) continue;
                     const double src = f->alpha_g[o] * f->vol[o] / c->dt *
-                                       (f->rho_g[o] - c->rho_gas * c->T_ambient / fmax(f->T_old_g[o], T_MIN_GAS)) /
+                                       (f->rho_g[o] - c->rho_gas * c->T_ambient / fmax(f->T_old_g[o], T_FLOOR)) /
                                        fmax(f->rho_g[o], SMALL);
                     const double cap = COMP_SRC_CAP * f->alpha_g[o] * f->vol[o] / c->dt;
-                    Su[o] = Su[o] - fmax(-cap, fmin(cap, src));
+                    Su[o] = Su[o] - COMP_CLAMP(cap, src);
                 }
     }
     for (int k = 1; k <= nz; ++k)
@@ -354,10 +394,10 @@ int oracle_pp_assemble_ex(int nr, int nth, int nz, const pp_fields* f, const pp_
                     aW[o] = 0.0; aE[o] = 0.0; aS[o] = 0.0; aN[o] = 0.0; aB[o] = 0.0; aT[o] = 0.0;
                     aP[o] = 1.0; Su[o] = 0.0;
                 } else {
-                    aP[o] = (aW[o] + aE[o] + aS[o] + aN[o] + aB[o] + aT[o]) * (1.0 + PP_COMPLIANCE);
-                    if (compress && x.act_g[o]) aP[o] = aP[o] + f->alpha_g[o] / P0_THERMO * f->vol[o] / c->dt;
+                    aP[o] = (aW[o] + aE[o] + aS[o] + aN[o] + aB[o] + aT[o]) * (1.0 + COMPLIANCE);
+                    if (compress && x.act_g[o]) aP[o] = aP[o] + f->alpha_g[o] / P0_THERMO * f->vol[o] / ACOUSTIC_DT(c->dt);
                     if (x.act_l[o])
-                        aP[o] = aP[o] + f->alpha_l[o] * f->vol[o] / (f->rho_l[o] * (C_SOUND_LIQ * C_SOUND_LIQ) * c->dt);
+                        aP[o] = aP[o] + f->alpha_l[o] * f->vol[o] / (f->rho_l[o] * C2_LIQ * c->dt);
                 }
             }
     /* src/mod_boundary_3d.f90:180-277 (serial: every boundary is physical) */
