@@ -54,7 +54,9 @@
  *                             geometry, the exact diagonal (compliance, gas
  *                             and liquid acoustic terms), and the gas
  *                             compressibility source of a uniform heating or
- *                             cooling step with its cap and T_MIN_GAS floor;
+ *                             cooling step with its cap and T_MIN_GAS floor,
+ *                             and the Rhie-Chow pressure term of u_f for a
+ *                             pressure that alternates along r;
  *                             each check is also run on this file rebuilt
  *                             with a planted mistake (ORACLE_PLANT below) and
  *                             must fail there.
@@ -99,6 +101,8 @@
 /* 7: the compressibility source without its cap; 8: without the T_MIN_GAS floor */
 #define COMP_CLAMP(cap, src) (ORACLE_PLANT == 7 ? (src) : fmax(-(cap), fmin((cap), (src))))
 #define T_FLOOR (ORACLE_PLANT == 8 ? 0.0 : T_MIN_GAS)
+/* 11: the Rhie-Chow term of the r faces' u_f without its d_f */
+#define RC_D(d) (ORACLE_PLANT == 11 ? 1.0 : (d))
 
 /* Same layout as eafp_pp_fields / eafp_pp_config in include/eafp.h. */
 typedef struct {
@@ -231,7 +235,7 @@ static void add_phase(const ctx_t* x, const phase_t* ph, double* aW, double* aE,
                     d_f = D_MEAN_R(aef[o] * f->vol[o] / fmax(ph->aPr[o], SMALL), aef[oW] * f->vol[oW] / fmax(ph->aPr[oW], SMALL));
                     af = 0.5 * (aef[o] + aef[oW]);
                     delta = f->r[i + 1] - f->r[i];
-                    u_f = 0.5 * (ph->ur[oW] + ph->ur[o]) + d_f * (0.5 * (gr[oW] + gr[o]) - (pf[o] - pf[oW]) / delta);
+                    u_f = 0.5 * (ph->ur[oW] + ph->ur[o]) + RC_D(d_f) * (0.5 * (gr[oW] + gr[o]) - (pf[o] - pf[oW]) / delta);
                     if (face_open(x, ph->isg, u_f, oW, o)) {
                         aW[o] = aW[o] + af * d_f * f->Ar[AREA_W(o, oW)] / delta;
                         Su[o] = Su[o] + af * u_f * f->Ar[oW];
@@ -242,7 +246,7 @@ static void add_phase(const ctx_t* x, const phase_t* ph, double* aW, double* aE,
                     d_f = D_MEAN_R(aef[o] * f->vol[o] / fmax(ph->aPr[o], SMALL), aef[oE] * f->vol[oE] / fmax(ph->aPr[oE], SMALL));
                     af = 0.5 * (aef[o] + aef[oE]);
                     delta = f->r[i + 2] - f->r[i + 1];
-                    u_f = 0.5 * (ph->ur[o] + ph->ur[oE]) + d_f * (0.5 * (gr[o] + gr[oE]) - (pf[oE] - pf[o]) / delta);
+                    u_f = 0.5 * (ph->ur[o] + ph->ur[oE]) + RC_D(d_f) * (0.5 * (gr[o] + gr[oE]) - (pf[oE] - pf[o]) / delta);
                     if (face_open(x, ph->isg, u_f, o, oE)) {
                         aE[o] = aE[o] + af * d_f * f->Ar[o] / delta;
                         Su[o] = Su[o] - af * u_f * f->Ar[o];

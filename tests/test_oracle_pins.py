@@ -363,6 +363,54 @@ def test_compressibility_planted_mistakes_fail(plant):
         assert not _comp_ok(F, cfg)
 
 
+# ---------------------------------------------------------------------------
+# The Rhie-Chow pressure term of u_f (src/mod_pressure_3d.f90:496-530)
+# ---------------------------------------------------------------------------
+
+def _alternating_case():
+    """The single-phase case at rest with p = 1e5 + 50 (-1)^i: the central
+    cell gradients of an r-alternating pressure vanish, so on the faces of a
+    cell whose two r-neighbours on each side are fluid, u_f reduces to
+    -d_f (p_N - p_P)/delta and each r face adds -a_f (p_P - p_N) to Su;
+    theta and z faces see no pressure difference."""
+    F, cfg = _single_phase_case()
+    for n in ("ur_l", "uth_l", "uz_l", "ur_g", "uth_g", "uz_g"):
+        F[n] = np.zeros_like(F[n], order="F")
+    ii = np.indices(F["p"].shape)[0] - 1
+    F["p"] = np.asfortranarray(1e5 + 50.0 * np.where(ii % 2 == 0, 1.0, -1.0))
+    return F, cfg
+
+
+def _rc_check(F, cfg):
+    out, _, _ = oracle.pp_assemble(F, cfg, SHAPE)
+    nr = SHAPE[0]
+    fl = (F["cell_type"] != 0) & (F["alpha_l"] >= 1e-2)
+    sel = np.zeros(SHAPE, dtype=bool)
+    for i in range(3, nr - 1):          # Fortran i; numpy index i + 1 of the padded arrays
+        sel[i - 1] = fl[i - 1:i + 4].all(axis=0)[2:-2, 2:-2]
+    p = interior(F["p"])
+    dp = p - np.roll(p, 1, axis=0)      # p_P - p_W (p_P - p_E is the same for an alternating p)
+    want = -(interior(out["aW"]) + interior(out["aE"])) * dp
+    got = interior(out["Su"])
+    return sel, got, want
+
+
+def test_rhie_chow_pressure_term():
+    F, cfg = _alternating_case()
+    sel, got, want = _rc_check(F, cfg)
+    assert sel.sum() > 100
+    scale = np.abs(want[sel]).max()
+    assert scale > 0
+    np.testing.assert_allclose(got[sel], want[sel], rtol=0, atol=1e-10 * scale)
+
+
+def test_rhie_chow_planted_mistake_fails():
+    F, cfg = _alternating_case()
+    with oracle.planted(11):
+        sel, got, want = _rc_check(F, cfg)
+    assert np.abs(got[sel] - want[sel]).max() > 1e-3 * np.abs(want[sel]).max()
+
+
 def test_unplanted_rebuild_matches_the_oracle():
     """ORACLE_PLANT=0 is the oracle as built: the same bits."""
     F, cfg = _two_phase_case()
