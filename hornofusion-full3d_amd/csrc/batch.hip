@@ -193,138 +193,242 @@ __global__ void k_pack(PackArgs a, SysGeom g) {
 }
 
 // ---------------------------------------------------------------------------
-// Device pre-pass of the singular-component analysis.  A component of the
-// link graph is singular when every one of its rows is a Neumann row (the
-// rule of analyze_components below); a linked row that is not a Neumann row
-// anchors its component.  k_comp_init stores per compact cell the six link
-// bits (a link exists when either side's coefficient is nonzero, theta
-// periodic, no link across the r/z walls) and the anchor bit; k_comp_sweep
-// spreads the anchor bit along links until a sweep changes nothing.  Then
-// every linked cell is anchored exactly when the system has no singular
-// component, and only otherwise does the host analysis run.
+// Singular components, labelled on the device.  A component of the link
+// graph (a link exists where either side's coefficient is nonzero; theta
+// periodic; none across the r/z walls) is singular -- its p' is fixed only
+// up to a constant -- when every one of its rows is a Neumann row: aP equal
+// to the sum of the row's interior links within 1e-10 relative, and no wall
+// link.  The mixed solver projects its residuals and its result to zero
+// mean on each singular component; rows without any link are never
+// projected.  Kernels, one block per tile of the systems being labelled:
+//   k_lab_init    link bits, the anchor bit (a linked row that is not a
+//                 Neumann row), label = own cell index
+//   k_lab_sweep   label = min over the linked neighbours' labels, then two
+//                 pointer jumps (label = label of label: every label is the
+//                 index of a cell of the same component, at most the
+//                 labelled cell's own index); in place, until a sweep
+//                 changes nothing, when every cell holds the smallest cell
+//                 index of its component (its root)
+//   k_lab_anchor  anchored cells mark their root
+//   k_lab_roots   unmarked roots, the singular components, are listed
+//   k_lab_assign  with each system's list sorted by the host (the order of
+//                 first cells), a cell of a singular component takes its
+//                 root's slot and is counted
 // ---------------------------------------------------------------------------
-constexpr unsigned char LB_W = 1, LB_E = 2, LB_S = 4, LB_N = 8, LB_B = 16, LB_T = 32, LB_LINKS = 63, LB_ANCHOR = 64;
+constexpr unsigned char LB_W = 1, LB_E = 2, LB_S = 4, LB_N = 8, LB_B = 16, LB_T = 32, LB_LINKS = 63,
+                        LB_ANCHOR = 64, LB_ROOT_ANCHORED = 128;
+constexpr int ROOT_CAP = MAX_COMP + 1;
 
-struct CompArgs {
-    const double* A[7];   // padded aW..aT, aP (device)
-    unsigned char* bits;  // per compact cell of the system
-    int* flag;
+struct LabArgs {
+    const TileDesc* tiles;
+    int tile;                   // cells per tile
+    const double* A[7];         // compact arenas aW..aT, aP
+    unsigned char* bits;        // per arena cell
+    int* lab;                   // per arena cell: a cell index of the same system
+    const int* pending;         // per system: 1 = being labelled
+    int* changed;
+    int* nroots;                // per system
+    int* roots;                 // per system: ROOT_CAP root indices (sorted by the host before k_lab_assign)
+    int* counts;                // per system: MAX_COMP cell counts
+    unsigned char* comp;        // projected-component slot per arena cell
 };
 
-__global__ void k_comp_init(CompArgs a, SysGeom g) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= g.n) return;
-    const CellIdx id = decompose(c, g);
-    const int i = id.i, j = id.j, k = id.k;
-    const long sy = g.nr + 4, sz = (long)(g.nr + 4) * (g.nth + 4);
-    const long o = (i + 2) + sy * (j + 2) + sz * (k + 2);
-    const int jn = j + 1 == g.nth ? 0 : j + 1, jp = j == 0 ? g.nth - 1 : j - 1;
-    const long on = o + sy * (jn - j), op = o + sy * (jp - j);
-    const double *AW = a.A[C_AW], *AE = a.A[C_AE], *AS = a.A[C_AS], *AN = a.A[C_AN];
-    const double *AB = a.A[C_AB], *AT = a.A[C_AT];
-    unsigned char b = 0;
-    if (i > 0 && (AE[o - 1] != 0.0 || AW[o] != 0.0)) b |= LB_W;
-    if (i + 1 < g.nr && (AE[o] != 0.0 || AW[o + 1] != 0.0)) b |= LB_E;
-    if (AN[op] != 0.0 || AS[o] != 0.0) b |= LB_S;
-    if (AN[o] != 0.0 || AS[on] != 0.0) b |= LB_N;
-    if (k > 0 && (AT[o - sz] != 0.0 || AB[o] != 0.0)) b |= LB_B;
-    if (k + 1 < g.nz && (AT[o] != 0.0 || AB[o + sz] != 0.0)) b |= LB_T;
-    // the Neumann test of analyze_components, in its order of additions
-    double links = 0.0;
-    if (i > 0) links += AW[o];
-    if (i + 1 < g.nr) links += AE[o];
-    links += AS[o] + AN[o];
-    if (k > 0) links += AB[o];
-    if (k + 1 < g.nz) links += AT[o];
-    const double ap = a.A[C_AP][o];
-    const bool wall = (i == 0 && AW[o] != 0.0) || (i + 1 == g.nr && AE[o] != 0.0) || (k == 0 && AB[o] != 0.0) ||
-                      (k + 1 == g.nz && AT[o] != 0.0);
-    const bool neumann = !wall && fabs(ap - links) <= 1e-10 * fabs(ap);
-    if (b != 0 && !neumann) b |= LB_ANCHOR;
-    a.bits[c] = b;
-}
+struct LabCell {
+    long o;                     // arena index
+    int c;                      // cell index in the system
+};
 
-// One anchor-spreading sweep (in place: a sweep may also use what it set).
-// flag: NULL, or set to 1 when the sweep anchors a cell.
-__global__ void k_comp_sweep(unsigned char* bits, SysGeom g, int* flag) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= g.n) return;
-    const unsigned char b = bits[c];
-    if ((b & LB_LINKS) == 0 || (b & LB_ANCHOR)) return;
-    const CellIdx id = decompose(c, g);
-    const int plane = g.nth * g.nz;
-    const int cs = id.j == 0 ? c + (g.nth - 1) * g.nz : c - g.nz;
-    const int cn = id.j + 1 == g.nth ? c - (g.nth - 1) * g.nz : c + g.nz;
-    bool hit = false;
-    if (b & LB_W) hit |= (bits[c - plane] & LB_ANCHOR) != 0;
-    if (b & LB_E) hit |= (bits[c + plane] & LB_ANCHOR) != 0;
-    if (b & LB_S) hit |= (bits[cs] & LB_ANCHOR) != 0;
-    if (b & LB_N) hit |= (bits[cn] & LB_ANCHOR) != 0;
-    if (b & LB_B) hit |= (bits[c - 1] & LB_ANCHOR) != 0;
-    if (b & LB_T) hit |= (bits[c + 1] & LB_ANCHOR) != 0;
-    if (hit) {
-        bits[c] = b | LB_ANCHOR;
-        if (flag) *flag = 1;
+// The cells of block b's tile, when its system is being labelled.
+#define LAB_FOR_CELLS(a, T, ...)                                                   \
+    const TileDesc T = (a).tiles[blockIdx.x];                                       \
+    if (!(a).pending[T.sys]) return;                                                \
+    const int base_ = (blockIdx.x - T.tile_begin) * (a).tile;                       \
+    const int end_ = min(T.n, base_ + (a).tile);                                    \
+    for (int c = base_ + (int)threadIdx.x; c < end_; c += BLOCK) {                  \
+        const long o = T.off + c;                                                   \
+        __VA_ARGS__                                                                 \
     }
+
+__global__ __launch_bounds__(BLOCK) void k_lab_init(LabArgs a) {
+    LAB_FOR_CELLS(a, T, {
+        const int k = c % T.nz, l = c / T.nz;
+        const int j = l % T.nth, i = l / T.nth;
+        const long plane = (long)T.nth * T.nz;
+        const long oS = j == 0 ? o + (long)(T.nth - 1) * T.nz : o - T.nz;
+        const long oN = j + 1 == T.nth ? o - (long)(T.nth - 1) * T.nz : o + T.nz;
+        const double *AW = a.A[C_AW], *AE = a.A[C_AE], *AS = a.A[C_AS], *AN = a.A[C_AN];
+        const double *AB = a.A[C_AB], *AT = a.A[C_AT];
+        unsigned char b = 0;
+        if (i > 0 && (AE[o - plane] != 0.0 || AW[o] != 0.0)) b |= LB_W;
+        if (i + 1 < T.nr && (AE[o] != 0.0 || AW[o + plane] != 0.0)) b |= LB_E;
+        if (AN[oS] != 0.0 || AS[o] != 0.0) b |= LB_S;
+        if (AN[o] != 0.0 || AS[oN] != 0.0) b |= LB_N;
+        if (k > 0 && (AT[o - 1] != 0.0 || AB[o] != 0.0)) b |= LB_B;
+        if (k + 1 < T.nz && (AT[o] != 0.0 || AB[o + 1] != 0.0)) b |= LB_T;
+        double links = 0.0;
+        if (i > 0) links += AW[o];
+        if (i + 1 < T.nr) links += AE[o];
+        links += AS[o] + AN[o];
+        if (k > 0) links += AB[o];
+        if (k + 1 < T.nz) links += AT[o];
+        const double ap = a.A[C_AP][o];
+        const bool wall = (i == 0 && AW[o] != 0.0) || (i + 1 == T.nr && AE[o] != 0.0) || (k == 0 && AB[o] != 0.0) ||
+                          (k + 1 == T.nz && AT[o] != 0.0);
+        const bool neumann = !wall && fabs(ap - links) <= 1e-10 * fabs(ap);
+        if (b != 0 && !neumann) b |= LB_ANCHOR;
+        a.bits[o] = b;
+        a.lab[o] = c;
+    })
 }
 
-// flag = 1 when a linked cell is still unanchored (a singular component).
-__global__ void k_comp_unanchored(const unsigned char* bits, SysGeom g, int* flag) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= g.n) return;
-    const unsigned char b = bits[c];
-    if ((b & LB_LINKS) != 0 && !(b & LB_ANCHOR)) *flag = 1;
+__global__ __launch_bounds__(BLOCK) void k_lab_sweep(LabArgs a, int flag) {
+    LAB_FOR_CELLS(a, T, {
+        const unsigned char b = a.bits[o];
+        if ((b & LB_LINKS) == 0) continue;
+        const int j = (c / T.nz) % T.nth;
+        const long plane = (long)T.nth * T.nz;
+        const long oS = j == 0 ? o + (long)(T.nth - 1) * T.nz : o - T.nz;
+        const long oN = j + 1 == T.nth ? o - (long)(T.nth - 1) * T.nz : o + T.nz;
+        const int cur = a.lab[o];
+        int m = cur;
+        if (b & LB_W) m = min(m, a.lab[o - plane]);
+        if (b & LB_E) m = min(m, a.lab[o + plane]);
+        if (b & LB_S) m = min(m, a.lab[oS]);
+        if (b & LB_N) m = min(m, a.lab[oN]);
+        if (b & LB_B) m = min(m, a.lab[o - 1]);
+        if (b & LB_T) m = min(m, a.lab[o + 1]);
+        m = a.lab[T.off + m];
+        m = a.lab[T.off + m];
+        if (m < cur) {
+            a.lab[o] = m;
+            if (flag) *a.changed = 1;
+        }
+    })
 }
 
-__global__ void k_set_int(int* p, int v) { *p = v; }
-
-__global__ void k_fill_u8(unsigned char* p, int n, unsigned char v) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < n) p[c] = v;
+__global__ __launch_bounds__(BLOCK) void k_lab_anchor(LabArgs a) {
+    LAB_FOR_CELLS(a, T, {
+        const unsigned char b = a.bits[o];
+        // only this bit of a root's byte changes in this kernel
+        if ((b & LB_LINKS) && (b & LB_ANCHOR)) a.bits[T.off + a.lab[o]] |= LB_ROOT_ANCHORED;
+    })
 }
 
-// Returns EAFP_OK with *singular = 0 when the system has no singular
-// component (its slots are then set to NO_COMP), *singular = 1 otherwise.
-int Batch::device_components(int s, const double* const A[7], int* singular) {
-    const SysGeom& g = geom[s];
-    EAFP_TRY_RC(comp_bits.alloc(std::max<size_t>(comp_bits.n, (size_t)g.n)));
-    if (!comp_flag.p) EAFP_TRY_RC(comp_flag.alloc(1));
-    CompArgs a;
-    for (int q = 0; q < 7; ++q) a.A[q] = A[q];
-    a.bits = comp_bits.p;
-    a.flag = comp_flag.p;
-    const int blocks = (g.n + 255) / 256;
-    k_comp_init<<<blocks, 256, 0, stream>>>(a, g);
+__global__ __launch_bounds__(BLOCK) void k_lab_roots(LabArgs a) {
+    LAB_FOR_CELLS(a, T, {
+        const unsigned char b = a.bits[o];
+        if ((b & LB_LINKS) && !(b & LB_ROOT_ANCHORED) && a.lab[o] == c) {
+            const int q = atomicAdd(&a.nroots[T.sys], 1);
+            if (q < ROOT_CAP) a.roots[T.sys * ROOT_CAP + q] = c;
+        }
+    })
+}
+
+__global__ __launch_bounds__(BLOCK) void k_lab_assign(LabArgs a) {
+    LAB_FOR_CELLS(a, T, {
+        const int nr = a.nroots[T.sys];
+        unsigned char slot = NO_COMP;
+        if (nr <= MAX_COMP && (a.bits[o] & LB_LINKS)) {
+            const int r = a.lab[o];
+            if (!(a.bits[T.off + r] & LB_ROOT_ANCHORED)) {
+                for (int q = 0; q < MAX_COMP; ++q)
+                    if (q < nr && a.roots[T.sys * ROOT_CAP + q] == r) slot = (unsigned char)q;
+                if (slot != NO_COMP) atomicAdd(&a.counts[T.sys * MAX_COMP + slot], 1);
+            }
+        }
+        a.comp[o] = slot;
+    })
+}
+
+#undef LAB_FOR_CELLS
+
+// Labels the singular components of every uploaded system not labelled since
+// its upload (the mixed solver's need); fills comp and comp_info.
+int Batch::label_components() {
+    std::vector<int> pend(nsys, 0);
+    int npend = 0;
+    for (int s = 0; s < nsys; ++s)
+        if (uploaded[s] == 1) {
+            pend[s] = 1;
+            ++npend;
+        }
+    if (npend == 0) return EAFP_OK;
+    EAFP_TRY_RC(lab.alloc(ncells));
+    EAFP_TRY_RC(lab_bits.alloc(ncells));
+    EAFP_TRY_RC(lab_small.alloc((size_t)nsys * (2 + ROOT_CAP + MAX_COMP) + 1));
+    int* pending_d = lab_small.p;
+    int* nroots_d = pending_d + nsys;
+    int* roots_d = nroots_d + nsys;
+    int* counts_d = roots_d + (size_t)nsys * ROOT_CAP;
+    int* changed_d = counts_d + (size_t)nsys * MAX_COMP;
+    EAFP_TRY(hipMemcpyAsync(pending_d, pend.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, stream));
+    EAFP_TRY(hipMemsetAsync(nroots_d, 0, sizeof(int) * nsys, stream));
+    EAFP_TRY(hipMemsetAsync(counts_d, 0, sizeof(int) * nsys * MAX_COMP, stream));
+    LabArgs a;
+    a.tiles = tiles.p;
+    a.tile = BLOCK * cpt;
+    for (int q = 0; q < 7; ++q) a.A[q] = coef[q].p;
+    a.bits = lab_bits.p;
+    a.lab = lab.p;
+    a.pending = pending_d;
+    a.changed = changed_d;
+    a.nroots = nroots_d;
+    a.roots = roots_d;
+    a.counts = counts_d;
+    a.comp = comp.p;
+    k_lab_init<<<ntiles, BLOCK, 0, stream>>>(a);
     EAFP_TRY(hipGetLastError());
-    // Every sweep that changes something anchors at least one cell, so
-    // g.n / SWEEPS + 1 rounds always reach the fixed point.
-    constexpr int SWEEPS = 16;
+    // Labels only decrease, so the sweeps end; a check after every SWEEPS,
+    // whose last sweep alone raises the flag: a sweep that changes nothing
+    // is the fixed point.
+    constexpr int SWEEPS = 4;
     int changed = 1;
-    for (long round = 0; changed && round <= g.n / SWEEPS + 1; ++round) {
-        for (int q = 0; q < SWEEPS - 1; ++q) k_comp_sweep<<<blocks, 256, 0, stream>>>(comp_bits.p, g, nullptr);
-        k_set_int<<<1, 1, 0, stream>>>(comp_flag.p, 0);
-        k_comp_sweep<<<blocks, 256, 0, stream>>>(comp_bits.p, g, comp_flag.p);
+    long sweeps = 0;
+    while (changed) {
+        if (sweeps > 4L * ncells + 64) return EAFP_ERR_STATE;   // unreachable: each changing sweep lowers a label
+        for (int q = 0; q < SWEEPS - 1; ++q) k_lab_sweep<<<ntiles, BLOCK, 0, stream>>>(a, 0);
+        EAFP_TRY(hipMemsetAsync(changed_d, 0, sizeof(int), stream));
+        k_lab_sweep<<<ntiles, BLOCK, 0, stream>>>(a, 1);
         EAFP_TRY(hipGetLastError());
-        EAFP_TRY(hipMemcpyAsync(&changed, comp_flag.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+        EAFP_TRY(hipMemcpyAsync(&changed, changed_d, sizeof(int), hipMemcpyDeviceToHost, stream));
         EAFP_TRY(hipStreamSynchronize(stream));
+        sweeps += SWEEPS;
     }
-    if (changed) return EAFP_ERR_STATE;   // unreachable: the bound above is exact
-    int un = 0;
-    k_set_int<<<1, 1, 0, stream>>>(comp_flag.p, 0);
-    k_comp_unanchored<<<blocks, 256, 0, stream>>>(comp_bits.p, g, comp_flag.p);
+    label_sweeps = sweeps;
+    k_lab_anchor<<<ntiles, BLOCK, 0, stream>>>(a);
+    k_lab_roots<<<ntiles, BLOCK, 0, stream>>>(a);
     EAFP_TRY(hipGetLastError());
-    EAFP_TRY(hipMemcpyAsync(&un, comp_flag.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    std::vector<int> nroots(nsys), roots((size_t)nsys * ROOT_CAP);
+    EAFP_TRY(hipMemcpyAsync(nroots.data(), nroots_d, sizeof(int) * nsys, hipMemcpyDeviceToHost, stream));
+    EAFP_TRY(hipMemcpyAsync(roots.data(), roots_d, sizeof(int) * nsys * ROOT_CAP, hipMemcpyDeviceToHost, stream));
     EAFP_TRY(hipStreamSynchronize(stream));
-    *singular = un;
-    if (!un) {
-        k_fill_u8<<<blocks, 256, 0, stream>>>(comp.p + g.off, g.n, NO_COMP);
-        EAFP_TRY(hipGetLastError());
-        comp_info[s] = CompInfo{};
+    for (int s = 0; s < nsys; ++s)
+        if (pend[s] && nroots[s] <= MAX_COMP) std::sort(roots.begin() + (long)s * ROOT_CAP,
+                                                        roots.begin() + (long)s * ROOT_CAP + nroots[s]);
+    EAFP_TRY(hipMemcpyAsync(roots_d, roots.data(), sizeof(int) * nsys * ROOT_CAP, hipMemcpyHostToDevice, stream));
+    k_lab_assign<<<ntiles, BLOCK, 0, stream>>>(a);
+    EAFP_TRY(hipGetLastError());
+    std::vector<int> counts((size_t)nsys * MAX_COMP);
+    EAFP_TRY(hipMemcpyAsync(counts.data(), counts_d, sizeof(int) * nsys * MAX_COMP, hipMemcpyDeviceToHost, stream));
+    EAFP_TRY(hipStreamSynchronize(stream));
+    for (int s = 0; s < nsys; ++s) {
+        if (!pend[s]) continue;
+        CompInfo& info = comp_info[s];
+        info = CompInfo{};
+        info.overflow = nroots[s] > MAX_COMP;
+        info.ncomp = info.overflow ? 0 : nroots[s];
+        for (int q = 0; q < info.ncomp; ++q) {
+            info.cells[q] = counts[(size_t)s * MAX_COMP + q];
+            info.inv_cnt[q] = 1.0 / (double)info.cells[q];
+        }
+        info.nsingular = nroots[s];
+        uploaded[s] = 2;
     }
+    counter_add(CTR_LABELLED, npend);
     return EAFP_OK;
 }
 
-int Batch::upload(int s, const double* const host[8], const double* phi0, int on_device, bool components) {
+int Batch::upload(int s, const double* const host[8], const double* phi0, int on_device) {
     if (s < 0 || s >= nsys) return EAFP_ERR_STATE;
     for (int q = 0; q < 8; ++q)
         if (!host[q]) return EAFP_ERR_ARG;
@@ -355,123 +459,9 @@ int Batch::upload(int s, const double* const host[8], const double* phi0, int on
     a.x0 = x0.p + g.off;
     k_pack<<<(g.n + 255) / 256, 256, 0, stream>>>(a, g);
     EAFP_TRY(hipGetLastError());
-
-    if (!components) {
-        EAFP_TRY(hipStreamSynchronize(stream));
-        uploaded[s] = 1;
-        return EAFP_OK;
-    }
-    int singular = 1;
-    EAFP_TRY_RC(device_components(s, src, &singular));
-    if (!singular) {
-        EAFP_TRY(hipStreamSynchronize(stream));
-        uploaded[s] = 2;
-        return EAFP_OK;
-    }
-    // singular components (host): needs the coefficients on the host
-    std::vector<double> hbuf;
-    const double* hc[7];
-    if (on_device) {
-        hbuf.resize((size_t)np * 7);
-        for (int q = 0; q < 7; ++q)
-            EAFP_TRY(hipMemcpyAsync(hbuf.data() + (size_t)q * np, src[q], sizeof(double) * np, hipMemcpyDeviceToHost,
-                                    stream));
-        EAFP_TRY(hipStreamSynchronize(stream));
-        for (int q = 0; q < 7; ++q) hc[q] = hbuf.data() + (size_t)q * np;
-    } else {
-        for (int q = 0; q < 7; ++q) hc[q] = host[q];
-    }
-    std::vector<unsigned char> slot(g.n);
-    EAFP_TRY_RC(analyze_components(g, hc, slot.data(), comp_info[s]));
-    EAFP_TRY(hipMemcpyAsync(comp.p + g.off, slot.data(), g.n, hipMemcpyHostToDevice, stream));
     EAFP_TRY(hipStreamSynchronize(stream));
-    uploaded[s] = 2;
-    return EAFP_OK;
-}
-
-// ---------------------------------------------------------------------------
-// Connected components of the link graph (union-find over nonzero links,
-// either side) and their singularity: a component is singular (constant
-// null vector) when every row has aP equal to the sum of its interior
-// links up to rounding, i.e. no compliance, acoustic, Dirichlet or wall
-// term on its diagonal.  Rows without any link are never projected.
-// ---------------------------------------------------------------------------
-namespace {
-int uf_find(std::vector<int>& p, int a) {
-    while (p[a] != a) {
-        p[a] = p[p[a]];
-        a = p[a];
-    }
-    return a;
-}
-}  // namespace
-
-int analyze_components(const SysGeom& g, const double* const A[7], unsigned char* slot, CompInfo& info) {
-    const long sy = g.nr + 4, sz = (long)(g.nr + 4) * (g.nth + 4);
-    auto po = [&](int i, int j, int k) { return (long)(i + 2) + sy * (j + 2) + sz * (k + 2); };
-    auto cidx = [&](int i, int j, int k) { return (i * g.nth + j) * g.nz + k; };
-    std::vector<int> parent(g.n);
-    std::vector<char> linked(g.n, 0), neumann(g.n, 1);
-    for (int c = 0; c < g.n; ++c) parent[c] = c;
-    auto unite = [&](int a, int b) {
-        a = uf_find(parent, a);
-        b = uf_find(parent, b);
-        if (a != b) parent[a < b ? b : a] = a < b ? a : b;
-    };
-    for (int i = 0; i < g.nr; ++i)
-        for (int j = 0; j < g.nth; ++j)
-            for (int k = 0; k < g.nz; ++k) {
-                const long o = po(i, j, k);
-                const int c = cidx(i, j, k);
-                const int jn = (j + 1) % g.nth;
-                double links = 0.0;
-                if (i + 1 < g.nr && (A[C_AE][o] != 0.0 || A[C_AW][po(i + 1, j, k)] != 0.0)) {
-                    unite(c, cidx(i + 1, j, k));
-                    linked[c] = linked[cidx(i + 1, j, k)] = 1;
-                }
-                if (A[C_AN][o] != 0.0 || A[C_AS][po(i, jn, k)] != 0.0) {
-                    unite(c, cidx(i, jn, k));
-                    linked[c] = linked[cidx(i, jn, k)] = 1;
-                }
-                if (k + 1 < g.nz && (A[C_AT][o] != 0.0 || A[C_AB][po(i, j, k + 1)] != 0.0)) {
-                    unite(c, cidx(i, j, k + 1));
-                    linked[c] = linked[cidx(i, j, k + 1)] = 1;
-                }
-                if (i > 0) links += A[C_AW][o];
-                if (i + 1 < g.nr) links += A[C_AE][o];
-                links += A[C_AS][o] + A[C_AN][o];
-                if (k > 0) links += A[C_AB][o];
-                if (k + 1 < g.nz) links += A[C_AT][o];
-                const double ap = A[C_AP][o];
-                const bool wall = (i == 0 && A[C_AW][o] != 0.0) || (i + 1 == g.nr && A[C_AE][o] != 0.0) ||
-                                  (k == 0 && A[C_AB][o] != 0.0) || (k + 1 == g.nz && A[C_AT][o] != 0.0);
-                neumann[c] = !wall && std::fabs(ap - links) <= 1e-10 * std::fabs(ap);
-            }
-    // a component is singular iff all its rows are Neumann rows
-    std::vector<char> root_sing(g.n, 1);
-    for (int c = 0; c < g.n; ++c) {
-        const int r = uf_find(parent, c);
-        if (!linked[c] || !neumann[c]) root_sing[r] = 0;
-    }
-    std::vector<int> root_slot(g.n, -1);
-    std::vector<long> cnt;
-    info = CompInfo{};
-    for (int c = 0; c < g.n; ++c) {
-        slot[c] = NO_COMP;
-        if (!linked[c]) continue;
-        const int r = uf_find(parent, c);
-        if (!root_sing[r]) continue;
-        if (root_slot[r] < 0) {
-            root_slot[r] = (int)cnt.size();
-            cnt.push_back(0);
-        }
-        const int sl = root_slot[r];
-        ++cnt[sl];
-        if (sl < MAX_COMP) slot[c] = (unsigned char)sl;
-    }
-    info.ncomp = (int)std::min<size_t>(cnt.size(), MAX_COMP);
-    info.overflow = cnt.size() > (size_t)MAX_COMP;
-    for (int q = 0; q < info.ncomp; ++q) info.inv_cnt[q] = 1.0 / (double)cnt[q];
+    uploaded[s] = 1;     // singular components: labelled by the first mixed solve that needs them
+    counter_add(CTR_UPLOADS, 1);
     return EAFP_OK;
 }
 

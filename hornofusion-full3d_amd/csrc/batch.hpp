@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 #include <vector>
 
 #include "eafp_common.hpp"
@@ -54,16 +55,77 @@ struct DevBuf {
     }
 };
 
-// Singular (pure-Neumann) connected components of one system, found on the
-// host at upload: the mixed solver projects its residuals and its result to
-// zero mean on each of them.
+// Instantiated graphs of one solver, kept until the solver's workspace is
+// freed.  An entry's key is the solver's integer settings (format,
+// preconditioner, iterations per graph, launch-shape generation) together
+// with the kernel-argument struct the capture baked into its nodes, byte for
+// byte, so a reallocated buffer is a new key and a graph never outlives the
+// buffers it names.  Switching between keys never destroys a graph between
+// solves; entries are dropped only once kMax are held, after the stream is
+// idle, stale launch shapes first.  Args must be trivially copyable with
+// deterministic padding (the solvers value-initialise theirs).
+template <typename Args>
+struct GraphCache {
+    static constexpr int kKeys = 4;
+    static constexpr size_t kMax = 12;
+    struct Entry {
+        int key[kKeys];
+        int gen;
+        Args args;
+        hipGraphExec_t exec;
+    };
+    std::vector<Entry> v;
+
+    hipGraphExec_t find(const int (&key)[kKeys], int gen, const Args& args) const {
+        for (const Entry& e : v)
+            if (e.gen == gen && memcmp(e.key, key, sizeof(e.key)) == 0 && memcmp(&e.args, &args, sizeof(Args)) == 0)
+                return e.exec;
+        return nullptr;
+    }
+    // Makes room for one more entry; the caller's stream must be idle.
+    void make_room(int gen) {
+        if (v.size() < kMax) return;
+        for (size_t i = 0; i < v.size() && v.size() >= kMax;) {
+            if (v[i].gen != gen) {
+                (void)hipGraphExecDestroy(v[i].exec);
+                v.erase(v.begin() + (long)i);
+            } else {
+                ++i;
+            }
+        }
+        while (v.size() >= kMax) {
+            (void)hipGraphExecDestroy(v.front().exec);
+            v.erase(v.begin());
+        }
+    }
+    void add(const int (&key)[kKeys], int gen, const Args& args, hipGraphExec_t exec) {
+        Entry e;
+        memcpy(e.key, key, sizeof(e.key));
+        e.gen = gen;
+        memcpy(&e.args, &args, sizeof(Args));
+        e.exec = exec;
+        v.push_back(e);
+    }
+    void clear() {
+        for (Entry& e : v) (void)hipGraphExecDestroy(e.exec);
+        v.clear();
+    }
+};
+
+// Singular (pure-Neumann) connected components of one system, labelled on
+// the device before its first mixed solve: the mixed solver projects its
+// residuals and its result to zero mean on each of them.
 struct CompInfo {
     int ncomp = 0;                 // projected components (<= MAX_COMP)
     int overflow = 0;              // more singular components than MAX_COMP
+    int nsingular = 0;             // singular components found
+    int cells[MAX_COMP] = {};      // cells of each projected component
     double inv_cnt[MAX_COMP] = {}; // 1 / cells of each
 };
 
-int analyze_components(const SysGeom& g, const double* const padded[7], unsigned char* slot, CompInfo& info);
+// Process-wide activity counters (eafp_get_counters).
+enum { CTR_UPLOADS = 0, CTR_LABELLED, CTR_CAPTURES, CTR_LAUNCHES, CTR_COUNT };
+void counter_add(int which, long long v);
 
 struct Cg64Work;     // cg64.hip
 struct MixedWork;    // mixed.hip
@@ -96,10 +158,12 @@ struct Batch {
     DevBuf<unsigned char> comp;          // projected-component slot per cell (NO_COMP: none)
     std::vector<CompInfo> comp_info;
     std::vector<DevBuf<double>*> phi_pad;  // padded phi per system (device)
-    std::vector<char> uploaded;          // 0: no; 1: yes; 2: yes, singular components analysed
+    std::vector<char> uploaded;          // 0: no; 1: yes; 2: yes, singular components labelled
     DevBuf<double> staging;              // padded staging for uploads
-    DevBuf<unsigned char> comp_bits;     // link and anchor bits of the component pre-pass
-    DevBuf<int> comp_flag;
+    DevBuf<int> lab;                     // component labelling: a cell index per cell
+    DevBuf<unsigned char> lab_bits;      // link, anchor and root bits per cell
+    DevBuf<int> lab_small;               // per-system flags, root lists and counts
+    long label_sweeps = 0;               // sweeps of the last labelling
     hipStream_t stream = nullptr;
     int grid = 0;                        // blocks per tile-walking launch
     Cg64Work* cg64 = nullptr;
@@ -110,12 +174,11 @@ struct Batch {
     int set_tiling(int cells_per_thread, int blocks);
     int set_columns(int r_splits);
     int upload_tables();
-    // components = false skips the singular-component analysis, which only
-    // the mixed solver needs (it reads the coefficients on the host).
-    int upload(int s, const double* const host[8], const double* phi0, int arrays_on_device,
-               bool components = true);
+    int upload(int s, const double* const host[8], const double* phi0, int arrays_on_device);
     int download(int s, double* phi_host, int to_device);
-    int device_components(int s, const double* const A[7], int* singular);
+    // labels the singular components of the systems uploaded since their
+    // last labelling (only the mixed solver needs them)
+    int label_components();
     long padded_len(int s) const {
         const SysGeom& g = geom[s];
         return (long)(g.nr + 4) * (g.nth + 4) * (g.nz + 4);

@@ -1,6 +1,7 @@
 // capi.hip -- extern "C" entry points of libeafp.so (declared in
 // include/eafp.h).  Argument checks happen before any HIP call, so invalid
 // descriptors are reported the same way with or without a GPU.
+#include <atomic>
 #include <memory>
 #include <mutex>
 
@@ -11,6 +12,14 @@ int solve_fp64(Batch& B, int max_iter, double tol, eafp_stats* stats, int chunk)
 int solve_mixed(Batch& B, const eafp_mixed_opts& o, eafp_stats* stats);
 int time_kernels(Batch& B, int solver, int fmt, int reps, float* ms_spmv, float* ms_update);
 int autotune(Batch& B, int solver, int fmt, int reps, int* best_cpt, int* best_blocks, float* best_ms);
+
+namespace {
+std::atomic<long long> g_ctr[CTR_COUNT];
+}
+
+void counter_add(int which, long long v) {
+    if (which >= 0 && which < CTR_COUNT) g_ctr[which].fetch_add(v, std::memory_order_relaxed);
+}
 }  // namespace eafp
 
 struct eafp_batch {
@@ -184,8 +193,30 @@ int eafp_batch_info(const eafp_batch* b, int* grid, int* tile_cells, long* ncell
     return EAFP_OK;
 }
 
+int eafp_batch_components(eafp_batch* b, int s, int* nsingular, int cells[4]) {
+    if (!b || !nsingular) return EAFP_ERR_ARG;
+    eafp::Batch& B = b->B;
+    if (s < 0 || s >= B.nsys || !B.uploaded[s]) return EAFP_ERR_STATE;
+    int rc = B.label_components();
+    if (rc != EAFP_OK) return rc;
+    const eafp::CompInfo& c = B.comp_info[s];
+    *nsingular = c.nsingular;
+    if (cells)
+        for (int q = 0; q < eafp::MAX_COMP; ++q) cells[q] = q < c.ncomp ? c.cells[q] : 0;
+    return EAFP_OK;
+}
+
 int eafp_batch_destroy(eafp_batch* b) {
     delete b;
+    return EAFP_OK;
+}
+
+int eafp_get_counters(eafp_counters* out) {
+    if (!out) return EAFP_ERR_ARG;
+    out->uploads = eafp::g_ctr[eafp::CTR_UPLOADS].load();
+    out->labelled = eafp::g_ctr[eafp::CTR_LABELLED].load();
+    out->graph_captures = eafp::g_ctr[eafp::CTR_CAPTURES].load();
+    out->graph_launches = eafp::g_ctr[eafp::CTR_LAUNCHES].load();
     return EAFP_OK;
 }
 

@@ -71,14 +71,14 @@ struct Cg64Work {
     int* ndone_host = nullptr;     // pinned, 2 slots
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t t0 = nullptr, t1 = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int exec_chunk = 0, exec_gen = -1;
-    Cg64Args args;
+    GraphCache<Cg64Args> graphs;    // one instantiated graph per (chunk, shape, buffers)
+    hipGraphExec_t exec = nullptr;  // the cached graph of the current solve
+    Cg64Args args{};
 };
 
 void free_cg64(Cg64Work* w) {
     if (!w) return;
-    if (w->exec) (void)hipGraphExecDestroy(w->exec);
+    w->graphs.clear();
     for (auto& e : w->ev)
         if (e) (void)hipEventDestroy(e);
     if (w->t0) (void)hipEventDestroy(w->t0);
@@ -457,13 +457,16 @@ static void launch_init64_t(Batch& B) {
         default: fn<2>(__VA_ARGS__); break;         \
     }
 
+// The graph of `chunk` iterations for every system, looked up in the
+// workspace's cache by chunk, launch shape and every buffer its kernels
+// name; captured and instantiated on a miss.
 static int cg64_graph(Batch& B, int chunk) {
     Cg64Work* w = B.cg64;
-    if (w->exec && w->exec_chunk == chunk && w->exec_gen == B.tiling_gen) return EAFP_OK;
-    if (w->exec) {
-        (void)hipGraphExecDestroy(w->exec);
-        w->exec = nullptr;
-    }
+    const int key[GraphCache<Cg64Args>::kKeys] = {chunk, 0, 0, 0};
+    w->exec = w->graphs.find(key, B.tiling_gen, w->args);
+    if (w->exec) return EAFP_OK;
+    EAFP_TRY(hipStreamSynchronize(B.stream));   // no earlier launch of a graph that make_room may drop
+    w->graphs.make_room(B.tiling_gen);
     hipGraph_t graph = nullptr;
     EAFP_TRY(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
     for (int c = 0; c < chunk; ++c) {
@@ -473,11 +476,13 @@ static int cg64_graph(Batch& B, int chunk) {
     k64_count<<<1, BLOCK, 0, B.stream>>>(w->args);
     hipError_t e = hipStreamEndCapture(B.stream, &graph);
     if (e != hipSuccess) return (int)e;
-    e = hipGraphInstantiate(&w->exec, graph, nullptr, nullptr, 0);
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return (int)e;
-    w->exec_chunk = chunk;
-    w->exec_gen = B.tiling_gen;
+    w->graphs.add(key, B.tiling_gen, w->args, exec);
+    w->exec = exec;
+    counter_add(CTR_CAPTURES, 1);
     return EAFP_OK;
 }
 
@@ -510,6 +515,7 @@ int solve_fp64(Batch& B, int max_iter, double tol, eafp_stats* stats, int chunk)
     for (;;) {
         if (launched < max_chunks) {
             EAFP_TRY(hipGraphLaunch(w->exec, B.stream));
+            counter_add(CTR_LAUNCHES, 1);
             const int slot = (int)(launched & 1);
             EAFP_TRY(hipMemcpyAsync(&w->ndone_host[slot], w->ndone.p, sizeof(int), hipMemcpyDeviceToHost, B.stream));
             EAFP_TRY(hipEventRecord(w->ev[slot], B.stream));

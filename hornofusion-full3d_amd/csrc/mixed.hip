@@ -1741,14 +1741,14 @@ struct MixedWork {
     int* host_cnt = nullptr;        // pinned, 2 slots
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t t0 = nullptr, t1 = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int exec_interval = 0, exec_fmt = -1, exec_gen = -1, exec_line = -1;
-    MxArgs args;
+    GraphCache<MxArgs> graphs;      // one instantiated graph per (format, interval, precond, shape, buffers)
+    hipGraphExec_t exec = nullptr;  // the cached graph of the current solve
+    MxArgs args{};
 };
 
 void free_mixed(MixedWork* w) {
     if (!w) return;
-    if (w->exec) (void)hipGraphExecDestroy(w->exec);
+    w->graphs.clear();
     for (auto& e : w->ev)
         if (e) (void)hipEventDestroy(e);
     if (w->t0) (void)hipEventDestroy(w->t0);
@@ -1926,15 +1926,16 @@ static void launch_refine(Batch& B) {
     else k_mreplace<<<B.grid, BLOCK, 0, B.stream>>>(B.mixed->args, tile);
 }
 
+// The graph of one refinement step for every system.  Looked up in the
+// workspace's cache by format, interval, preconditioner, launch shape and
+// every buffer its kernels name; captured and instantiated on a miss.
 static int mixed_graph(Batch& B, int fmt, int interval) {
     MixedWork* w = B.mixed;
-    if (w->exec && w->exec_interval == interval && w->exec_fmt == fmt && w->exec_gen == B.tiling_gen &&
-        w->exec_line == w->args.line)
-        return EAFP_OK;
-    if (w->exec) {
-        (void)hipGraphExecDestroy(w->exec);
-        w->exec = nullptr;
-    }
+    const int key[GraphCache<MxArgs>::kKeys] = {fmt, interval, w->args.line, 0};
+    w->exec = w->graphs.find(key, B.tiling_gen, w->args);
+    if (w->exec) return EAFP_OK;
+    EAFP_TRY(hipStreamSynchronize(B.stream));   // no earlier launch of a graph that make_room may drop
+    w->graphs.make_room(B.tiling_gen);
     hipGraph_t graph = nullptr;
     EAFP_TRY(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
     for (int c = 0; c < interval; ++c) {
@@ -1945,13 +1946,13 @@ static int mixed_graph(Batch& B, int fmt, int interval) {
     k_mcount<<<1, BLOCK, 0, B.stream>>>(w->args);
     hipError_t e = hipStreamEndCapture(B.stream, &graph);
     if (e != hipSuccess) return (int)e;
-    e = hipGraphInstantiate(&w->exec, graph, nullptr, nullptr, 0);
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return (int)e;
-    w->exec_interval = interval;
-    w->exec_fmt = fmt;
-    w->exec_gen = B.tiling_gen;
-    w->exec_line = w->args.line;
+    w->graphs.add(key, B.tiling_gen, w->args, exec);
+    w->exec = exec;
+    counter_add(CTR_CAPTURES, 1);
     return EAFP_OK;
 }
 
@@ -1988,15 +1989,18 @@ static int mixed_start(Batch& B, const eafp_mixed_opts& o, int fmt) {
     return EAFP_OK;
 }
 
+// Argument and state checks, then the singular components of every system
+// uploaded since its last labelling.
 static int check_mixed(Batch& B, const eafp_mixed_opts& o) {
-    for (int s = 0; s < B.nsys; ++s) {
-        if (B.uploaded[s] != 2) return EAFP_ERR_STATE;
-        if (B.comp_info[s].overflow) return EAFP_ERR_COMPONENTS;
-    }
+    for (int s = 0; s < B.nsys; ++s)
+        if (!B.uploaded[s]) return EAFP_ERR_STATE;
     if (!(o.tol > 0.0) || o.max_steps < 1 || o.interval < 0) return EAFP_ERR_ARG;
     if (o.coef_format < EAFP_COEF_FP32 || o.coef_format > EAFP_COEF_FP16) return EAFP_ERR_ARG;
     if (o.precond != EAFP_PRECOND_JACOBI && o.precond != EAFP_PRECOND_ZLINE) return EAFP_ERR_ARG;
     if (o.precond == EAFP_PRECOND_ZLINE && B.nltiles == 0) return EAFP_ERR_UNSUPPORTED;
+    EAFP_TRY_RC(B.label_components());
+    for (int s = 0; s < B.nsys; ++s)
+        if (B.comp_info[s].overflow) return EAFP_ERR_COMPONENTS;
     return EAFP_OK;
 }
 
@@ -2019,6 +2023,7 @@ int solve_mixed(Batch& B, const eafp_mixed_opts& o, eafp_stats* stats) {
     for (;;) {
         if (launched < max_launch) {
             EAFP_TRY(hipGraphLaunch(w->exec, B.stream));
+            counter_add(CTR_LAUNCHES, 1);
             const int slot = (int)(launched & 1);
             EAFP_TRY(hipMemcpyAsync(&w->host_cnt[slot], w->ndone.p, sizeof(int), hipMemcpyDeviceToHost, B.stream));
             EAFP_TRY(hipEventRecord(w->ev[slot], B.stream));

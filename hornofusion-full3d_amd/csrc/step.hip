@@ -185,7 +185,7 @@ int pp_step(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config* c
     const double* csys[8];
     for (int q = 0; q < 8; ++q) csys[q] = sys[q];
     const bool mixed = o->solver == EAFP_STEP_MIXED;
-    EAFP_TRY_RC(B.upload(0, csys, nullptr, 1, mixed));
+    EAFP_TRY_RC(B.upload(0, csys, nullptr, 1));
     // launch shape by solver, as tuned on the 60x120x84 production mesh: the
     // FP64 CG at one cell per thread, the mixed solver at two; four blocks
     // per CU, at most one per tile

@@ -8,9 +8,9 @@ the build step (``build``).
 """
 from . import systems  # noqa: F401
 from ._lib import EafpError  # noqa: F401
-from .solver import (Batch, Mesh, cg_3d_mpi, pp_assemble, pp_correct, pp_step, release, tdma_3d_mpi,  # noqa: F401
-                     tdma_batch)
+from .solver import (Batch, Mesh, cg_3d_mpi, counters, pp_assemble, pp_correct, pp_step, release,  # noqa: F401
+                     tdma_3d_mpi, tdma_batch)
 
-__all__ = ["Batch", "EafpError", "Mesh", "cg_3d_mpi", "pp_assemble", "pp_correct", "pp_step", "release",
+__all__ = ["Batch", "EafpError", "Mesh", "cg_3d_mpi", "counters", "pp_assemble", "pp_correct", "pp_step", "release",
            "systems",
            "tdma_3d_mpi", "tdma_batch"]

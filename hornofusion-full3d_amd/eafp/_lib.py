@@ -30,7 +30,8 @@ EXPORTS = (
     "eafp_tdma_solve_batch", "eafp_batch_create", "eafp_batch_upload",
     "eafp_batch_solve_fp64", "eafp_batch_solve_mixed", "eafp_batch_download",
     "eafp_batch_info", "eafp_batch_set_tiling", "eafp_batch_set_columns", "eafp_batch_get_columns",
-    "eafp_batch_time_kernels", "eafp_batch_autotune", "eafp_batch_destroy", "eafp_version",
+    "eafp_batch_time_kernels", "eafp_batch_autotune", "eafp_batch_components", "eafp_batch_destroy",
+    "eafp_get_counters", "eafp_version",
 )
 
 
@@ -86,6 +87,12 @@ class Desc(ctypes.Structure):
     _fields_ = [("nr", ctypes.c_int), ("ntheta", ctypes.c_int), ("nz", ctypes.c_int),
                 ("halo", ctypes.c_int), ("lbound", ctypes.c_int), ("is_parallel", ctypes.c_int),
                 ("periodic_theta", ctypes.c_int)]
+
+
+class Counters(ctypes.Structure):
+    """eafp_counters: process-wide activity counters of the library."""
+    _fields_ = [("uploads", ctypes.c_longlong), ("labelled", ctypes.c_longlong),
+                ("graph_captures", ctypes.c_longlong), ("graph_launches", ctypes.c_longlong)]
 
 
 class Stats(ctypes.Structure):
@@ -168,6 +175,10 @@ def load():
     L.eafp_batch_time_kernels.restype = i
     L.eafp_batch_autotune.argtypes = [vp, i, i, i, ip, ip, fp]
     L.eafp_batch_autotune.restype = i
+    L.eafp_batch_components.argtypes = [vp, i, ip, ip]
+    L.eafp_batch_components.restype = i
+    L.eafp_get_counters.argtypes = [ctypes.POINTER(Counters)]
+    L.eafp_get_counters.restype = i
     L.eafp_batch_destroy.argtypes = [vp]
     L.eafp_batch_destroy.restype = i
     L.eafp_version.argtypes = []

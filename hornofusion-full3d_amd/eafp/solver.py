@@ -218,6 +218,14 @@ def tdma_batch(systems, meshes, n_sweep):
     check(L.eafp_tdma_solve_batch(nsys, descs, tab, int(n_sweep), 0), "eafp_tdma_solve_batch")
 
 
+def counters():
+    """The library's process-wide activity counters (eafp_get_counters)."""
+    L = _lib.load()
+    c = _lib.Counters()
+    check(L.eafp_get_counters(ctypes.byref(c)), "eafp_get_counters")
+    return {n: int(getattr(c, n)) for n, _ in _lib.Counters._fields_}
+
+
 def release():
     _lib.load().eafp_release()
 
@@ -289,6 +297,14 @@ class Batch:
         check(self._L.eafp_batch_autotune(self._h, int(solver), int(coef_format), int(reps), ctypes.byref(cpt),
                                           ctypes.byref(blocks), ctypes.byref(ms)), "eafp_batch_autotune")
         return {"cells_per_thread": cpt.value, "blocks": blocks.value, "ms_per_iter": ms.value}
+
+    def components(self, s):
+        """System s's singular components: (how many, their cell counts in
+        order of first cell when there are at most 4, else [])."""
+        n = ctypes.c_int()
+        cells = (ctypes.c_int * 4)()
+        check(self._L.eafp_batch_components(self._h, int(s), ctypes.byref(n), cells), "eafp_batch_components")
+        return n.value, [cells[q] for q in range(n.value)] if n.value <= 4 else []
 
     def download(self, s, phi=None):
         nr, nth, nz = self.shapes[s]

@@ -262,8 +262,8 @@ int eafp_batch_create(int nsys, const eafp_desc* descs, eafp_batch** out);
 /* Uploads system s.  Arrays are padded Fortran layout; arrays_on_device = 0
  * for host pointers, 1 for device pointers (packed where they lie).  phi0
  * may be NULL (zero).  The singular components the mixed solver projects
- * are found here: a GPU pass proves most systems free of them; otherwise
- * the coefficients are analysed on the host. */
+ * are labelled on the GPU by the first mixed solve (or
+ * eafp_batch_components call) after the upload; FP64 solves never label. */
 int eafp_batch_upload(eafp_batch* b, int s,
                       const double* aW, const double* aE, const double* aS, const double* aN,
                       const double* aB, const double* aT, const double* aP, const double* Su,
@@ -331,7 +331,34 @@ int eafp_batch_time_kernels(eafp_batch* b, int solver, int coef_format, int reps
 int eafp_batch_autotune(eafp_batch* b, int solver, int coef_format, int reps, int* cells_per_thread,
                         int* blocks, float* ms_iter);
 
+/* The singular components of system s: connected components of the link
+ * graph (a link where either side's coefficient is nonzero, theta periodic)
+ * whose every row is a pure-Neumann row (aP = sum of the interior links
+ * within 1e-10 relative, no wall link), i.e. the blocks whose p' is fixed
+ * only up to a constant.  Labels the batch's systems on the GPU if an
+ * upload has not been labelled yet.  *nsingular: how many; cells[q]: the
+ * cell count of each, ordered by first cell (cell index
+ * (i*ntheta + j)*nz + k), when there are at most 4 (0 past the last, and
+ * all 0 beyond 4).  The mixed solver projects at most 4 per system and
+ * refuses more (EAFP_ERR_COMPONENTS).  No reference
+ * counterpart (the reference regularises with PP_COMPLIANCE instead,
+ * src/mod_pressure_3d.f90:261-265). */
+int eafp_batch_components(eafp_batch* b, int s, int* nsingular, int cells[4]);
+
 int eafp_batch_destroy(eafp_batch* b);
+
+/* Process-wide activity counters, for diagnostics and tests: systems packed
+ * into HBM (every eafp_cg_solve, eafp_batch_upload and eafp_pp_step call),
+ * systems whose singular components were labelled, solver graphs captured
+ * and instantiated, and solver graph launches.  No reference counterpart. */
+typedef struct eafp_counters {
+    long long uploads;
+    long long labelled;
+    long long graph_captures;
+    long long graph_launches;
+} eafp_counters;
+
+int eafp_get_counters(eafp_counters* out);
 
 /* ---------------------------------------------------------------------------
  * Line-by-line TDMA smoother: tdma_3d_mpi (src/mod_solver_3d.f90:252-366)

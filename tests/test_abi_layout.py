@@ -23,6 +23,7 @@ STRUCTS = {
     "eafp_pp_update": _lib.PPUpdate,
     "eafp_pp_step_opts": _lib.PPStepOpts,
     "eafp_tdma_system": _lib.TdmaSystem,
+    "eafp_counters": _lib.Counters,
 }
 
 

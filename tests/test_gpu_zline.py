@@ -199,3 +199,34 @@ def test_zline_invalid_precond_refused():
         assert e.value.rc == EAFP_ERR_ARG
         st = b.solve_mixed(tol=1e-8, coef_format=COEF_BF16, precond=PRECOND_ZLINE)[0]
         assert st["status"] == "converged"
+
+
+def test_graph_survives_format_and_precond_switches():
+    """Cached solver graphs name the buffers they were captured with.  A
+    kernel timing in another weight format reallocates the weight buffers, and
+    switching the preconditioner picks another graph: a BF16 z-line solve
+    after FP32 timing, after a Jacobi solve and after an FP64 solve repeats the
+    first BF16 z-line solve bit for bit, and the Jacobi solve repeats too."""
+    ps = systems.make_system(2, 0, (20, 36, 28))
+    with eafp.Batch.from_systems([ps]) as b:
+        st0 = b.solve_mixed(tol=1e-8, coef_format=COEF_BF16, precond=PRECOND_ZLINE)[0]
+        phi0 = b.download(0)
+        b.time_kernels(eafp.Batch.SOLVER_MIXED, COEF_FP32, reps=2)
+        st1 = b.solve_mixed(tol=1e-8, coef_format=COEF_BF16, precond=PRECOND_ZLINE)[0]
+        phi1 = b.download(0)
+        stj = b.solve_mixed(tol=1e-8, coef_format=COEF_BF16)[0]
+        phij = b.download(0)
+        b.time_kernels(eafp.Batch.SOLVER_MIXED_ZLINE, COEF_FP16, reps=2)
+        b.solve_fp64(tol=1e-8)
+        st2 = b.solve_mixed(tol=1e-8, coef_format=COEF_BF16, precond=PRECOND_ZLINE)[0]
+        phi2 = b.download(0)
+        stj2 = b.solve_mixed(tol=1e-8, coef_format=COEF_BF16)[0]
+        phij2 = b.download(0)
+    print(f"case graph switches: zline inner {st0['n_iter']} {st1['n_iter']} {st2['n_iter']} "
+          f"jacobi {stj['n_iter']} {stj2['n_iter']}", flush=True)
+    check_answer(ps, phi0, st0, (2, (20, 36, 28)))
+    for st, phi in ((st1, phi1), (st2, phi2)):
+        assert st["n_iter"] == st0["n_iter"] and st["residual"] == st0["residual"]
+        np.testing.assert_array_equal(phi, phi0)
+    assert stj2["n_iter"] == stj["n_iter"]
+    np.testing.assert_array_equal(phij2, phij)
