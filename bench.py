@@ -10,11 +10,21 @@ symmetric face weights + FP64 refinement, setup pass included) of B seeded
 a true relative residual of 1e-8 on every system.  `value` = B*K / elapsed
 seconds.
 
-Beside it, on the same GPU: the plain FP64 Jacobi-PCG (the cg_3d_mpi
-algorithm, the drop-in path) on the single 60x120x84 production system, the
-mixed solver on that same single system, the relative p' error of the mixed
-results against FP64 reference solutions, and the CPU restatement of
-cg_3d_mpi (1 core) on the production system.
+Beside it, on the same GPU:
+- the same batch with the Jacobi inner preconditioner (the reference's);
+- the same batch solved by the batched plain FP64 Jacobi-PCG (the cg_3d_mpi
+  algorithm): the like-for-like denominator of the speedup;
+- uploads from device-resident arrays, the singular-component labelling and
+  the solve after them (upload_plus_solve_solves_per_s);
+- parity of every result: the relative p' error of each of the B systems
+  against an FP64 Jacobi-PCG run to 1e-12, and its true and L1 continuity
+  residuals recomputed on the host;
+- the plain FP64 GPU CG and the mixed solver on the single 60x120x84
+  production system, the drop-in on host arrays;
+- the CPU restatement of cg_3d_mpi (OpenMP and 1 core) on the production
+  system;
+- the p' assembly, the fused p' step, the line TDMA, and the other workloads
+  (256 ragged systems; one 120x240x168 system), each with its parity.
 """
 from __future__ import annotations
 
@@ -167,7 +177,8 @@ def derive_traffic(fetch_path, write_path, workload, algo):
 
 def measure_workload(eafp, syss, fmt, coef, interval, label, precond=0):
     """Mixed (batched) and plain FP64 GPU solves of one workload to 1e-8:
-    solves/s, iterations, and the iteration kernels' algorithmic GB/s.  The
+    solves/s, iterations, the iteration kernels' algorithmic GB/s, and the
+    parity of every mixed result against FP64 references (tol 1e-12).  The
     z-line preconditioner is used when asked for and every nz allows it."""
     ncells = sum(p.ncells for p in syss)
     b = eafp.Batch.from_systems(syss)
@@ -181,6 +192,7 @@ def measure_workload(eafp, syss, fmt, coef, interval, label, precond=0):
     t = time.perf_counter()
     st = b.solve_mixed(tol=1e-8, interval=interval, coef_format=fmt, precond=precond)
     tm = time.perf_counter() - t
+    xs = [b.download(s) for s in range(len(syss))]
     ms_sp, ms_up = b.time_kernels(msolver, fmt, reps=10)
     rec["mixed"] = {"solves_per_s": round(len(syss) / tm, 3), "seconds": round(tm, 4),
                     "all_converged": all(x["status"] == "converged" for x in st),
@@ -200,6 +212,9 @@ def measure_workload(eafp, syss, fmt, coef, interval, label, precond=0):
                    "k64_spmv_GBps": round(88 * ncells / (ms64[0] * 1e-3) / 1e9, 1),
                    "k64_update_GBps": round(64 * ncells / (ms64[1] * 1e-3) / 1e9, 1)}
     rec["speedup_mixed_vs_fp64"] = round(t64 / tm, 3)
+    refs, stref = fp64_references(eafp, syss, batch=b)
+    rec["parity"] = parity_summary(syss, xs, refs)
+    rec["parity"]["fp64_reference_converged"] = all(x["status"] == "converged" for x in stref)
     b.close()
     return rec
 
@@ -453,11 +468,106 @@ def measure_tdma(eafp, batches=(1, 8, 64), n_sweep=3, reps=5):
     return rec
 
 
+def measure_upload(eafp, batch, syss, fmt, interval, precond):
+    """Every system of `batch` uploaded again from device-resident padded
+    arrays (eafp_batch_upload with arrays_on_device = 1: packed where they
+    lie), then the first mixed solve after it, which labels the singular
+    components of each system on the GPU (timed apart through
+    eafp_batch_components) before solving."""
+    from eafp import _lib
+    hip = ctypes.CDLL("libamdhip64.so")
+    bufs = []
+    ptrs = []
+    for ps in syss:
+        row = []
+        for n in NAMES8:
+            a = np.asfortranarray(ps.padded(n))
+            p = ctypes.c_void_p()
+            _lib.check(hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(a.nbytes)), "hipMalloc")
+            _lib.check(hip.hipMemcpy(p, ctypes.c_void_p(a.ctypes.data), ctypes.c_size_t(a.nbytes), 1), "hipMemcpy")
+            bufs.append(p)
+            row.append(p.value)
+        ptrs.append(row)
+    device_sync()
+    t = time.perf_counter()
+    for s, row in enumerate(ptrs):
+        batch.upload_device(s, row)
+    t_up = time.perf_counter() - t
+    t = time.perf_counter()
+    ns, _ = batch.components(0)
+    t_lab = time.perf_counter() - t
+    t = time.perf_counter()
+    st = batch.solve_mixed(tol=1e-8, interval=interval, coef_format=fmt, precond=precond)
+    t_solve = time.perf_counter() - t
+    for p in bufs:
+        hip.hipFree(p)
+    B = len(syss)
+    return {"arrays_on_device": 1, "upload_s": round(t_up, 4), "label_s": round(t_lab, 4),
+            "solve_s": round(t_solve, 4), "all_converged": all(x["status"] == "converged" for x in st),
+            "singular_components_system0": ns,
+            "upload_plus_solve_solves_per_s": round(B / (t_up + t_lab + t_solve), 3)}
+
+
 def rel_err(x, xr, ps):
     from eafp.systems import project_zero_mean
     px = project_zero_mean(x, ps.labels, ps.ncomp)
     pr = project_zero_mean(xr, ps.labels, ps.ncomp)
     return float(np.linalg.norm(px - pr) / max(np.linalg.norm(pr), 1e-300))
+
+
+NAMES8 = ("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su")
+
+
+def host_residuals(ps, phi):
+    """FP64 residual of a downloaded phi, recomputed on the host with numpy
+    (independent of the library): (||b - A x||_2 / ||b||_2, sum|b - A x| /
+    sum|b|, the L1 form of res_cont in src/mod_pressure_3d.f90:293-306), with
+    A x as the reference's matvec forms it (phi's theta halos hold their
+    periodic copies after a download, its r/z halos what was uploaded)."""
+    a = {n: ps.padded(n) for n in NAMES8}
+    c = (slice(2, -2),) * 3
+    x = phi
+    ax = (a["aP"][c] * x[c] - a["aW"][c] * x[1:-3, 2:-2, 2:-2] - a["aE"][c] * x[3:-1, 2:-2, 2:-2]
+          - a["aS"][c] * x[2:-2, 1:-3, 2:-2] - a["aN"][c] * x[2:-2, 3:-1, 2:-2]
+          - a["aB"][c] * x[2:-2, 2:-2, 1:-3] - a["aT"][c] * x[2:-2, 2:-2, 3:-1])
+    b = a["Su"][c]
+    r = b - ax
+    return (float(np.linalg.norm(r) / max(np.linalg.norm(b), 1e-300)),
+            float(np.abs(r).sum() / max(np.abs(b).sum(), 1e-300)))
+
+
+PARITY_BAR = ("each result against an FP64 Jacobi-PCG run (the cg_3d_mpi algorithm, GPU, tol 1e-12) of the same "
+              "system: relative p' error ||P(x - x_ref)||/||P x_ref|| with P the zero-mean projection of every "
+              "singular component; true relative residual ||b - A x||/||b|| and L1 continuity residual "
+              "sum|b - A x|/sum|b|, both recomputed on the host in FP64")
+
+
+def fp64_references(eafp, syss, tol=1e-12, batch=None):
+    """The FP64 reference solutions: every system solved by the GPU FP64
+    Jacobi-PCG (the drop-in's algorithm) to `tol`.  Returns (padded phis,
+    stats)."""
+    own = batch is None
+    b = eafp.Batch.from_systems(syss) if own else batch
+    st = b.solve_fp64(400000, tol)
+    xs = [b.download(s) for s in range(len(syss))]
+    if own:
+        b.close()
+    return xs, st
+
+
+def parity_summary(syss, xs, refs):
+    """Per-system p' error against the FP64 references and host-recomputed
+    residuals, summarised over the systems."""
+    errs, l2s, l1s = [], [], []
+    for ps, x, xr in zip(syss, xs, refs):
+        errs.append(rel_err(x[2:-2, 2:-2, 2:-2], xr[2:-2, 2:-2, 2:-2], ps))
+        l2, l1 = host_residuals(ps, x)
+        l2s.append(l2)
+        l1s.append(l1)
+    return {"systems_checked": len(errs), "max_rel_err_p": max(errs), "mean_rel_err_p": float(np.mean(errs)),
+            "max_true_rel_residual_host": max(l2s), "max_l1_continuity_residual_host": max(l1s),
+            "mean_l1_continuity_residual_host": float(np.mean(l1s)),
+            "all_below_1e-8": bool(max(l2s) < 1e-8)}
 
 
 def main():
@@ -597,6 +707,7 @@ def main():
     refine = [s["n_refine"] for s in stats]
     res = [s["residual"] for s in stats]
     log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, {value:.2f} solves/s, all converged={conv}")
+    xs_main = [batch.download(s) for s in range(B)] if not args.no_reference else None
 
     # the two inner-iteration kernels, timed with HIP events on the library
     # stream; the roofline entry is the one with the longer launch
@@ -632,6 +743,7 @@ def main():
                            "inner_iters_max": int(max(inner)), "refine_steps_mean": float(np.mean(refine)),
                            "max_true_rel_residual": float(max(res)), "algorithmic_GBps": round(solve_gbs, 1),
                            "autotune": tune}}
+    xs_jac = None
     if zline:
         # the same batch with the Jacobi inner preconditioner, timed the same way (not retuned)
         batch.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt)
@@ -646,8 +758,54 @@ def main():
             "all_converged": all(x["status"] == "converged" for x in stj),
             "max_true_rel_residual": float(max(x["residual"] for x in stj))}
         log(f"jacobi inner preconditioner: {B / tj:.2f} solves/s")
+        if not args.no_reference:
+            xs_jac = [batch.download(s) for s in range(B)]
+
+    # upload from device-resident padded arrays (arrays_on_device = 1, the
+    # case of a GPU-resident caller), the singular-component labelling the
+    # first mixed solve after an upload runs, and that solve: the rate a
+    # caller sees when every solve brings new systems
+    up = measure_upload(eafp, batch, syss, fmt, args.interval, precond)
+    out["upload"] = up
+    out["upload_plus_solve_solves_per_s"] = up["upload_plus_solve_solves_per_s"]
+    log(f"upload {up['upload_s']:.3f}s, labelling {up['label_s']:.3f}s, "
+        f"upload+label+solve {up['upload_plus_solve_solves_per_s']:.2f} solves/s")
 
     if not args.no_reference:
+        # the plain FP64 GPU CG on the same batch (the drop-in's algorithm, batched):
+        # the like-for-like denominator of the mixed speedup; then the same solver to
+        # 1e-12 gives the FP64 references of the parity figures
+        fb = eafp.Batch.from_systems(syss)
+        fb.autotune(eafp.Batch.SOLVER_FP64, reps=3)
+        fb.solve_fp64(200000, 1e-8)
+        nrep = max(1, min(args.steps, 2))
+        t = time.perf_counter()
+        for _ in range(nrep):
+            sfb = fb.solve_fp64(200000, 1e-8)
+        tfb = (time.perf_counter() - t) / nrep
+        out["fp64_batch"] = {"workload": f"the same {B} systems, batched FP64 Jacobi-PCG (cg_3d_mpi algorithm)",
+                             "solves_per_s": round(B / tfb, 3), "n_iter_mean": float(np.mean([x["n_iter"] for x in sfb])),
+                             "all_converged": all(x["status"] == "converged" for x in sfb),
+                             "launch_shape": fb.info()}
+        out["speedup_mixed_batch_vs_fp64_batch"] = round(value * tfb / B, 3)
+        if "jacobi_inner" in out["mixed_batch"]:
+            out["speedup_mixed_batch_jacobi_inner_vs_fp64_batch"] = round(
+                out["mixed_batch"]["jacobi_inner"]["solves_per_s"] * tfb / B, 3)
+        log(f"fp64 batch: {B / tfb:.2f} solves/s")
+        refs, stref = fp64_references(eafp, syss, batch=fb)
+        fb.close()
+        par = {"bar": PARITY_BAR,
+               "fp64_reference_converged": all(x["status"] == "converged" for x in stref)}
+        par["mixed_batch"] = parity_summary(syss, xs_main, refs)
+        if xs_jac is not None:
+            par["mixed_batch_jacobi_inner"] = parity_summary(syss, xs_jac, refs)
+        out["mixed_batch"]["max_rel_err_vs_fp64_ref"] = par["mixed_batch"]["max_rel_err_p"]
+        out["mixed_batch"]["rel_err_checked_systems"] = par["mixed_batch"]["systems_checked"]
+        out["parity"] = par
+        del xs_main, xs_jac
+        log(f"parity of the batch: max rel p' err {par['mixed_batch']['max_rel_err_p']:.2e}, "
+            f"max host residual {par['mixed_batch']['max_true_rel_residual_host']:.2e}")
+
         # the plain FP64 GPU CG (drop-in algorithm) on the single production system
         ps = systems.make_system(2)
         single = eafp.Batch.from_systems([ps])
@@ -658,11 +816,11 @@ def main():
         for _ in range(reps):
             s64 = single.solve_fp64(100000, 1e-8)[0]
         t64 = (time.perf_counter() - t) / reps
-        x64 = single.download(0)[2:-2, 2:-2, 2:-2].copy()
+        x64 = single.download(0)
         ms64 = single.time_kernels(eafp.Batch.SOLVER_FP64, reps=20)
         # the drop-in entry point on host arrays (cg_3d_mpi's call): PCIe copies included
         mesh = eafp.Mesh(*ps.shape)
-        arrs = [ps.padded(n) for n in ("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su")]
+        arrs = [ps.padded(n) for n in NAMES8]
         phi_h = np.zeros_like(arrs[0], order="F")
         eafp.cg_3d_mpi(*arrs, phi_h, mesh, 100000, 1e-8)
         t = time.perf_counter()
@@ -677,7 +835,8 @@ def main():
         for _ in range(reps):
             smx = single.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt, precond=precond)[0]
         tmx = (time.perf_counter() - t) / reps
-        xmx = single.download(0)[2:-2, 2:-2, 2:-2].copy()
+        xmx = single.download(0)
+        ref1, _ = fp64_references(eafp, [ps], batch=single)
         out["fp64_single"] = {"workload": "60x120x84 production system (liquid 7000 below 0.9 m, gas 0.3)",
                               "solves_per_s": round(1.0 / t64, 3), "n_iter": s64["n_iter"],
                               "residual": s64["residual"], "ms_per_iter": round(1e3 * t64 / s64["n_iter"], 4),
@@ -687,21 +846,11 @@ def main():
         out["mixed_single"] = {"solves_per_s": round(1.0 / tmx, 3), "inner_iters": smx["n_iter"],
                                "refine_steps": smx["n_refine"], "true_rel_residual": smx["true_rel_l2"],
                                "l1_continuity_residual": smx["true_rel_l1"]}
+        par["fp64_single"] = parity_summary([ps], [x64], ref1)
+        par["mixed_single"] = parity_summary([ps], [xmx], ref1)
         out["speedup_mixed_batch_vs_fp64"] = round(value * t64, 3)
         out["speedup_mixed_single_vs_fp64"] = round(t64 / tmx, 3)
         single.close()
-
-        # relative p' error of the batched mixed results against FP64 reference
-        # solutions of the same systems (GPU FP64 CG at tol 1e-11, the drop-in
-        # algorithm whose parity with the CPU oracle the test suite checks)
-        nchk = min(B, 8)
-        ref = eafp.Batch.from_systems(syss[:nchk])
-        ref.solve_fp64(200000, 1e-11)
-        errs = [rel_err(batch.download(s)[2:-2, 2:-2, 2:-2], ref.download(s)[2:-2, 2:-2, 2:-2], syss[s])
-                for s in range(nchk)]
-        ref.close()
-        out["mixed_batch"]["max_rel_err_vs_fp64_ref"] = max(errs)
-        out["mixed_batch"]["rel_err_checked_systems"] = nchk
 
         if not args.no_cpu_baseline:
             # CPU legs: the oracle restatement of cg_3d_mpi on the production
@@ -735,8 +884,12 @@ def main():
             out["cpu_baseline"]["config_12x24x16"] = {"threads_solves_per_s": round(1.0 / t1p, 2),
                                                       "one_core_solves_per_s": round(1.0 / t1c, 2),
                                                       "n_iter": nit1}
-            out["mixed_single"]["rel_err_vs_cpu_oracle"] = rel_err(xmx, xc, ps)
-            out["fp64_single"]["rel_err_vs_cpu_oracle"] = rel_err(x64, xc, ps)
+            out["mixed_single"]["rel_err_vs_cpu_oracle"] = rel_err(xmx[2:-2, 2:-2, 2:-2], xc, ps)
+            out["fp64_single"]["rel_err_vs_cpu_oracle"] = rel_err(x64[2:-2, 2:-2, 2:-2], xc, ps)
+            # the GPU FP64 reference (tol 1e-12) against the CPU oracle run to the same tolerance
+            phi_r, _, nit_r = oracle.cg_omp(ps, max_iter=400000, tol=1e-12)
+            par["fp64_reference_vs_cpu_oracle_1e-12"] = {
+                "rel_diff_p": rel_err(ref1[0][2:-2, 2:-2, 2:-2], phi_r[2:-2, 2:-2, 2:-2], ps), "oracle_n_iter": nit_r}
     batch.close()
 
     if not args.no_extras:

@@ -76,10 +76,34 @@ def cg_3d_mpi(aW, aE, aS, aN, aB, aT, aP, Su, phi, m, max_iter, tol):
     return res.value, nit.value
 
 
-def _pp_structs(F, cfg):
+def _field_shape(n, m):
+    """The padded shape the library reads for field ``n`` on mesh ``m``: the
+    coordinates r, theta, z have bounds (-1:n+2), the outlet mask is the top
+    layer (nr+4, ntheta+4), every other field (nr+4, ntheta+4, nz+4)."""
+    if n == "r":
+        return (m.nr + 4,)
+    if n == "theta":
+        return (m.ntheta + 4,)
+    if n == "z":
+        return (m.nz + 4,)
+    if n == "outlet":
+        return (m.nr + 4, m.ntheta + 4)
+    return (m.nr + 4, m.ntheta + 4, m.nz + 4)
+
+
+def _checked(a, shape, name):
+    """``a`` as the library's padded array, or ValueError when its shape is
+    not ``shape`` (the library copies exactly that many values)."""
+    if np.shape(a) != shape:
+        raise ValueError(f"{name} must have the padded shape {shape}, not {np.shape(a)}")
+    return a
+
+
+def _pp_structs(F, cfg, m):
     """eafp_pp_fields / eafp_pp_config for the arrays of ``F`` (absent
-    workspace masks are not valid) and the config dict ``cfg``; ``keep``
-    holds the converted arrays alive."""
+    workspace masks are not valid; each array checked against its padded
+    shape on mesh ``m``) and the config dict ``cfg``; ``keep`` holds the
+    converted arrays alive."""
     keep = []
     vals = {}
     for n in _lib.PP_FIELDS:
@@ -87,6 +111,7 @@ def _pp_structs(F, cfg):
         if a is None:
             vals[n] = None
             continue
+        _checked(a, _field_shape(n, m), n)
         a = np.asfortranarray(a, dtype=np.int32 if n in _lib.PP_INT else np.float64)
         keep.append(a)
         vals[n] = a.ctypes.data
@@ -108,7 +133,7 @@ def pp_assemble(F, cfg, m, exports=False):
     padded arrays, plus the dict of the liquid's face exports (names of
     ``_lib.PP_EXPORTS``) when ``exports``."""
     L = _lib.load()
-    f, c, keep = _pp_structs(F, cfg)
+    f, c, keep = _pp_structs(F, cfg, m)
     names = ("aW", "aE", "aS", "aN", "aB", "aT", "aP", "Su")
     out = {n: _padded_zeros(m) for n in names}
     ex = {n: _padded_zeros(m, np.int32 if n.startswith("lk") else np.float64) for n in _lib.PP_EXPORTS} if exports else None
@@ -132,14 +157,16 @@ def pp_correct(F, cfg, m, alpha_p, pp, ex):
     arrays; ``F`` is not modified.  ``pp`` is the solved correction (theta
     halos filled), ``ex`` the exports of ``pp_assemble(..., exports=True)``."""
     L = _lib.load()
-    f, c, keep = _pp_structs(F, cfg)
-    exa = {n: np.asfortranarray(ex[n], dtype=np.int32 if n.startswith("lk") else np.float64) for n in _lib.PP_EXPORTS}
+    f, c, keep = _pp_structs(F, cfg, m)
+    pshape = _field_shape("p", m)
+    exa = {n: np.asfortranarray(_checked(ex[n], pshape, n), dtype=np.int32 if n.startswith("lk") else np.float64)
+           for n in _lib.PP_EXPORTS}
     exs = _lib.PPExports(**{n: a.ctypes.data for n, a in exa.items()})
     out = {n: _padded_zeros(m, np.int32 if n.startswith("lk") else np.float64) for n in _lib.PP_UPDATE[:6]}
     for n in _lib.PP_UPDATE[6:]:
         out[n] = np.array(F[n], dtype=np.float64, order="F")
     io = _lib.PPUpdate(**{n: out[n].ctypes.data for n in _lib.PP_UPDATE})
-    ppa = np.asfortranarray(pp, dtype=np.float64)
+    ppa = np.asfortranarray(_checked(pp, pshape, "pp"), dtype=np.float64)
     d = desc_of(m)
     check(L.eafp_pp_correct(ctypes.byref(d), ctypes.byref(f), ctypes.byref(c), float(alpha_p), ppa.ctypes.data,
                             ctypes.byref(exs), ctypes.byref(io), 0), "eafp_pp_correct")
@@ -160,7 +187,7 @@ def pp_step(F, cfg, m, alpha_p, solver="fp64", max_iter=1000, tol=1e-8, mixed=No
     they lie and returned in ``out``, as the reference updates liq%ur and
     sh%p, which are also the assembly's inputs."""
     L = _lib.load()
-    f, c, keep = _pp_structs(F, cfg)
+    f, c, keep = _pp_structs(F, cfg, m)
     out = {n: _padded_zeros(m, np.int32 if n.startswith("lk") else np.float64) for n in _lib.PP_UPDATE[:6]}
     for n in _lib.PP_UPDATE[6:]:
         if in_place:
@@ -258,6 +285,16 @@ class Batch:
         ptrs = [_padded(ps.padded(n), shape, n) for n in COEF_NAMES + ("Su",)]
         p0 = _padded(phi0, shape, "phi0") if phi0 is not None else None
         check(self._L.eafp_batch_upload(self._h, s, *ptrs, p0, 0), "eafp_batch_upload")
+
+    def upload_device(self, s, ptrs, phi0=None):
+        """Uploads system s from device memory: ``ptrs`` are the device
+        addresses of its padded aW, aE, aS, aN, aB, aT, aP and Su (packed
+        where they lie); ``phi0`` a device address or None (zero)."""
+        if len(ptrs) != 8:
+            raise ValueError("upload_device needs the 8 arrays aW..aT, aP, Su")
+        check(self._L.eafp_batch_upload(self._h, s, *[ctypes.c_void_p(int(p)) for p in ptrs],
+                                        None if phi0 is None else ctypes.c_void_p(int(phi0)), 1),
+              "eafp_batch_upload")
 
     def solve_fp64(self, max_iter=100000, tol=1e-8):
         st = (Stats * len(self))()

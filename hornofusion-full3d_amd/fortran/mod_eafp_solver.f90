@@ -274,6 +274,7 @@ contains
         real(c_double)            :: res
         integer(c_int)            :: rc
         integer                   :: sh(3)
+        logical                   :: bad
 
         d%nr = m%nr
         d%ntheta = m%ntheta
@@ -287,11 +288,28 @@ contains
         cg_res = residual
         n_iter = 0
         sh = [m%nr + 4, m%ntheta + 4, m%nz + 4]
+        ! every array against the padded shape the library copies from or to
         if (size(r) /= sh(1) .or. size(theta) /= sh(2) .or. size(z) /= sh(3) .or. &
             any(shape(outlet) /= sh(1:2)) .or. any(shape(cell_type) /= sh) .or. &
-            any(shape(vol) /= sh) .or. any(shape(p) /= sh) .or. any(shape(pp) /= sh) .or. &
-            any(shape(ur_l) /= sh) .or. any(shape(ur_g) /= sh) .or. any(shape(Fc_r) /= sh) .or. &
-            any(shape(lk_r) /= sh)) then
+            any(shape(vol) /= sh) .or. any(shape(Ar) /= sh) .or. any(shape(Ath) /= sh) .or. &
+            any(shape(Az) /= sh) .or. any(shape(p) /= sh) .or. any(shape(pp) /= sh) .or. &
+            any(shape(alpha_l) /= sh) .or. any(shape(aPr_l) /= sh) .or. any(shape(aPth_l) /= sh) .or. &
+            any(shape(aPz_l) /= sh) .or. any(shape(rho_l) /= sh) .or. any(shape(T_l) /= sh) .or. &
+            any(shape(alpha_g) /= sh) .or. any(shape(aPr_g) /= sh) .or. any(shape(aPth_g) /= sh) .or. &
+            any(shape(aPz_g) /= sh) .or. any(shape(rho_g) /= sh) .or. any(shape(T_old_g) /= sh) .or. &
+            any(shape(ur_l) /= sh) .or. any(shape(uth_l) /= sh) .or. any(shape(uz_l) /= sh) .or. &
+            any(shape(ur_g) /= sh) .or. any(shape(uth_g) /= sh) .or. any(shape(uz_g) /= sh) .or. &
+            any(shape(Fc_r) /= sh) .or. any(shape(Fc_th) /= sh) .or. any(shape(Fc_z) /= sh) .or. &
+            any(shape(lk_r) /= sh) .or. any(shape(lk_th) /= sh) .or. any(shape(lk_z) /= sh)) then
+            last_status = -1
+            return
+        end if
+        bad = .false.
+        if (present(liq_cont)) bad = bad .or. any(shape(liq_cont) /= sh)
+        if (present(pv_active)) bad = bad .or. any(shape(pv_active) /= sh)
+        if (present(gas_reach)) bad = bad .or. any(shape(gas_reach) /= sh)
+        if (present(liq_room)) bad = bad .or. any(shape(liq_room) /= sh)
+        if (bad) then
             last_status = -1
             return
         end if

@@ -113,6 +113,32 @@ def test_pp_assemble_rejects_bad_calls():
     assert e.value.rc == _lib.EAFP_ERR_ARG
 
 
+@pytest.mark.parametrize("name,bad", [("rho_l", (8, 10, 8)), ("Ar", (8, 10)), ("outlet", (8, 10, 9)),
+                                      ("r", (7,)), ("theta", (11,)), ("liq_room", (9, 10, 9))])
+def test_pp_field_shapes_checked(name, bad):
+    """Every field is checked against its padded shape before the library
+    copies (nr+4)(ntheta+4)(nz+4) values (or n+4 coordinates, or the outlet
+    layer) from it: a short array is a ValueError, never an over-read."""
+    from eafp import fields
+    shape = (4, 6, 5)
+    F, cfg = fields.make_fields(*shape, masks=True)
+    G = dict(F)
+    G[name] = np.zeros(bad, dtype=F[name].dtype, order="F")
+    m = eafp.Mesh(*shape)
+    for call in (lambda: eafp.pp_assemble(G, cfg, m), lambda: eafp.pp_step(G, cfg, m, 0.3)):
+        with pytest.raises(ValueError, match=name):
+            call()
+    pad = tuple(n + 4 for n in shape)
+    ex = {n: np.zeros(pad, dtype=np.int32 if n.startswith("lk") else np.float64, order="F") for n in _lib.PP_EXPORTS}
+    with pytest.raises(ValueError, match=name):
+        eafp.pp_correct(G, cfg, m, 0.3, np.zeros(pad, order="F"), ex)
+    with pytest.raises(ValueError, match="pp"):
+        eafp.pp_correct(F, cfg, m, 0.3, np.zeros((8, 10, 8), order="F"), ex)
+    ex["ac_z"] = np.zeros((8, 10, 8), order="F")
+    with pytest.raises(ValueError, match="ac_z"):
+        eafp.pp_correct(F, cfg, m, 0.3, np.zeros(pad, order="F"), ex)
+
+
 def test_pp_post_solve_struct_layouts():
     # eafp_pp_exports: 9 pointers; eafp_pp_update: 13 pointers
     assert ctypes.sizeof(_lib.PPExports) == 9 * 8
@@ -139,7 +165,7 @@ def test_pp_correct_rejects_bad_calls():
     assert e.value.rc == _lib.EAFP_ERR_ARG
     # a NULL export array, through the C-ABI directly
     L = _lib.load()
-    f, c, keep = eafp.solver._pp_structs(F, cfg)
+    f, c, keep = eafp.solver._pp_structs(F, cfg, eafp.Mesh(*shape))
     vals = {n: a.ctypes.data for n, a in ex.items()}
     vals["ac_th"] = None
     exs = _lib.PPExports(**vals)
@@ -210,7 +236,7 @@ def test_pp_step_rejects_bad_calls():
         eafp.pp_step(F, cfg, eafp.Mesh(*shape), 0.3, solver="sor")
     # an unknown solver code, a NULL pp and a NULL io array, through the C-ABI directly
     L = _lib.load()
-    f, c, keep = eafp.solver._pp_structs(F, cfg)
+    f, c, keep = eafp.solver._pp_structs(F, cfg, eafp.Mesh(*shape))
     pad = tuple(n + 4 for n in shape)
     io_arrays = {n: np.zeros(pad, dtype=np.int32 if n.startswith("lk") else np.float64, order="F")
                  for n in _lib.PP_UPDATE}
