@@ -62,6 +62,40 @@ def test_drop_in_matches_oracle(config, shape):
     check_halos(phi, phi_in, ps.nth)
 
 
+def test_fold_and_initial_phi():
+    """Nonzero wall links, r/z halos, interior x0 and RHS: the first residual
+    takes each wall link times its halo value (the fold vector), as the
+    reference's first matvec does; the iterates, iteration count and halos
+    match the oracle, and dropping the halos changes the answer."""
+    from fold_case import fold_case
+    ps, phi0 = fold_case()
+    print(f"case fold shape={ps.shape}", flush=True)
+    m = eafp.Mesh(*ps.shape)
+    phi = phi0.copy(order="F")
+    res, nit = eafp.cg_3d_mpi(*[ps.padded(n) for n in COEF_NAMES], ps.Su, phi, m, 100000, 1e-8)
+    phi_ref, res_ref, nit_ref, _ = oracle.cg(ps, phi0, max_iter=100000, tol=1e-8)
+    print(f"  gpu n_iter={nit} res={res:.2e} | oracle n_iter={nit_ref} res={res_ref:.2e}", flush=True)
+    assert res < 1e-8 and abs(nit - nit_ref) <= max(2, nit_ref // 50)
+    x, xr = oracle.interior(phi), oracle.interior(phi_ref)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-7
+    l2, _ = oracle.true_residual(ps, phi)
+    assert l2 < 1.5e-8
+    check_halos(phi, phi0, ps.nth)
+    # the same solve with zero r/z halos converges to another answer
+    phi2 = phi0.copy(order="F")
+    for sl in ((slice(0, 2),), (slice(-2, None),), (slice(None), slice(None), slice(0, 2)),
+               (slice(None), slice(None), slice(-2, None))):
+        phi2[sl] = 0.0
+    eafp.cg_3d_mpi(*[ps.padded(n) for n in COEF_NAMES], ps.Su, phi2, m, 100000, 1e-8)
+    assert np.linalg.norm(oracle.interior(phi2) - x) > 1e-3 * np.linalg.norm(x)
+    # and max_iter = 3 stops on the oracle's third iterate
+    phi3 = phi0.copy(order="F")
+    eafp.cg_3d_mpi(*[ps.padded(n) for n in COEF_NAMES], ps.Su, phi3, m, 3, 1e-8)
+    phi3_ref, _, _, _ = oracle.cg(ps, phi0, max_iter=3, tol=1e-8)
+    np.testing.assert_allclose(oracle.interior(phi3), oracle.interior(phi3_ref), rtol=0,
+                               atol=1e-9 * np.abs(oracle.interior(phi3_ref)).max())
+
+
 def test_production_size_true_residual():
     """60x120x84 (604,800 cells): size-independent checks at full size."""
     ps = systems.make_system(2)

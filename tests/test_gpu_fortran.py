@@ -30,6 +30,26 @@ def test_fortran_drop_in(config, shape, max_iter):
     np.testing.assert_array_equal(phi_out[:, ps.nth + 2, :], phi_out[:, 2, :])
 
 
+def test_fortran_fold_and_initial_phi():
+    """cg_3d_mpi from Fortran with nonzero wall links, r/z halos, interior
+    x0 and RHS: the oracle's iterates and halo semantics."""
+    from fold_case import fold_case
+    ps, phi0 = fold_case()
+    print(f"case fortran fold shape={ps.shape}", flush=True)
+    res, nit, status, phi_out = run_driver(ps, phi0, 100000, 1e-8)
+    phi_ref, res_ref, nit_ref, _ = oracle.cg(ps, phi0, max_iter=100000, tol=1e-8)
+    assert status == 0 and res < 1e-8 and abs(nit - nit_ref) <= max(2, nit_ref // 50)
+    x, xr = oracle.interior(phi_out), oracle.interior(phi_ref)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-7
+    # r/z halos untouched (the theta copy overwrites their theta-halo corners)
+    np.testing.assert_array_equal(phi_out[:2, 2:-2], phi0[:2, 2:-2])
+    np.testing.assert_array_equal(phi_out[-2:, 2:-2], phi0[-2:, 2:-2])
+    np.testing.assert_array_equal(phi_out[2:-2, 2:-2, :2], phi0[2:-2, 2:-2, :2])
+    np.testing.assert_array_equal(phi_out[2:-2, 2:-2, -2:], phi0[2:-2, 2:-2, -2:])
+    np.testing.assert_array_equal(phi_out[:, 1, :], phi_out[:, ps.nth + 1, :])
+    np.testing.assert_array_equal(phi_out[:, ps.nth + 3, :], phi_out[:, 3, :])
+
+
 @pytest.mark.parametrize("shape,n_sweep", [((12, 24, 16), 3), ((60, 120, 84), 1)])
 def test_fortran_tdma_drop_in(shape, n_sweep):
     """tdma_3d_mpi called from Fortran: phi bit-identical to the oracle's restatement."""
