@@ -591,14 +591,18 @@ int ensure_pp(const eafp_desc* d) {
 
 }  // namespace
 
-int pp_assemble(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config* cfg, double* const out[8],
-                double* residual, double* flux_ref, const eafp_pp_exports* ex, int on_device) {
+namespace {
+
+// Argument checks of eafp_pp_assemble (before any device call); fills the
+// export pointers (all NULL without exports).
+int pp_assemble_check(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config* cfg, double* const out[8],
+                      const eafp_pp_exports* ex, double* ex_d[6], int* ex_i[3]) {
     if (!d || !f || !cfg || !out) return EAFP_ERR_ARG;
     EAFP_TRY_RC(validate_desc(d));
     for (int q = 0; q < 8; ++q)
         if (!out[q]) return EAFP_ERR_ARG;
-    double* ex_d[6] = {};
-    int* ex_i[3] = {};
+    for (int q = 0; q < 6; ++q) ex_d[q] = nullptr;
+    for (int q = 0; q < 3; ++q) ex_i[q] = nullptr;
     if (ex) {
         double* const ed[6] = {ex->Fs_r, ex->Fs_th, ex->Fs_z, ex->ac_r, ex->ac_th, ex->ac_z};
         int* const ei[3] = {ex->lk_r, ex->lk_th, ex->lk_z};
@@ -614,44 +618,96 @@ int pp_assemble(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_confi
         if (!q) return EAFP_ERR_ARG;
     if (!f->r || !f->theta || !f->z || !f->cell_type || !f->outlet) return EAFP_ERR_ARG;
     if (!(cfg->dt > 0.0)) return EAFP_ERR_ARG;
+    return EAFP_OK;
+}
+
+// The kernels' arguments apart from inputs: the mesh and the config values.
+void pp_args_common(PPArgs& a, const eafp_desc* d, const eafp_pp_config* cfg) {
+    a.nr = d->nr;
+    a.nth = d->ntheta;
+    a.nz = d->nz;
+    a.sy = d->nr + 4;
+    a.sz = (long)(d->nr + 4) * (d->ntheta + 4);
+    a.n = a.sz * (d->nz + 4);
+    a.dt = cfg->dt;
+    a.rho_gas = cfg->rho_gas;
+    a.T_amb = cfg->T_ambient;
+    a.beta = cfg->beta_expansion;
+    a.two_phase = cfg->gas_in_poisson && cfg->solve_multiphase;
+    a.compress = cfg->gas_compressibility && cfg->solve_multiphase;
+    a.chunk = pp_chunk(a.sz);
+}
+
+// The input pointers, from an array of the 23 three-dimensional fields.
+void pp_args_fields(PPArgs& a, const double* const dev3[NF3]) {
+    a.vol = dev3[0]; a.Ar = dev3[1]; a.Ath = dev3[2]; a.Az = dev3[3];
+    a.al = dev3[4]; a.url = dev3[5]; a.uthl = dev3[6]; a.uzl = dev3[7];
+    a.aPrl = dev3[8]; a.aPthl = dev3[9]; a.aPzl = dev3[10]; a.rhol = dev3[11]; a.Tl = dev3[12];
+    a.ag = dev3[13]; a.urg = dev3[14]; a.uthg = dev3[15]; a.uzg = dev3[16];
+    a.aPrg = dev3[17]; a.aPthg = dev3[18]; a.aPzg = dev3[19]; a.rhog = dev3[20]; a.Toldg = dev3[21];
+    a.p = dev3[22];
+}
+
+// Every input and output in device memory, where the caller holds it.
+void pp_args_device(PPArgs& a, const eafp_pp_fields* f, double* const out[8], double* const ex_d[6],
+                    int* const ex_i[3]) {
+    const double* f3[NF3] = {f->vol,    f->Ar,    f->Ath,    f->Az,    f->alpha_l, f->ur_l,    f->uth_l,   f->uz_l,
+                             f->aPr_l,  f->aPth_l, f->aPz_l, f->rho_l, f->T_l,     f->alpha_g, f->ur_g,    f->uth_g,
+                             f->uz_g,   f->aPr_g,  f->aPth_g, f->aPz_g, f->rho_g,  f->T_old_g, f->p};
+    pp_args_fields(a, f3);
+    a.r = f->r;
+    a.th = f->theta;
+    a.z = f->z;
+    a.ct = f->cell_type;
+    a.outlet = f->outlet;
+    a.liq_cont = f->liq_cont;
+    a.pv_active = f->pv_active;
+    a.gas_reach = f->gas_reach;
+    a.liq_room = f->liq_room;
+    double** dst_out[8] = {&a.aW, &a.aE, &a.aS, &a.aN, &a.aB, &a.aT, &a.aP, &a.Su};
+    for (int q = 0; q < 8; ++q) *dst_out[q] = out[q];
+    for (int dd = 0; dd < 3; ++dd) {
+        a.Fs[dd] = ex_d[dd];
+        a.ac[dd] = ex_d[3 + dd];
+        a.lk[dd] = ex_i[dd];
+    }
+}
+
+}  // namespace
+
+int pp_assemble(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config* cfg, double* const out[8],
+                double* residual, double* flux_ref, const eafp_pp_exports* ex, int on_device) {
+    double* ex_d[6];
+    int* ex_i[3];
+    EAFP_TRY_RC(pp_assemble_check(d, f, cfg, out, ex, ex_d, ex_i));
     std::lock_guard<std::mutex> lock(g_pp_mutex);
     EAFP_TRY_RC(ensure_pp(d));
     PPWork& w = *g_pp;
-    const int nr = d->nr, nth = d->ntheta, nz = d->nz;
-    const long sy = nr + 4, sz = (long)(nr + 4) * (nth + 4), n = sz * (nz + 4);
-    const long n2 = (long)(nr + 4) * (nth + 4);
     PPArgs a{};
-    a.nr = nr;
-    a.nth = nth;
-    a.nz = nz;
-    a.sy = sy;
-    a.sz = sz;
-    a.n = n;
-    const double* dev3[NF3];
+    pp_args_common(a, d, cfg);
+    const int nr = d->nr, nth = d->ntheta;
+    const long n = a.n, n2 = (long)(nr + 4) * (nth + 4);
     if (on_device) {
-        a.r = f->r;
-        a.th = f->theta;
-        a.z = f->z;
-        for (int q = 0; q < NF3; ++q) dev3[q] = f3[q];
-        a.ct = f->cell_type;
-        a.outlet = f->outlet;
-        a.liq_cont = f->liq_cont;
-        a.pv_active = f->pv_active;
-        a.gas_reach = f->gas_reach;
-        a.liq_room = f->liq_room;
+        pp_args_device(a, f, out, ex_d, ex_i);
     } else {
+        const double* f3[NF3] = {f->vol,    f->Ar,    f->Ath,    f->Az,    f->alpha_l, f->ur_l,    f->uth_l,
+                                 f->uz_l,   f->aPr_l, f->aPth_l, f->aPz_l, f->rho_l,   f->T_l,     f->alpha_g,
+                                 f->ur_g,   f->uth_g, f->uz_g,   f->aPr_g, f->aPth_g,  f->aPz_g,   f->rho_g,
+                                 f->T_old_g, f->p};
         const hipMemcpyKind H2D = hipMemcpyHostToDevice;
         double* c = w.coord.p;
         EAFP_TRY(hipMemcpyAsync(c, f->r, sizeof(double) * (nr + 4), H2D, w.stream));
         EAFP_TRY(hipMemcpyAsync(c + nr + 4, f->theta, sizeof(double) * (nth + 4), H2D, w.stream));
-        EAFP_TRY(hipMemcpyAsync(c + nr + nth + 8, f->z, sizeof(double) * (nz + 4), H2D, w.stream));
+        EAFP_TRY(hipMemcpyAsync(c + nr + nth + 8, f->z, sizeof(double) * (d->nz + 4), H2D, w.stream));
         a.r = c;
         a.th = c + nr + 4;
         a.z = c + nr + nth + 8;
+        const double* dev3[NF3];
         for (int q = 0; q < NF3; ++q) {
             EAFP_TRY(hipMemcpyAsync(w.f3.p + (long)q * n, f3[q], sizeof(double) * n, H2D, w.stream));
             dev3[q] = w.f3.p + (long)q * n;
         }
+        pp_args_fields(a, dev3);
         EAFP_TRY(hipMemcpyAsync(w.ct.p, f->cell_type, sizeof(int) * n, H2D, w.stream));
         EAFP_TRY(hipMemcpyAsync(w.outlet.p, f->outlet, sizeof(int) * n2, H2D, w.stream));
         a.ct = w.ct.p;
@@ -672,34 +728,21 @@ int pp_assemble(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_confi
         } else {
             a.liq_room = nullptr;
         }
+        double** dst_out[8] = {&a.aW, &a.aE, &a.aS, &a.aN, &a.aB, &a.aT, &a.aP, &a.Su};
+        for (int q = 0; q < 8; ++q) *dst_out[q] = w.outs.p + (long)q * n;
+        if (ex)
+            for (int dd = 0; dd < 3; ++dd) {
+                a.Fs[dd] = w.exd.p + (long)dd * n;
+                a.ac[dd] = w.exd.p + (long)(3 + dd) * n;
+                a.lk[dd] = w.lki.p + (long)dd * n;
+            }
     }
-    a.vol = dev3[0]; a.Ar = dev3[1]; a.Ath = dev3[2]; a.Az = dev3[3];
-    a.al = dev3[4]; a.url = dev3[5]; a.uthl = dev3[6]; a.uzl = dev3[7];
-    a.aPrl = dev3[8]; a.aPthl = dev3[9]; a.aPzl = dev3[10]; a.rhol = dev3[11]; a.Tl = dev3[12];
-    a.ag = dev3[13]; a.urg = dev3[14]; a.uthg = dev3[15]; a.uzg = dev3[16];
-    a.aPrg = dev3[17]; a.aPthg = dev3[18]; a.aPzg = dev3[19]; a.rhog = dev3[20]; a.Toldg = dev3[21];
-    a.p = dev3[22];
-    a.dt = cfg->dt;
-    a.rho_gas = cfg->rho_gas;
-    a.T_amb = cfg->T_ambient;
-    a.beta = cfg->beta_expansion;
-    a.two_phase = cfg->gas_in_poisson && cfg->solve_multiphase;
-    a.compress = cfg->gas_compressibility && cfg->solve_multiphase;
     a.gr = w.grads.p;
     a.gth = w.grads.p + n;
     a.gz = w.grads.p + 2 * n;
     a.gzg = w.grads.p + 3 * n;
-    double** dst_out[8] = {&a.aW, &a.aE, &a.aS, &a.aN, &a.aB, &a.aT, &a.aP, &a.Su};
-    for (int q = 0; q < 8; ++q) *dst_out[q] = on_device ? out[q] : w.outs.p + (long)q * n;
     a.part = w.part.p;
-    if (ex)
-        for (int dd = 0; dd < 3; ++dd) {
-            a.Fs[dd] = on_device ? ex_d[dd] : w.exd.p + (long)dd * n;
-            a.ac[dd] = on_device ? ex_d[3 + dd] : w.exd.p + (long)(3 + dd) * n;
-            a.lk[dd] = on_device ? ex_i[dd] : w.lki.p + (long)dd * n;
-        }
     const int blocks = pp_blocks(n, a.sz);
-    a.chunk = pp_chunk(a.sz);
     k_pp_grad<<<blocks, BLOCK, 0, w.stream>>>(a);
     k_pp_assemble<<<blocks, BLOCK, 0, w.stream>>>(a);
     EAFP_TRY(hipGetLastError());
@@ -848,6 +891,109 @@ int pp_correct(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config
         EAFP_TRY(hipMemcpyAsync(io->p, a.p_out, nd, D2H, w.stream));
     }
     EAFP_TRY(hipStreamSynchronize(w.stream));
+    return EAFP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free assembly (eafp_pp_assemble_noalloc).  The kernels' scratch
+// -- the four gradient arrays and the continuity residual's block partials
+// and sums -- lives in `scratch`, a static device array of the library sized
+// for EAFP_NOALLOC_MAX_CELLS padded cells (32.5 MiB; each GPU the library
+// runs on holds one until the process ends).  Host code never names the
+// array itself, only its device address from hipGetSymbolAddress, taken
+// once per device with that device current.  Every call on a GPU shares its
+// array, which keeps the last call's contents: the kernels write every
+// value they later read (every padded cell's gradients, every block's
+// partials), so nothing needs clearing; calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr long NA_CELLS = EAFP_NOALLOC_MAX_CELLS;
+constexpr long NA_BLOCKS = 32768;                        // block partial pairs
+constexpr long NA_DOUBLES = 4 * NA_CELLS + 2 * NA_BLOCKS + 32;   // gradients, partials, the two sums (padded)
+static_assert(NA_DOUBLES % 32 == 0, "256-byte multiple");
+
+static __device__ __align__(256) double scratch[NA_DOUBLES];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_na_mutex;
+double* g_na_ptr[NA_MAX_DEVICES] = {};
+
+// The current device's address of `scratch` (looked up on its first use).
+int noalloc_scratch(double** out) {
+    int dev = -1;
+    EAFP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NA_MAX_DEVICES) return EAFP_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (!g_na_ptr[dev]) {
+        void* p = nullptr;
+        EAFP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return EAFP_ERR_STATE;
+        g_na_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_na_ptr[dev];
+    return EAFP_OK;
+}
+
+// The two sums of the block partials in block order (the host's order in
+// pp_assemble, so the same bits): the block stages 256 pairs in LDS, one
+// thread adds them in turn.
+__global__ __launch_bounds__(BLOCK) void k_pp_partsum(const double* part, int blocks, double* sums) {
+    __shared__ double sh[2 * BLOCK];
+    double fl = 0.0, su = 0.0;
+    for (int b0 = 0; b0 < blocks; b0 += BLOCK) {
+        const int b = b0 + (int)threadIdx.x;
+        if (b < blocks) {
+            sh[2 * threadIdx.x] = part[2 * (long)b];
+            sh[2 * threadIdx.x + 1] = part[2 * (long)b + 1];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int m = min(BLOCK, blocks - b0);
+            for (int q = 0; q < m; ++q) {
+                fl += sh[2 * q];
+                su += sh[2 * q + 1];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sums[0] = fl;
+        sums[1] = su;
+    }
+}
+
+}  // namespace
+
+int pp_assemble_noalloc(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config* cfg, double* const out[8],
+                        double* residual, double* flux_ref, const eafp_pp_exports* ex, hipStream_t stream) {
+    double* ex_d[6];
+    int* ex_i[3];
+    EAFP_TRY_RC(pp_assemble_check(d, f, cfg, out, ex, ex_d, ex_i));
+    const long n = (long)(d->nr + 4) * (d->ntheta + 4) * (d->nz + 4);
+    if (n > NA_CELLS) return EAFP_ERR_SIZE;
+    PPArgs a{};
+    pp_args_common(a, d, cfg);
+    const long blocks = pp_blocks(n, a.sz);
+    if (blocks > NA_BLOCKS) return EAFP_ERR_SIZE;
+    double* sc = nullptr;
+    EAFP_TRY_RC(noalloc_scratch(&sc));
+    pp_args_device(a, f, out, ex_d, ex_i);
+    a.gr = sc;
+    a.gth = sc + n;
+    a.gz = sc + 2 * n;
+    a.gzg = sc + 3 * n;
+    a.part = sc + 4 * NA_CELLS;
+    double* sums = a.part + 2 * NA_BLOCKS;
+    k_pp_grad<<<(int)blocks, BLOCK, 0, stream>>>(a);
+    k_pp_assemble<<<(int)blocks, BLOCK, 0, stream>>>(a);
+    k_pp_partsum<<<1, BLOCK, 0, stream>>>(a.part, (int)blocks, sums);
+    EAFP_TRY(hipGetLastError());
+    double hs[2] = {0.0, 0.0};
+    EAFP_TRY(hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, stream));
+    EAFP_TRY(hipStreamSynchronize(stream));
+    if (flux_ref) *flux_ref = hs[0];
+    if (residual) *residual = hs[1] / std::max(hs[0], SMALL);
     return EAFP_OK;
 }
 

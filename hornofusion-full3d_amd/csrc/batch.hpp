@@ -190,6 +190,8 @@ int pp_assemble(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_confi
                 double* residual, double* flux_ref, const eafp_pp_exports* ex, int on_device);
 int pp_correct(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config* cfg, double alpha_p,
                const double* pp, const eafp_pp_exports* ex, const eafp_pp_update* io, int on_device);
+int pp_assemble_noalloc(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config* cfg, double* const out[8],
+                        double* residual, double* flux_ref, const eafp_pp_exports* ex, hipStream_t stream);
 void pp_release();
 int pp_step(const eafp_desc* d, const eafp_pp_fields* f, const eafp_pp_config* cfg, const eafp_pp_step_opts* o,
             double* pp, const eafp_pp_update* io, double* residual, eafp_stats* stats, int on_device);

@@ -107,6 +107,14 @@ int eafp_pp_assemble(const eafp_desc* desc, const eafp_pp_fields* fields, const 
     return eafp::pp_assemble(desc, fields, cfg, out, residual, flux_ref, exports, arrays_on_device);
 }
 
+int eafp_pp_assemble_noalloc(const eafp_desc* desc, const eafp_pp_fields* fields, const eafp_pp_config* cfg,
+                             double* aW, double* aE, double* aS, double* aN, double* aB, double* aT, double* aP,
+                             double* Su, double* residual, double* flux_ref, eafp_pp_exports* exports, void* stream) {
+    double* out[8] = {aW, aE, aS, aN, aB, aT, aP, Su};
+    return eafp::pp_assemble_noalloc(desc, fields, cfg, out, residual, flux_ref, exports,
+                                     static_cast<hipStream_t>(stream));
+}
+
 int eafp_pp_correct(const eafp_desc* desc, const eafp_pp_fields* fields, const eafp_pp_config* cfg, double alpha_p,
                     const double* pp, const eafp_pp_exports* exports, const eafp_pp_update* io,
                     int arrays_on_device) {

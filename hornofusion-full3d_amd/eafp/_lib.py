@@ -18,6 +18,8 @@ EAFP_ERR_UNSUPPORTED = -2
 EAFP_ERR_COMPONENTS = -3
 EAFP_ERR_NO_DEVICE = -4
 EAFP_ERR_STATE = -5
+EAFP_ERR_SIZE = -6
+NOALLOC_MAX_CELLS = 1048576       # EAFP_NOALLOC_MAX_CELLS
 
 STOP_NAMES = {0: "running", 1: "converged", 2: "breakdown", 3: "max_iter", 4: "zero_rhs"}
 
@@ -26,7 +28,7 @@ PRECOND_JACOBI, PRECOND_ZLINE = 0, 1
 
 # Every symbol declared in include/eafp.h.
 EXPORTS = (
-    "eafp_cg_solve", "eafp_release", "eafp_pp_assemble", "eafp_pp_correct", "eafp_pp_step", "eafp_tdma_solve",
+    "eafp_cg_solve", "eafp_release", "eafp_pp_assemble", "eafp_pp_assemble_noalloc", "eafp_pp_correct", "eafp_pp_step", "eafp_tdma_solve",
     "eafp_tdma_solve_batch", "eafp_batch_create", "eafp_batch_upload",
     "eafp_batch_solve_fp64", "eafp_batch_solve_mixed", "eafp_batch_download",
     "eafp_batch_info", "eafp_batch_set_tiling", "eafp_batch_set_columns", "eafp_batch_get_columns",
@@ -77,7 +79,7 @@ class EafpError(RuntimeError):
     def __init__(self, rc, where):
         names = {EAFP_ERR_ARG: "invalid argument", EAFP_ERR_UNSUPPORTED: "unsupported mesh/descriptor",
                  EAFP_ERR_COMPONENTS: "too many singular components", EAFP_ERR_NO_DEVICE: "no HIP device",
-                 EAFP_ERR_STATE: "invalid state"}
+                 EAFP_ERR_STATE: "invalid state", EAFP_ERR_SIZE: "mesh beyond the allocation-free scratch"}
         msg = names.get(rc, f"HIP error {rc}" if rc > 0 else f"error {rc}")
         super().__init__(f"{where}: {msg} (rc={rc})")
         self.rc = rc
@@ -157,6 +159,9 @@ def load():
     L.eafp_pp_assemble.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(PPFields), ctypes.POINTER(PPConfig)] + \
         [vp] * 8 + [_dp, _dp, ctypes.POINTER(PPExports), i]
     L.eafp_pp_assemble.restype = i
+    L.eafp_pp_assemble_noalloc.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(PPFields), ctypes.POINTER(PPConfig)] + \
+        [vp] * 8 + [_dp, _dp, ctypes.POINTER(PPExports), vp]
+    L.eafp_pp_assemble_noalloc.restype = i
     L.eafp_pp_correct.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(PPFields), ctypes.POINTER(PPConfig), d, vp,
                                   ctypes.POINTER(PPExports), ctypes.POINTER(PPUpdate), i]
     L.eafp_pp_correct.restype = i

@@ -34,6 +34,7 @@ extern "C" {
 #define EAFP_ERR_COMPONENTS (-3)   /* more singular components than the projection holds */
 #define EAFP_ERR_NO_DEVICE (-4)    /* no HIP device visible */
 #define EAFP_ERR_STATE (-5)        /* solve before upload, index out of range */
+#define EAFP_ERR_SIZE (-6)         /* mesh beyond an allocation-free entry point's scratch */
 
 /* Mesh descriptor: the fields of mesh_t that cg_3d_mpi reads
  * (src/mod_types_3d.f90:143-193; bounds() src/mod_solver_3d.f90:573-585). */
@@ -176,6 +177,28 @@ int eafp_pp_assemble(const eafp_desc* desc, const eafp_pp_fields* fields, const 
                      double* aW, double* aE, double* aS, double* aN, double* aB, double* aT,
                      double* aP, double* Su, double* residual, double* flux_ref,
                      eafp_pp_exports* exports, int arrays_on_device);
+
+/* Largest padded mesh, (nr+4)*(ntheta+4)*(nz+4) cells, that
+ * eafp_pp_assemble_noalloc takes (60x120x84 is 698,368). */
+#define EAFP_NOALLOC_MAX_CELLS 1048576
+
+/* eafp_pp_assemble with every array in device memory, for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  No counterpart in the
+ * reference.  The same kernels and results, bit for bit (the residual's
+ * block sums are reduced on the device in the same order), but no
+ * hipMalloc, hipFree or other allocating call: the kernels' scratch (the
+ * cell gradients, the residual's block sums) is a static device array of
+ * the library, 32.5 MiB held by each GPU the library runs on until the
+ * process ends.  Meshes beyond EAFP_NOALLOC_MAX_CELLS padded cells:
+ * EAFP_ERR_SIZE.  Runs on `stream` (a hipStream_t; NULL: the default
+ * stream) and returns synchronised; residual and flux_ref are host
+ * pointers (may be NULL).  Calls on one GPU share its scratch array, so
+ * calls that overlap on one GPU (from other threads or streams) race. */
+int eafp_pp_assemble_noalloc(const eafp_desc* desc, const eafp_pp_fields* fields, const eafp_pp_config* cfg,
+                             double* aW, double* aE, double* aS, double* aN, double* aB, double* aT,
+                             double* aP, double* Su, double* residual, double* flux_ref,
+                             eafp_pp_exports* exports, void* stream);
 
 /* State the post-solve consumers write (padded arrays). */
 typedef struct {

@@ -139,6 +139,30 @@ def test_pp_field_shapes_checked(name, bad):
         eafp.pp_correct(F, cfg, m, 0.3, np.zeros(pad, order="F"), ex)
 
 
+def test_pp_assemble_noalloc_refuses_before_the_device():
+    """The allocation-free assembly checks its arguments and its size limit
+    before any device call: NULL arrays, a parallel mesh, and a mesh beyond
+    EAFP_NOALLOC_MAX_CELLS padded cells (EAFP_ERR_SIZE)."""
+    from eafp import fields
+    shape = (4, 6, 5)
+    F, cfg = fields.make_fields(*shape)
+    f, c, keep = eafp.solver._pp_structs(F, cfg, eafp.Mesh(*shape))
+    L = _lib.load()
+    pad = np.zeros(tuple(n + 4 for n in shape), order="F")
+    outs = [pad.ctypes.data] * 8
+    big = eafp.solver.desc_of(eafp.Mesh(100, 100, 100))
+    assert 104 ** 3 > _lib.NOALLOC_MAX_CELLS
+    assert L.eafp_pp_assemble_noalloc(ctypes.byref(big), ctypes.byref(f), ctypes.byref(c), *outs,
+                                      None, None, None, None) == _lib.EAFP_ERR_SIZE
+    d = eafp.solver.desc_of(eafp.Mesh(*shape))
+    assert L.eafp_pp_assemble_noalloc(ctypes.byref(d), ctypes.byref(f), ctypes.byref(c), *outs[:7], None,
+                                      None, None, None, None) == _lib.EAFP_ERR_ARG
+    par = eafp.solver.desc_of(eafp.Mesh(*shape, is_parallel=True))
+    assert L.eafp_pp_assemble_noalloc(ctypes.byref(par), ctypes.byref(f), ctypes.byref(c), *outs,
+                                      None, None, None, None) == _lib.EAFP_ERR_UNSUPPORTED
+    del keep
+
+
 def test_pp_post_solve_struct_layouts():
     # eafp_pp_exports: 9 pointers; eafp_pp_update: 13 pointers
     assert ctypes.sizeof(_lib.PPExports) == 9 * 8
