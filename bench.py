@@ -581,6 +581,8 @@ def main():
     ap.add_argument("--precond", choices=("jacobi", "zline"), default="zline",
                     help="inner preconditioner of the mixed solver")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-jacobi", action="store_true",
+                    help="skip the Jacobi-inner leg of the z-line bench (one solver graph per run, for tracing)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-reference", action="store_true", help="skip the FP64 legs and error checks")
     ap.add_argument("--no-extras", action="store_true",
@@ -744,7 +746,7 @@ def main():
                            "max_true_rel_residual": float(max(res)), "algorithmic_GBps": round(solve_gbs, 1),
                            "autotune": tune}}
     xs_jac = None
-    if zline:
+    if zline and not args.no_jacobi:
         # the same batch with the Jacobi inner preconditioner, timed the same way (not retuned)
         batch.solve_mixed(tol=1e-8, interval=args.interval, coef_format=fmt)
         device_sync()

@@ -188,12 +188,18 @@ constexpr int COL_MAX_NZ = 256;
 constexpr int COL_CELLS = 4 * BLOCK;   // cells of one plane of a column, at most
 
 struct ColDesc {
-    int tile;      // the system's first tile (its TileDesc carries the geometry); -1: empty
+    int tile;      // the system's first tile; -1: empty
     int j0, nj;    // theta rows j0 .. j0+nj-1
     int i0, i1;    // r planes of the segment
     int seg;       // index among the system's segments (0 writes the system's state)
     int nseg;      // segments of the system
     int dir;       // +1: the march goes up in r, -1: down (odd segments of a column)
+    // the system, as its TileDesc carries it: a block needs this one load
+    // before it can address its cells
+    int sys, nr, nth, nz;
+    int tile_end;            // the system's tiles: tile .. tile_end-1
+    int lt_begin, lt_end;    // the system's line tiles
+    long off;                // first cell of the system in the arenas
 };
 
 // Rows per column for a given nz (every thread holds one quad of a row).
@@ -235,6 +241,14 @@ inline bool make_col_table(const std::vector<SysGeom>& g, int splits, std::vecto
             for (int sp = 0; sp < nsp; ++sp) {
                 ColDesc d{};
                 d.tile = G.tile_begin;
+                d.sys = (int)(&G - g.data());
+                d.nr = G.nr;
+                d.nth = G.nth;
+                d.nz = G.nz;
+                d.tile_end = G.tile_end;
+                d.lt_begin = G.lt_begin;
+                d.lt_end = G.lt_end;
+                d.off = G.off;
                 d.j0 = c * rows;
                 d.nj = (G.nth - d.j0) < rows ? (G.nth - d.j0) : rows;
                 d.i0 = (int)(((long)G.nr * sp) / nsp);

@@ -94,6 +94,7 @@ struct MxArgs {
     // inner FP32 vectors
     float *d, *r, *z, *q;
     float* p[2];
+    long sink;           // a 16-byte slot past the cells of p[0], p[1] and q: stores that must be issued but not kept
     double* partR;       // ntiles * NRES (k_res)
     double* sumR;        // nsys * NRES: partR reduced per system (k_rsum)
     const SysGeom* geom; // per-system geometry and tile ranges
@@ -835,25 +836,99 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate4(MxArgs a, int par) {
 }
 
 // ---------------------------------------------------------------------------
+// Weights of one quad as loaded, decoded only where they are used: a weight
+// converted right after its load makes the wave wait for that load there.
+// ---------------------------------------------------------------------------
+template <int F>
+struct WRaw;
+template <>
+struct WRaw<EAFP_COEF_FP32> {
+    F4 v;
+};
+template <>
+struct WRaw<EAFP_COEF_BF16> {
+    unsigned x, y;   // two BF16 each, low half first
+};
+template <>
+struct WRaw<EAFP_COEF_FP16> {
+    unsigned x, y;   // two FP16 each
+    F4 s;            // the row scales
+};
+
+template <int F>
+__device__ __forceinline__ WRaw<F> wraw(const void* w, const float* sc, long o);
+template <>
+__device__ __forceinline__ WRaw<EAFP_COEF_FP32> wraw<EAFP_COEF_FP32>(const void* w, const float*, long o) {
+    return {ld4((const float*)w + o)};
+}
+template <>
+__device__ __forceinline__ WRaw<EAFP_COEF_BF16> wraw<EAFP_COEF_BF16>(const void* w, const float*, long o) {
+    WRaw<EAFP_COEF_BF16> r;
+    __builtin_memcpy(&r, __builtin_assume_aligned((const unsigned short*)w + o, 8), 8);
+    return r;
+}
+template <>
+__device__ __forceinline__ WRaw<EAFP_COEF_FP16> wraw<EAFP_COEF_FP16>(const void* w, const float* sc, long o) {
+    WRaw<EAFP_COEF_FP16> r;
+    __builtin_memcpy(&r, __builtin_assume_aligned((const _Float16*)w + o, 8), 8);
+    r.s = ld4(sc + o);
+    return r;
+}
+
+// The same values as WFmt<F>::load4 at the same cells.
+__device__ __forceinline__ F4 wdec(const WRaw<EAFP_COEF_FP32>& r) { return r.v; }
+__device__ __forceinline__ F4 wdec(const WRaw<EAFP_COEF_BF16>& r) {
+    F4 f;
+    f.v[0] = __uint_as_float(r.x << 16);
+    f.v[1] = __uint_as_float(r.x & 0xffff0000u);
+    f.v[2] = __uint_as_float(r.y << 16);
+    f.v[3] = __uint_as_float(r.y & 0xffff0000u);
+    return f;
+}
+__device__ __forceinline__ F4 wdec(const WRaw<EAFP_COEF_FP16>& r) {
+    _Float16 h[4];
+    __builtin_memcpy(h, &r, 8);
+    F4 f = r.s;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) f.v[q] *= (float)h[q];
+    return f;
+}
+
+// The loads of one step of the column march (centre plane i): z and p_old
+// of the plane beyond it (i+dir), its weights (w0 of its outer face, w1, w2,
+// e) and its theta halo rows (z, p_old, w1 of row -1).
+template <int F>
+struct ColSet {
+    F4 zE, oE, zH, oH;
+    WRaw<F> r0, r1, r2, re, r1H;
+};
+
+// ---------------------------------------------------------------------------
 // Column-marching form of k_mspmv (same state logic, same arithmetic per
-// cell).  A block takes one column segment (ColDesc): nj theta-rows of one
-// system over the r-planes i0 .. i1-1; each thread holds one quad (4
-// z-cells of a row).  p = z + beta*p_old of the planes i-1, i and i+1 stays
-// in registers, so the r-neighbours are loaded once, as the plane itself;
-// p and the +theta weight of plane i go to a double-buffered LDS slab (with
-// the two theta halo rows, loaded by the first 2*nz/4 threads), from which
-// the theta and z neighbours are read: one barrier per plane.  Per cell it
-// reads z, p_old, the 3 weights and e once, plus the halo rows (z and p_old
-// of 2 rows, w1 of one, per nj rows), and writes p and q.  A segment marches
-// up in r (C.dir = +1) or down (-1): the segments of a column alternate, so
-// the two segments on either side of a boundary read the planes there (each
-// one's extra plane beyond its range) at about the same time, and the
-// second read finds them in L2.  Registers keep W/E by position in r, so
-// the arithmetic per cell is the same in both directions.
+// cell).  A block takes one column segment (ColDesc, which carries the
+// system's geometry): nj theta-rows of one system over the r-planes
+// i0 .. i1-1; each thread holds one quad (4 z-cells of a row).  p = z +
+// beta*p_old of the planes i-1, i and i+1 stays in registers, so the
+// r-neighbours are loaded once, as the plane itself; p and the +theta weight
+// of plane i go to a double-buffered LDS slab (with the two theta halo rows,
+// loaded by the first 2*nz/4 threads), from which the theta and z
+// neighbours are read: one barrier per plane.  Per cell it reads z, p_old,
+// the 3 weights and e once, plus the halo rows (z and p_old of 2 rows, w1 of
+// one, per nj rows), and writes p and q.  A segment marches up in r
+// (C.dir = +1) or down (-1): the segments of a column alternate, so the two
+// segments on either side of a boundary read the planes there (each one's
+// extra plane beyond its range) at about the same time, and the second read
+// finds them in L2.
+// Pipelining: every load is issued one whole step before its use and no
+// loaded value is carried across the loop's back-edge.  The march is
+// unrolled by two over two ColSets: step t first issues the loads of step
+// t+1 into one set, then works from the other, loaded during step t-1.  The
+// weights stay raw until after the plane's barrier, and every lane issues
+// the same memory instructions (lanes past the slab store to the sink past
+// the arenas), so the waits of a step never cover the loads it just issued.
 // ---------------------------------------------------------------------------
 template <int F>
 __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
-    using W = WFmt<F>;
     constexpr int SP = COL_CELLS + 2 * COL_MAX_NZ;   // p slab: rows -1 .. nj
     constexpr int SW = COL_CELLS + COL_MAX_NZ;       // w1 slab: rows -1 .. nj-1
     __shared__ double sh[2 * (BLOCK / 64)];
@@ -863,13 +938,12 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
     const int G = gridDim.x;
     const ColDesc C = a.cols[(blockIdx.x % NXCD) * (G / NXCD) + blockIdx.x / NXCD];
     if (C.tile < 0) return;
-    const TileDesc T = a.tiles[C.tile];
-    const int s = T.sys;
+    const int s = C.sys;
     const float* __restrict__ pold = a.p[par];
     float* __restrict__ pnew = a.p[par ^ 1];
     const float* __restrict__ z = a.z;
     float* __restrict__ qo = a.q;
-    const int nz = T.nz, nth = T.nth, nr = T.nr;
+    const int nz = C.nz, nth = C.nth, nr = C.nr;
     const int Qr = nz >> 2;                       // quads per row
     const int tid = threadIdx.x;
     const int row = tid / Qr, kq = tid - row * Qr;
@@ -883,31 +957,35 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
     const long sr = (long)nth * nz;
     // a plane outside 0 .. nr-1 is clamped to the nearest one: its loads stay
     // in the system, and as a neighbour it equals the cell's own p (face term 0)
-    auto base = [&](int i) { return T.off + (long)(i < 0 ? 0 : i < nr ? i : nr - 1) * sr; };
+    auto base = [&](int i) { return C.off + (long)(i < 0 ? 0 : i < nr ? i : nr - 1) * sr; };
     const int dir = C.dir;
     const int i0 = dir > 0 ? C.i0 : C.i1 - 1;   // first plane of the march
     const int np = C.i1 - C.i0;
 
-    // planes i0-1 (W), i0 and i0+1 (E) of z and p_old, the weights of i0
-    // (w0 is the face to i0+1, w0W the face to i0-1), the halos of i0: in
-    // flight while the partials are reduced (in a batch, only once the
-    // system's state shows it still running)
-    F4 zW, oW, w0W, zC, oC, zE, oE, w0, w1, w2, ev, zH, oH, w1H;
+    auto issue = [&](int i, ColSet<F>& S) {
+        S.zE = ld4(z + base(i + dir) + qoff);
+        S.oE = ld4(pold + base(i + dir) + qoff);
+        S.r0 = wraw<F>(a.w[0], a.scale, base(dir > 0 ? i : i - 1) + qoff);   // the face towards i+dir
+        S.r1 = wraw<F>(a.w[1], a.scale, base(i) + qoff);
+        S.r2 = wraw<F>(a.w[2], a.scale, base(i) + qoff);
+        S.re = wraw<F>(a.e, a.scale, base(i) + qoff);
+        S.zH = ld4(z + base(i) + hoff);
+        S.oH = ld4(pold + base(i) + hoff);
+        S.r1H = wraw<F>(a.w[1], a.scale, base(i) + hoff);
+    };
+    // planes i0-dir and i0 of z and p_old, the face between them, and the
+    // first step's set: in flight while the partials are reduced (in a
+    // batch, only once the system's state shows it still running)
+    F4 zP, oP, zC, oC;
+    WRaw<F> rIn;   // the face between the centre plane and the one before it in the march
+    ColSet<F> A, B;
     auto prologue = [&]() {
-        zW = ld4(z + base(i0 - 1) + qoff);
-        oW = ld4(pold + base(i0 - 1) + qoff);
-        w0W = W::load4(a.w[0], a.scale, base(i0 - 1) + qoff);
+        zP = ld4(z + base(i0 - dir) + qoff);
+        oP = ld4(pold + base(i0 - dir) + qoff);
         zC = ld4(z + base(i0) + qoff);
         oC = ld4(pold + base(i0) + qoff);
-        zE = ld4(z + base(i0 + 1) + qoff);
-        oE = ld4(pold + base(i0 + 1) + qoff);
-        w0 = W::load4(a.w[0], a.scale, base(i0) + qoff);
-        w1 = W::load4(a.w[1], a.scale, base(i0) + qoff);
-        w2 = W::load4(a.w[2], a.scale, base(i0) + qoff);
-        ev = W::load4(a.e, a.scale, base(i0) + qoff);
-        zH = ld4(z + base(i0) + hoff);
-        oH = ld4(pold + base(i0) + hoff);
-        w1H = W::load4(a.w[1], a.scale, base(i0) + hoff);
+        rIn = wraw<F>(a.w[0], a.scale, base(dir > 0 ? i0 - 1 : i0) + qoff);
+        issue(i0, A);
     };
     MxDyn P;
     if (a.state_first) {
@@ -921,7 +999,7 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
     bool skip = P.done || P.broken;
     if (!skip) {
         double tot[2];
-        reduce_tiles<2>(a.partB, a.line ? T.lt_begin : T.tile_begin, a.line ? T.lt_end : T.tile_end, tot, sh);
+        reduce_tiles<2>(a.partB, a.line ? C.lt_begin : C.tile, a.line ? C.lt_end : C.tile_end, tot, sh);
         if (!(tot[1] > 0.0)) {
             skip = true;
             N.broken = 1;
@@ -940,28 +1018,14 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
         for (int q = 0; q < 4; ++q) r.v[q] = zz.v[q] + fb * oo.v[q];
         return r;
     };
-    F4 pW = pv(zW, oW), pC = pv(zC, oC), pE = pv(zE, oE), pH = pv(zH, oH);
+    F4 pW = pv(zP, oP), pC = pv(zC, oC);   // W: the plane before the centre in the march
     double acc[1] = {0.0};
-    for (int t = 0; t < np; ++t) {
+    auto step = [&](int t, ColSet<F>& cur, ColSet<F>& nxt) {
         const int i = i0 + dir * t;
-        const int b = i & 1;
-        // loads for the next plane i+dir: z/p_old of i+2*dir (its outer
-        // neighbour), its weights and halos, and its outer face (w0 of i+1
-        // up, w0 of i-2 down)
-        const bool more = t + 1 < np;
-        F4 zN2, oN2, w0n, w1n, w2n, evn, zHn, oHn, w1Hn;
-        if (more) {
-            const int in = i + dir;
-            zN2 = ld4(z + base(i + 2 * dir) + qoff);
-            oN2 = ld4(pold + base(i + 2 * dir) + qoff);
-            w0n = W::load4(a.w[0], a.scale, base(dir > 0 ? in : in - 1) + qoff);
-            w1n = W::load4(a.w[1], a.scale, base(in) + qoff);
-            w2n = W::load4(a.w[2], a.scale, base(in) + qoff);
-            evn = W::load4(a.e, a.scale, base(in) + qoff);
-            zHn = ld4(z + base(in) + hoff);
-            oHn = ld4(pold + base(in) + hoff);
-            w1Hn = W::load4(a.w[1], a.scale, base(in) + hoff);
-        }
+        const int b = t & 1;
+        issue(t + 1 < np ? i + dir : i, nxt);   // past the last plane: a reload, unused
+        const F4 pE = pv(cur.zE, cur.oE);
+        const F4 w1 = wdec(cur.r1), w2 = wdec(cur.r2);
         // plane i into the slab: p (rows -1 .. nj), w1 (rows -1 .. nj-1), w2 of each quad's last cell
         if (act) {
 #pragma unroll
@@ -971,69 +1035,63 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
             }
         }
         if (hal) {
+            const F4 pH = pv(cur.zH, cur.oH);
 #pragma unroll
             for (int q = 0; q < 4; ++q) Sp[b][(hrow + 1) * nz + hk + q] = pH.v[q];
             if (hrow < 0) {
+                const F4 w1H = wdec(cur.r1H);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) Sw1[b][hk + q] = w1H.v[q];
             }
         }
         Sw2[b][tid] = w2.v[3];
         __syncthreads();
-        if (act) {
-            const float* sp = &Sp[b][(row + 1) * nz + 4 * kq];
-            const float* sw = &Sw1[b][row * nz + 4 * kq];
-            const bool hW = i > 0;
-            const float pB0 = kq > 0 ? sp[-1] : pC.v[0];
-            const float pT3 = kq < Qr - 1 ? sp[4] : pC.v[3];
-            const float wB0 = kq > 0 ? Sw2[b][tid - 1] : 0.f;
-            F4 y;
+        // +r face of the cell: w0 of plane i; -r face: w0 of plane i-1
+        const F4 w0 = wdec(dir > 0 ? cur.r0 : rIn), w0W = wdec(dir > 0 ? rIn : cur.r0), ev = wdec(cur.re);
+        const F4& pRE = dir > 0 ? pE : pW;   // p of plane i+1
+        const F4& pRW = dir > 0 ? pW : pE;   // p of plane i-1
+        const float* sp = &Sp[b][(row + 1) * nz + 4 * kq];
+        const float* sw = &Sw1[b][row * nz + 4 * kq];
+        const bool hW = i > 0;
+        const float pB0 = kq > 0 ? sp[-1] : pC.v[0];
+        const float pT3 = kq < Qr - 1 ? sp[4] : pC.v[3];
+        const float wB0 = kq > 0 ? Sw2[b][tid - 1] : 0.f;
+        F4 y;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float pc = pC.v[q];
-                const float pS = sp[q - nz], pN = sp[q + nz];
-                const float pT = q == 3 ? pT3 : pC.v[q + 1];
-                const float pB = q == 0 ? pB0 : pC.v[q - 1];
-                const float wb = q == 0 ? wB0 : w2.v[q - 1];
-                float yy = ev.v[q] * pc;
-                yy += w0.v[q] * (pc - pE.v[q]);
-                yy += (hW ? w0W.v[q] : 0.f) * (pc - pW.v[q]);
-                yy += w1.v[q] * (pc - pN);
-                yy += sw[q] * (pc - pS);
-                yy += w2.v[q] * (pc - pT);
-                yy += wb * (pc - pB);
-                y.v[q] = yy;
-            }
-            const long o = base(i) + qoff;
-            st4(pnew + o, pC);
-            st4(qo + o, y);
+        for (int q = 0; q < 4; ++q) {
+            const float pc = pC.v[q];
+            const float pS = sp[q - nz], pN = sp[q + nz];
+            const float pT = q == 3 ? pT3 : pC.v[q + 1];
+            const float pB = q == 0 ? pB0 : pC.v[q - 1];
+            const float wb = q == 0 ? wB0 : w2.v[q - 1];
+            float yy = ev.v[q] * pc;
+            yy += w0.v[q] * (pc - pRE.v[q]);
+            yy += (hW ? w0W.v[q] : 0.f) * (pc - pRW.v[q]);
+            yy += w1.v[q] * (pc - pN);
+            yy += sw[q] * (pc - pS);
+            yy += w2.v[q] * (pc - pT);
+            yy += wb * (pc - pB);
+            y.v[q] = yy;
+        }
+        const long o = act ? base(i) + qoff : a.sink;
+        st4(pnew + o, pC);
+        st4(qo + o, y);
+        if (act) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) acc[0] += (double)pC.v[q] * (double)y.v[q];
         }
-        if (more) {
-            const F4 pn = pv(zN2, oN2);
-            if (dir > 0) {
-                pW = pC;
-                pC = pE;
-                pE = pn;
-                w0W = w0;
-                w0 = w0n;
-            } else {
-                pE = pC;
-                pC = pW;
-                pW = pn;
-                w0 = w0W;
-                w0W = w0n;
-            }
-            w1 = w1n;
-            w2 = w2n;
-            ev = evn;
-            pH = pv(zHn, oHn);
-            w1H = w1Hn;
-        }
+        rIn = cur.r0;   // the face between i and i+dir: the next centre's inner face
+        pW = pC;
+        pC = pE;
+    };
+    int t = 0;
+    for (; t + 1 < np; t += 2) {
+        step(t, A, B);
+        step(t + 1, B, A);
     }
+    if (t < np) step(t, A, B);
     block_sum<1>(acc, sh);
-    if (tid == 0) a.partA[(long)T.tile_begin + C.seg] = acc[0];   // the consumers read nseg slots
+    if (tid == 0) a.partA[(long)C.tile + C.seg] = acc[0];   // the consumers read nseg slots
 }
 
 // ---------------------------------------------------------------------------
@@ -1727,6 +1785,8 @@ __global__ __launch_bounds__(BLOCK) void k_mproject(MxArgs a, int tile) {
     }
 }
 
+constexpr long SINK_CELLS = 64;   // past the arenas of p and q: the column stencil's discarded stores
+
 struct MixedWork {
     int fmt = -1;
     DevBuf<double> X0, X1, r64, xout, partR, partA, partB, sumR;
@@ -1772,12 +1832,12 @@ static int init_mixed(Batch& B, MixedWork* w) {
         EAFP_TRY_RC(w->d.alloc(n));
         EAFP_TRY_RC(w->r.alloc(n));
         EAFP_TRY_RC(w->z.alloc(n));
-        EAFP_TRY_RC(w->q.alloc(n));
-        EAFP_TRY_RC(w->p0.alloc(n));
-        EAFP_TRY_RC(w->p1.alloc(n));
+        EAFP_TRY_RC(w->q.alloc(n + SINK_CELLS));
+        EAFP_TRY_RC(w->p0.alloc(n + SINK_CELLS));
+        EAFP_TRY_RC(w->p1.alloc(n + SINK_CELLS));
         EAFP_TRY(hipMemset(w->d.p, 0, n * sizeof(float)));
-        EAFP_TRY(hipMemset(w->p0.p, 0, n * sizeof(float)));
-        EAFP_TRY(hipMemset(w->p1.p, 0, n * sizeof(float)));
+        EAFP_TRY(hipMemset(w->p0.p, 0, (n + SINK_CELLS) * sizeof(float)));
+        EAFP_TRY(hipMemset(w->p1.p, 0, (n + SINK_CELLS) * sizeof(float)));
         EAFP_TRY_RC(w->dyn0.alloc(B.nsys));
         EAFP_TRY_RC(w->dyn1.alloc(B.nsys));
         EAFP_TRY_RC(w->stat0.alloc(B.nsys));
@@ -1849,6 +1909,7 @@ static int ensure_mixed(Batch& B, int fmt) {
     a.q = w->q.p;
     a.p[0] = w->p0.p;
     a.p[1] = w->p1.p;
+    a.sink = B.ncells;   // a multiple of 32 cells: 16-byte aligned
     a.partR = w->partR.p;
     a.sumR = w->sumR.p;
     a.geom = B.geom_d.p;

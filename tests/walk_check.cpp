@@ -182,6 +182,11 @@ static void check_columns(std::vector<SysGeom> g, int cpt, int splits) {
         const int Qr = T.nz / 4;
         CHECK(C.nj >= 1 && C.nj * T.nz <= COL_CELLS && C.j0 + C.nj <= T.nth && C.i0 < C.i1 && C.i1 <= T.nr,
               "column shape");
+        // the geometry the column descriptor carries is its system's
+        CHECK(C.sys == T.sys && C.off == T.off && C.nr == T.nr && C.nth == T.nth && C.nz == T.nz &&
+                  C.tile == T.tile_begin && C.tile_end == T.tile_end && C.lt_begin == T.lt_begin &&
+                  C.lt_end == T.lt_end,
+              "column descriptor geometry");
         CHECK(C.dir == 1 || C.dir == -1, "march direction");
         auto base = [&](int i) { return T.off + (long)(i < 0 ? 0 : i < T.nr ? i : T.nr - 1) * sr; };
         const int ib = C.dir > 0 ? C.i0 : C.i1 - 1, np = C.i1 - C.i0;
@@ -194,19 +199,25 @@ static void check_columns(std::vector<SysGeom> g, int cpt, int splits) {
             const int hk = hal ? 4 * (tid < Qr ? tid : tid - Qr) : 0;
             const long hoff = (long)(((C.j0 + hrow) % T.nth + T.nth) % T.nth) * T.nz + hk;
             auto vec = [&](long a) { CHECK(a % 4 == 0 && a >= lo && a + 4 <= hi, "column vector %ld", a); };
-            vec(base(ib - 1) + qoff);
+            // one step's loads (ColSet): z, p_old of i+dir; the face towards i+dir, w1, w2, e of i; halos of i
+            auto issue = [&](int i) {
+                vec(base(i + C.dir) + qoff);
+                vec(base(C.dir > 0 ? i : i - 1) + qoff);
+                vec(base(i) + qoff);
+                vec(base(i) + hoff);
+            };
+            // prologue: planes ib-dir and ib, the face between them, the first step's set
+            vec(base(ib - C.dir) + qoff);
             vec(base(ib) + qoff);
-            vec(base(ib + 1) + qoff);
-            vec(base(ib) + hoff);
+            vec(base(C.dir > 0 ? ib - 1 : ib) + qoff);
+            issue(ib);
             for (int t = 0; t < np; ++t) {
                 const int i = ib + C.dir * t;
                 CHECK(i >= C.i0 && i < C.i1, "march plane");
-                if (t + 1 < np) {
-                    const int in = i + C.dir;
-                    vec(base(i + 2 * C.dir) + qoff);
-                    vec(base(C.dir > 0 ? in : in - 1) + qoff);
-                    vec(base(in) + qoff);
-                    vec(base(in) + hoff);
+                issue(t + 1 < np ? i + C.dir : i);
+                if (!act) {
+                    // stores of lanes past the slab go to the sink past the arenas
+                    CHECK(ncells % 4 == 0, "sink alignment");
                 }
                 if (act) {
                     for (int q = 0; q < 4; ++q) {
