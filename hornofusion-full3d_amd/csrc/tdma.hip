@@ -46,6 +46,8 @@
 // order, so every value of phi is the double the sequential loops compute.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -81,6 +83,7 @@ __global__ __launch_bounds__(TD_PASS_BLOCK) void k_tdma_pass(const TdSys* __rest
     __shared__ double sd[TD_LDS_ELEMS], sa[TD_LDS_ELEMS], sm[TD_LDS_ELEMS], sc[TD_LDS_ELEMS];
     const TdSys& s = sys[blockIdx.x];
     const TdPass& p = s.ps[P];
+    if (p.rec && !s.off_band[P]) return;     // k_tdma_chain runs this pass
     if (P != 1) {     // the halo copy before the pass: snapshot of the lines the wrap reads
         const long nsnap = (long)p.n * (P == 0 ? p.nv : p.nu);
         for (long t = threadIdx.x; t < nsnap; t += TD_PASS_BLOCK) snapshot_value(p, P, t);
@@ -109,6 +112,103 @@ __global__ __launch_bounds__(TD_PASS_BLOCK) void k_tdma_pass(const TdSys* __rest
             __syncthreads();
         }
     }
+}
+
+// The per-pass products of the chain kernel's records (prep_elem), one
+// thread per element of every line.
+template <int P>
+__global__ __launch_bounds__(TD_BLOCK) void k_tdma_prep(const TdSys* __restrict__ sys) {
+    const TdPass& p = sys[blockIdx.y].ps[P];
+    const long t = (long)blockIdx.x * TD_BLOCK + threadIdx.x;
+    if (t < (long)p.n * p.NL) prep_elem<P>(p, (int)(t % p.NL), (int)(t / p.NL));
+}
+
+// One forward chunk f for a helper lane: wait at the barriers of the phases
+// up to the one before f's, fill f's ring slot from `use` (loaded two chunks
+// ago), copy the same chunk of the diagonal before out to phi (its last
+// reader was this fill; the chain overwrites it next), issue the loads of
+// chunk f+2 into `next`, and close the phase.  Past the pass's last chunk
+// (`live` false, the same for every lane) and on the first diagonal the
+// copies go to the sink, and past the last chunk nothing is filled and no
+// barrier passed: every call issues the same stores and loads, so the
+// compiler's waits for `use` count exactly the accesses issued after it.
+template <int P>
+__device__ __forceinline__ void helper_chunk(const TdPass& p, const TdLds& L, long& X, TdTask& f, TdTask& fl, int h,
+                                             int HL, bool live, const TdPre (&use)[TDC_IPL], TdPre (&next)[TDC_IPL]) {
+    const long ph = chain_prep_phase(p, f);
+    if (live) {
+        for (; X < ph; ++X) __syncthreads();
+#pragma unroll
+        for (int i = 0; i < TDC_IPL; ++i) chain_prepare<P>(p, L, f, (int)((ph + 1) & 1), h + i * HL, use[i]);
+    }
+    const bool real = live && f.w > 0;
+    int v_lo = 1, hi = 1;
+    if (real) diagonal_range(f.w - 1, p.nu, p.nv, v_lo, hi);
+    const int mp = hi - v_lo + 1, l0p = f.l0 - mp;
+    const unsigned mM = td_magic((unsigned)mp);
+#pragma unroll
+    for (int i = 0; i < TDC_IPL; ++i)
+        chain_copy_out(p, L, real, f.w - 1, v_lo, mp, mM, l0p, f.e0, f.ce, h + i * HL);
+#pragma unroll
+    for (int i = 0; i < TDC_IPL; ++i) chain_load<P>(p, fl, h + i * HL, next[i]);
+    if (live) {
+        __syncthreads();
+        ++X;
+    }
+    chain_advance_fwd(p, f);
+    chain_advance_fwd(p, fl);
+}
+
+// One pass of one system per work-group (the schedule is described in
+// tdma_line.hpp): the first NW waves run the recurrences at raised wave
+// priority, the other waves feed them.  Every wave passes the same number
+// of barriers, one per phase; the branch between the two roles is uniform
+// per wave.
+template <int P>
+__global__ __launch_bounds__(TDC_BLOCK) void k_tdma_chain(const TdSys* __restrict__ sys) {
+    __shared__ __align__(16) double lds[TDC_LDS];
+    if (sys[blockIdx.x].off_band[P]) return;    // a pivot outside the band: k_tdma_pass runs this pass
+    const TdPass p = sys[blockIdx.x].ps[P];     // by value: its fields stay in scalar registers
+    const TdLds L = chain_lds(p, lds);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const long nph = chain_phases(p);
+    if (wave < p.NW) {
+        __builtin_amdgcn_s_setprio(3);
+        const int j = (int)threadIdx.x;
+        double dp = 0.0;
+        TdTask t = chain_first(p);
+        for (long X = 0; X < nph; ++X) {
+            if (t.kind == 1) chain_forward(p, L, t, (int)(X & 1), j, dp);
+            else if (t.kind == 2) chain_backward(p, L, t, j);
+            __syncthreads();
+            chain_advance(p, t);
+        }
+        __builtin_amdgcn_s_setprio(0);
+    } else {
+        const int HL = TDC_BLOCK - 64 * p.NW, h = (int)threadIdx.x - 64 * p.NW;
+        const long nf = (long)(p.nu + p.nv - 1) * p.K;     // forward chunks of the pass
+        TdPre r0[TDC_IPL], r1[TDC_IPL], r2[TDC_IPL];
+        TdTask f = chain_first_fwd(p), fl = f;
+#pragma unroll
+        for (int i = 0; i < TDC_IPL; ++i) chain_load<P>(p, fl, h + i * HL, r0[i]);
+        chain_advance_fwd(p, fl);
+#pragma unroll
+        for (int i = 0; i < TDC_IPL; ++i) chain_load<P>(p, fl, h + i * HL, r1[i]);
+        chain_advance_fwd(p, fl);
+        long X = 0;
+        for (long q = 0; q < nf; q += 3) {
+            helper_chunk<P>(p, L, X, f, fl, h, HL, true, r0, r2);
+            helper_chunk<P>(p, L, X, f, fl, h, HL, q + 1 < nf, r1, r0);
+            helper_chunk<P>(p, L, X, f, fl, h, HL, q + 2 < nf, r2, r1);
+        }
+        for (; X < nph; ++X) __syncthreads();
+    }
+    // the last diagonal's x, after the last barrier
+    TdTask t = chain_first(p);
+    for (int w = 0; w < p.nu + p.nv - 2; ++w) chain_next_diagonal(p, t);
+    const int cnt = p.n * t.m;
+    for (int it = (int)threadIdx.x; it < cnt; it += TDC_BLOCK)
+        chain_copy_out(p, L, true, t.w, t.v_lo, t.m, t.mM, t.l0, 0, p.n, it);
 }
 
 template <int P, int Q>
@@ -149,6 +249,17 @@ int halo_lines(int P, int nu, int nv) { return P == 0 ? 2 * nu + nv : P == 1 ? 2
 int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* sys, int n_sweep, int on_device) {
     if (nsys < 1 || !descs || !sys || n_sweep < 0) return EAFP_ERR_ARG;
     long dwork = 0, iwork = 0, stage = 0, max_lines = 0, max_cells = 0, max_cols = 0;
+    // A pass runs the chain kernel when every system admits it (chain_plan);
+    // EAFP_TDMA_KERNEL=pass keeps k_tdma_pass for every pass.
+    const char* force = std::getenv("EAFP_TDMA_KERNEL");
+    bool chain[3] = {true, true, true};
+    if (force && std::strcmp(force, "pass") == 0) chain[0] = chain[1] = chain[2] = false;
+    for (int q = 0; q < nsys; ++q)
+        for (int P = 0; P < 3; ++P) {
+            TdPass t{};
+            pass_shape(descs[q], P, t.nu, t.nv, t.n);
+            if (!chain_plan(t, 1 << 20)) chain[P] = false;
+        }
     for (int q = 0; q < nsys; ++q) {
         EAFP_TRY_RC(validate_desc(&descs[q]));
         const eafp_tdma_system& y = sys[q];
@@ -163,10 +274,11 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
             int nu, nv, n;
             pass_shape(d, P, nu, nv, n);
             const long NL = (long)nu * nv, NLX = NL + halo_lines(P, nu, nv);
-            dwork += PK_N * ncells + LN_N * NL + (long)n * NLX;
+            dwork += PK_N * ncells + LN_N * NL + (long)n * NLX + (chain[P] ? RC_N * ncells + TDC_BLOCK : 0);
             iwork += nu + nv - 1;
             lines += NL;
         }
+        iwork += 3;     // off_band flags
         stage += 9 * (long)(d.nr + 4) * (d.ntheta + 4) * (d.nz + 4);
         max_lines = std::max(max_lines, lines);
         max_cells = std::max(max_cells, ncells);
@@ -213,6 +325,8 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
             EAFP_TRY(hipMemcpyAsync((void*)t.home, y.phi, sizeof(double) * t.n, hipMemcpyHostToDevice, w.stream));
             soff += 9 * t.n;
         }
+        t.off_band = (decltype(t.off_band))(icur + ioff);   // three zeros of ihost: cleared by the upload below
+        ioff += 3;
         for (int P = 0; P < 3; ++P) {
             TdPass& p = t.ps[P];
             pass_shape(d, P, p.nu, p.nv, p.n);
@@ -225,6 +339,15 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
             dcur += LN_N * (long)p.NL;
             p.phi = (gdptr)dcur;
             dcur += (long)p.n * p.NLX;
+            p.rec = p.sink = nullptr;
+            p.C = p.K = p.ML = p.by_u = p.NW = p.pad_ = 0;
+            if (chain[P]) {
+                if (!chain_plan(p, 1 << 20)) return EAFP_ERR_UNSUPPORTED;   // admitted above: cannot happen
+                p.rec = (gdptr)dcur;
+                dcur += RC_N * t.ncells;
+                p.sink = (gdptr)dcur;
+                dcur += TDC_BLOCK;
+            }
             p.dstart = (cgiptr)(icur + ioff);
             int acc = 0;
             for (int wd = 0; wd <= p.nu + p.nv - 2; ++wd) {
@@ -245,10 +368,24 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
     const TdSys* tp = w.table.p;
     k_tdma_setup<<<g_lines, TD_BLOCK, 0, w.stream>>>(tp);
     for (int sweep = 0; sweep < n_sweep; ++sweep) {
+        // with the chain kernel planned, k_tdma_pass still runs the systems whose
+        // pivots leave its band (each kernel returns at once for the others)
+        if (chain[0]) {
+            k_tdma_prep<0><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
+            k_tdma_chain<0><<<nsys, TDC_BLOCK, 0, w.stream>>>(tp);
+        }
         k_tdma_pass<0><<<nsys, TD_PASS_BLOCK, 0, w.stream>>>(tp);
         k_tdma_move<0, 1><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
+        if (chain[1]) {
+            k_tdma_prep<1><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
+            k_tdma_chain<1><<<nsys, TDC_BLOCK, 0, w.stream>>>(tp);
+        }
         k_tdma_pass<1><<<nsys, TD_PASS_BLOCK, 0, w.stream>>>(tp);
         k_tdma_move<1, 2><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
+        if (chain[2]) {
+            k_tdma_prep<2><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
+            k_tdma_chain<2><<<nsys, TDC_BLOCK, 0, w.stream>>>(tp);
+        }
         k_tdma_pass<2><<<nsys, TD_PASS_BLOCK, 0, w.stream>>>(tp);
         if (sweep + 1 < n_sweep) k_tdma_move<2, 0><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
         else k_tdma_move<2, 3><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);

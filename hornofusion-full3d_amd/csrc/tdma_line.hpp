@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
 
 #include "eafp_common.hpp"
 
@@ -47,6 +48,14 @@ enum { CW = 0, CE, CS, CN, CB, CT, CP_, CSU, TD_NCOEF };
 enum { PK_SU = 0, PK_O1, PK_O2, PK_O3, PK_O4, PK_A, PK_M, PK_C, PK_N };
 // Packed per-line arrays of a pass.
 enum { LN_AMF = 0, LN_APL, LN_LO, LN_HI, LN_N };
+// Per element-line record of a pass for the chain kernel, 80 B: the
+// right-hand side's coefficients and, per pass, its products with the
+// values that do not change during the pass (RC_O1 / RC_O3 hold the product
+// itself on lines whose first / second updated neighbour is a wall or a
+// wrap line; RC_Q the end term a_minus(1) phi(before) of element 0 or
+// a_plus(n) phi(after) of element n-1), the forward elimination's a(e),
+// m(e) and c'(e), and the division's reciprocal part 1/m(e).
+enum { RC_SU = 0, RC_O1, RC_O3, RC_A, RC_M, RC_Q, RC_P2, RC_P4, RC_C, RC_R, RC_N };
 
 constexpr int TD_LDS_ELEMS = 5120;   // elements staged per group: d, a, m, c', 160 KiB
 constexpr int TD_RING = 8;           // staged values read this many elements ahead of the recurrences
@@ -58,6 +67,11 @@ struct TdPass {
     gdptr pk;                  // PK_N * n * NL
     gdptr ln;                  // LN_N * NL
     gdptr phi;                 // n * NLX
+    gdptr rec;                 // RC_N * n * NL records, or null: the pass runs k_tdma_pass
+    gdptr sink;                // TDC_BLOCK doubles the helpers' idle stores go to
+    int C, K;                  // chain kernel: elements per chunk, chunks per line
+    int ML, by_u;              // chain kernel: x slots (min(nu, nv)), slot = u-1 (by_u) or v-1
+    int NW, pad_;              // chain kernel: chain waves
 };
 
 struct TdSys {
@@ -66,6 +80,7 @@ struct TdSys {
     long ncells;               // nr * nth * nz
     cgdptr c[TD_NCOEF];        // aW aE aS aN aB aT aP Su, padded
     gdptr home;                // the caller's phi (padded)
+    TD_GLOBAL int* off_band;   // per pass: nonzero when a pivot is outside the chain kernel's band
     TdPass ps[3];
 };
 
@@ -110,6 +125,84 @@ __host__ __device__ inline double td_div_fast(double num, double den, double r, 
     (void)ok;
     return num / den;
 #endif
+}
+
+// Biased exponent of x (bits 52..62).
+__host__ __device__ inline unsigned td_bexp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((unsigned)__double2hiint(x) >> 20) & 0x7ffu;
+#else
+    unsigned long long b;
+    std::memcpy(&b, &x, sizeof b);
+    return (unsigned)(b >> 52) & 0x7ffu;
+#endif
+}
+
+// The chain kernel's division (td_div_band) holds for a divisor m with
+// 2^-300 <= |m| < 2^301 (biased exponent 723..1323) and a numerator num
+// that is +0 or has 2^-400 <= |num| < 2^401 (biased exponent 623..1423);
+// td_recip_banded is td_recip for such m and NaN otherwise, and
+// td_num_dev(num) - the caller's check - is at most TD_N_SPAN exactly for
+// such num.
+constexpr unsigned TD_M_LO = 723, TD_M_SPAN = 600;
+constexpr unsigned TD_N_LO = 623, TD_N_SPAN = 800;
+
+__host__ __device__ inline double td_recip_banded(double m) {
+    return td_bexp(m) - TD_M_LO <= TD_M_SPAN ? td_recip(m) : __builtin_nan("");
+}
+
+// 0 for num = +0, else the distance of num's biased exponent above
+// TD_N_LO (wrapping to a large value below it).
+__host__ __device__ inline unsigned td_num_dev(double num) {
+    unsigned long long b;
+#if defined(__HIP_DEVICE_COMPILE__)
+    b = __builtin_bit_cast(unsigned long long, num);
+#else
+    std::memcpy(&b, &num, sizeof b);
+#endif
+    return b == 0 ? 0u : ((unsigned)(b >> 52) & 0x7ffu) - TD_N_LO;
+}
+
+// num / m from r = td_recip_banded(m): the hardware-assisted division
+// without its scaling and fix-up steps (q = num r, the residual, the fused
+// correction).  For m and num in the bands above the division's scale steps
+// change neither operand (their exponents are less than 768 apart, neither
+// is denormal or zero-divisor, the numerator is above 2^-970, the quotient
+// and 1/m are normal) and the fix-up returns its input (a finite, normal or
+// +0 quotient of finite operands; for num = +0 the product q = num r already
+// carries the quotient's sign, and the residual and the correction are +0).
+// So the result is the IEEE quotient whenever td_num_dev(num) <= TD_N_SPAN;
+// the caller divides with `/` when a chunk fails.  A numerator of -0 fails
+// the check: the correction would turn the quotient -0 / m (m > 0) into +0.
+// A pass runs the chain kernel only when every pivot m of the pass is in
+// band (k_tdma_setup flags the others, and k_tdma_pass takes those passes).
+// The host (the replay) divides directly.
+__host__ __device__ inline double td_div_band(double num, double m, double r) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double q = num * r;
+    const double res = fma(-m, q, num);
+    return fma(res, r, q);
+#else
+    (void)r;
+    return num / m;
+#endif
+}
+
+// n / m and n % m for n * m < 2^32 from M = td_magic(m): the quotient is
+// the high word of n * M (M = floor((2^32 - 1) / m) + 1; m = 1 takes M = 0
+// and the quotient n; so does m = 0, the empty diagonal past a pass's last,
+// which divides nothing).  tests/tdma_check.cpp checks it against `/` for every
+// m up to 1024 and n up to 2^20 / m.
+__host__ __device__ inline unsigned td_magic(unsigned m) { return m <= 1 ? 0u : 0xffffffffu / m + 1u; }
+
+__host__ __device__ inline void td_divmod(int n, int m, unsigned M, int& q, int& r) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const unsigned hi = __umulhi((unsigned)n, M);
+#else
+    const unsigned hi = (unsigned)(((unsigned long long)(unsigned)n * M) >> 32);
+#endif
+    q = M == 0u ? n : (int)hi;
+    r = n - q * m;
 }
 
 __host__ __device__ inline void diagonal_range(int w, int nu, int nv, int& v_lo, int& v_hi) {
@@ -288,7 +381,18 @@ __host__ __device__ inline void setup_line(const TdSys& s, int P, int u, int v) 
         cp = -s.c[ap][c] / m;
         x[PK_A * plane] = a;
         x[PK_M * plane] = m;
-        x[PK_C * plane] = cp;
+        x[PK_C * plane] = e == p.n - 1 ? 0.0 : cp;    // c'(n-1) takes no part in the recurrences
+        if (p.rec) {
+            gdptr r = p.rec + ((long)e * NL + l) * RC_N;
+            r[RC_SU] = s.c[CSU][c];
+            r[RC_O1] = s.c[o[0]][c];
+            r[RC_O3] = s.c[o[2]][c];
+            r[RC_A] = e == 0 ? 0.0 : a;                    // a(0) takes no part in the recurrences
+            r[RC_M] = m;
+            r[RC_C] = e == p.n - 1 ? 0.0 : cp;
+            r[RC_R] = td_recip_banded(m);
+            if (td_bexp(m) - TD_M_LO > TD_M_SPAN) s.off_band[P] = 1;
+        }
         if (P == 0) p.phi[(long)e * p.NLX + l] = s.home[c];
     }
     int i0, j0, k0, i1, j1, k1;
@@ -484,6 +588,435 @@ __host__ __device__ inline void wall_halo_column(const TdSys& s, long t) {
     const int src[4] = {s.nth, s.nth - 1, 1, 2};
 #pragma unroll
     for (int q = 0; q < 4; ++q) s.home[tat(s, i, halo[q], k)] = s.home[tat(s, i, src[q], k)];
+}
+
+// ---------------------------------------------------------------------------
+// The chain kernel (k_tdma_chain): one work-group per system and pass.
+//
+// The lines of a diagonal are independent; diagonal w + 1 needs the x of
+// diagonal w, and its first forward step needs x(0), the last value of w's
+// back substitution, so a pass is one chain of dependent steps: for every
+// diagonal, n forward and n backward steps.  The kernel keeps that chain
+// short and feeds it from LDS:
+//   - chain waves (the first NW waves, one lane per line of the diagonal)
+//     run only the recurrences: forward d'(e) = (d(e) - a(e) d'(e-1)) / m(e)
+//     with the division's reciprocal precomputed (td_div_band), then back
+//     substitution x(e) = d'(e) - c'(e) x(e+1), storing each x to the pass's
+//     phi row as it is formed (stores nobody waits for);
+//   - helper waves (the rest) stream the static per-element data from HBM
+//     into LDS ahead of the chain and form d(e) from the previous diagonal's
+//     x, kept in LDS.  They issue loads and nothing else to global memory,
+//     the same loads for every task, so their waits are exact.
+// x lives in LDS slots: the line (u,v) uses slot u-1 (by_u) or v-1, which
+// it shares with one of its two updated neighbours, (u-1,v) or (u,v-1); the
+// other sits in the slot before.  Every LDS array keeps a line's (or a
+// slot's) elements contiguous, rows of odd length so that the lanes of a
+// wave, one row each, fall in different banks; the chain then reads and
+// writes its elements at fixed offsets from one address per row.  Each slot holds, element by element, the
+// previous diagonal's x until the helpers have read it, then d'(e), then
+// the new x.  The forward sweep takes a line's elements in K chunks of C;
+// each diagonal is K+2 phases separated by work-group barriers:
+//   G             the helpers form d of chunk 0 (it needs x(0) of the
+//                 diagonal before, just finished); the chain waits
+//   F_0..F_{K-1}  the chain's forward chunks, while the helpers fill the
+//                 other ring slot with the next chunk
+//   B             the chain's whole back substitution, with c'(e) of the
+//                 diagonal's lines, which the helpers copied to LDS with
+//                 each chunk
+// A helper prepares forward chunk f during the phase before it, from
+// registers loaded when it prepared chunk f-2 (three register sets in
+// rotation), so loads have two chunks to arrive.  The d of chunk k is
+// formed after the previous diagonal's B and before F_k overwrites the
+// slots with d'.
+constexpr int TDC_BLOCK = 512;      // threads of a chain work-group
+constexpr int TDC_IPL = 3;          // ring entries a helper lane fills per chunk
+constexpr int TDC_LDS = 20480;      // doubles of LDS (160 KiB)
+constexpr int TDC_G = 4;            // ring entries the chain reads per group
+static_assert((TDC_BLOCK & (TDC_BLOCK - 1)) == 0, "the sink index masks with TDC_BLOCK - 1");
+
+// Plans the chain kernel for pass p: chunk length C (at most cmax), chunks
+// K, slots ML, slot direction, chain waves NW.  False when the pass cannot
+// run it (lines of one element, diagonals over 256 lines, or LDS too small):
+// then it runs k_tdma_pass.
+__host__ __device__ inline int odd_up(int x) { return x | 1; }
+
+constexpr int TDC_PAD = 4 * TDC_G;  // doubles before and after the arrays: reads ahead stay in the block
+
+__host__ __device__ inline bool chain_plan(TdPass& p, int cmax) {
+    if (p.n < 2) return false;
+    const int ML = p.nu < p.nv ? p.nu : p.nv;
+    const int NW = (ML + 63) / 64;
+    if (NW > 4) return false;
+    const long HL = TDC_BLOCK - 64 * NW;
+    // ring (two slots of ML rows of odd_up(C) entries, d/r and a/m), c' and
+    // x slots (ML rows of odd_up(n)) and the padding fit the LDS
+    long C = ((long)TDC_LDS - 2 * TDC_PAD - 2L * odd_up(p.n) * ML) / (8L * ML) - 1;
+    if ((long)TDC_IPL * HL / ML < C) C = (long)TDC_IPL * HL / ML;
+    if (C > p.n) C = p.n;
+    if (C > cmax) C = cmax;
+    if (C < 1) return false;
+    // whole groups of the recurrences' reads where that costs no extra chunk
+    const long Cg = C >= TDC_G ? C - C % TDC_G : C;
+    if ((p.n + Cg - 1) / Cg == (p.n + C - 1) / C) C = Cg;
+    const long K = (p.n + C - 1) / C;
+    long Cb = (p.n + K - 1) / K;                               // the same K with the shortest chunks
+    if (C % TDC_G == 0 && C >= TDC_G) Cb += (TDC_G - Cb % TDC_G) % TDC_G;
+    p.K = (int)K;
+    p.C = (int)(Cb < C ? Cb : C);
+    p.ML = ML;
+    p.by_u = p.nu <= p.nv;
+    p.NW = NW;
+    return true;
+}
+
+__host__ __device__ inline long chain_lds_doubles(const TdPass& p) {
+    return 2L * TDC_PAD + 8L * odd_up(p.C) * p.ML + 2L * odd_up(p.n) * p.ML;
+}
+
+// The two ring slots b = 0, 1 of each array follow each other: slot b of dr
+// starts at dr + b * ring (no pointer arrays, which a run-time index would
+// put in scratch memory).
+struct TdLds {
+    double2* dr;        // ring: {d(e), 1/m(e)} of chunk element t of line j at j*CS + t
+    double2* am;        //       {a(e), m(e)}
+    double* cp;         // c'(e) of line j at j*LS + e
+    double* xb;         // x slots: element e of slot s at s*LS + e
+    long ring;          // entries per slot: ML * CS
+    int CS, LS;         // odd row lengths: odd_up(C), odd_up(n)
+};
+
+__host__ __device__ inline TdLds chain_lds(const TdPass& p, double* base) {
+    TdLds L;
+    L.CS = odd_up(p.C);
+    L.LS = odd_up(p.n);
+    L.ring = (long)p.ML * L.CS;
+    L.dr = reinterpret_cast<double2*>(base + TDC_PAD);
+    L.am = L.dr + 2 * L.ring;
+    L.cp = base + TDC_PAD + 8 * L.ring;
+    L.xb = L.cp + (long)p.ML * L.LS;
+    return L;
+}
+
+// The phase (w, idx) of a pass: idx 0 is G, 1..K are F_0..F_{K-1}, K+1 is
+// B.  The iterator carries the diagonal's first v, line count and first
+// line, advanced without division or table lookups.
+struct TdTask {
+    int w, idx, kind;   // kind: 0 none (G, or past the pass), 1 forward chunk, 2 back substitution
+    int k, e0, ce;      // forward chunk, its first element, its length
+    int v_lo, m, l0;    // the diagonal's first v, lines, first line index
+    unsigned mM;        // td_magic(m)
+};
+
+__host__ __device__ inline void chain_set_kind(const TdPass& p, TdTask& t) {
+    const bool none = t.w >= p.nu + p.nv - 1 || t.idx == 0, fwd = t.idx <= p.K;
+    const int k = none || !fwd ? 0 : t.idx - 1;
+    const int e0 = k * p.C, rest = p.n - e0;
+    t.kind = none ? 0 : fwd ? 1 : 2;
+    t.k = k;
+    t.e0 = e0;
+    t.ce = none || !fwd ? 0 : rest < p.C ? rest : p.C;
+}
+
+__host__ __device__ inline TdTask chain_first(const TdPass& p) {
+    TdTask t;
+    t.w = 0;
+    t.idx = 0;
+    t.v_lo = 1;
+    t.m = 1;
+    t.mM = td_magic(1);
+    t.l0 = 0;
+    chain_set_kind(p, t);
+    return t;
+}
+
+__host__ __device__ inline void chain_next_diagonal(const TdPass& p, TdTask& t) {
+    t.l0 += t.m;
+    ++t.w;
+    int hi;
+    diagonal_range(t.w, p.nu, p.nv, t.v_lo, hi);
+    t.m = hi - t.v_lo + 1;
+    t.mM = td_magic((unsigned)t.m);
+}
+
+__host__ __device__ inline void chain_advance(const TdPass& p, TdTask& t) {
+    if (++t.idx == p.K + 2) {
+        t.idx = 0;
+        chain_next_diagonal(p, t);
+    }
+    chain_set_kind(p, t);
+}
+
+// The forward chunks alone, in order (the helpers' tasks); past the last
+// one the iterator stays on it, so its loads repeat valid addresses.
+__host__ __device__ inline TdTask chain_first_fwd(const TdPass& p) {
+    TdTask t = chain_first(p);
+    t.idx = 1;
+    chain_set_kind(p, t);
+    return t;
+}
+
+__host__ __device__ inline void chain_advance_fwd(const TdPass& p, TdTask& t) {
+    if (t.idx == p.K && t.w == p.nu + p.nv - 2) return;
+    if (++t.idx == p.K + 1) {
+        t.idx = 1;
+        chain_next_diagonal(p, t);
+    }
+    chain_set_kind(p, t);
+}
+
+// The phase during which the helpers prepare forward chunk t: the one
+// before it.
+__host__ __device__ inline long chain_prep_phase(const TdPass& p, const TdTask& t) {
+    return (long)t.w * (p.K + 2) + t.idx - 1;
+}
+
+__host__ __device__ inline long chain_phases(const TdPass& p) { return (long)(p.nu + p.nv - 1) * (p.K + 2); }
+
+__host__ __device__ inline int chain_slot(const TdPass& p, int u, int v) { return p.by_u ? u - 1 : v - 1; }
+
+// What a helper lane loads for one ring entry: the record.
+struct TdPre {
+    double r[RC_N];
+};
+
+// Ring entry `it` of forward chunk t: (element e0 + it / m, line it % m).
+// The loads are unconditional (entries past the chunk repeat its last one
+// and go unused), so every chunk issues the same loads.
+template <int P>
+__host__ __device__ inline void chain_load(const TdPass& p, const TdTask& t, int it, TdPre& x) {
+    const int cnt = t.ce * t.m, itc = it < cnt ? it : cnt - 1;
+    int tt, j;
+    td_divmod(itc, t.m, t.mM, tt, j);
+    const long x0 = (long)(t.e0 + tt) * p.NL + t.l0 + j;
+    typedef const TD_GLOBAL double2* cgd2ptr;
+    const cgd2ptr rp = reinterpret_cast<cgd2ptr>(p.rec + x0 * RC_N);
+#pragma unroll
+    for (int q = 0; q < RC_N / 2; ++q) {
+        const double2 v2 = rp[q];
+        x.r[2 * q] = v2.x;
+        x.r[2 * q + 1] = v2.y;
+    }
+}
+
+// d(e) of ring entry `it` of forward chunk t, in the reference's operand
+// order (rhs_elem): Su + o1 phi1 + o2 phi2 + o3 phi3 + o4 phi4, then the end
+// term.  phi1 and phi3 are the updated neighbours, read from the x slots
+// unless they are walls or wrap lines (then RC_O1 / RC_O3 already hold the
+// product); phi2 and phi4 have not changed since the pass began (RC_P2,
+// RC_P4).  With it, a(e), m(e) and 1/m(e) into ring slot b, and c'(e) into
+// the diagonal's c' array.  Branch-free: entries past the chunk redo its
+// last one (the same values to the same places), both slots are read and
+// the products chosen.
+template <int P>
+__host__ __device__ inline void chain_prepare(const TdPass& p, const TdLds& L, const TdTask& t, int b, int it,
+                                              const TdPre& x) {
+    const int cnt = t.ce * t.m, itc = it < cnt ? it : cnt - 1;
+    int tt, j;
+    td_divmod(itc, t.m, t.mM, tt, j);
+    const int e = t.e0 + tt;
+    const int v = t.v_lo + j, u = t.w + 2 - v;
+    const bool a_edge = P == 2 ? v == 1 : u == 1;     // phi1: (u-1,v), or (u,v-1) for the z lines
+    const bool b_edge = P == 2 ? u == 1 : v == 1;     // phi3: (u,v-1), or (u-1,v) for the z lines
+    const int sa = P == 2 ? chain_slot(p, u, v - 1) : chain_slot(p, u - 1, v);
+    const int sb = P == 2 ? chain_slot(p, u - 1, v) : chain_slot(p, u, v - 1);
+    const double* col = L.xb + e;
+    const long LS = L.LS;
+    const double na = col[LS * (sa > 0 ? sa : 0)], nb = col[LS * (sb > 0 ? sb : 0)];
+    const double p1 = a_edge ? x.r[RC_O1] : x.r[RC_O1] * na;
+    const double p3 = b_edge ? x.r[RC_O3] : x.r[RC_O3] * nb;
+    double d = x.r[RC_SU] + p1;
+    d = d + x.r[RC_P2];
+    d = d + p3;
+    d = d + x.r[RC_P4];
+    const double dq = d + x.r[RC_Q];
+    d = e == 0 || e == p.n - 1 ? dq : d;
+    const long at = b * L.ring + (long)j * L.CS + tt;
+    L.dr[at] = make_double2(d, x.r[RC_R]);
+    L.am[at] = make_double2(x.r[RC_A], x.r[RC_M]);
+    L.cp[(long)j * L.LS + e] = x.r[RC_C];
+}
+
+// Entry `it` of chunk (e0, ce) of diagonal w's x (m lines from v_lo, first
+// line l0), from its x slot to the pass's phi row.  Entries past the chunk
+// store its last entry again (the same value to the same address), and with
+// `real` false the store goes to the sink, so every call issues one store.
+__host__ __device__ inline void chain_copy_out(const TdPass& p, const TdLds& L, bool real, int w, int v_lo, int m,
+                                               unsigned mM, int l0, int e0, int ce, int it) {
+    const int cnt = ce * m, itc = it < cnt ? it : cnt - 1;
+    int tt, j;
+    td_divmod(itc, m, mM, tt, j);
+    const int e = e0 + tt;
+    const int v = v_lo + j, u = w + 2 - v, sl = chain_slot(p, u, v);
+    const double x = L.xb[(long)(sl > 0 ? sl : 0) * L.LS + e];
+    gdptr dst = real ? p.phi + (long)e * p.NLX + l0 + j : p.sink + (it & (TDC_BLOCK - 1));
+    *dst = x;
+}
+
+// The chain lane of line j of forward chunk t: elements e0..e0+ce-1, d'
+// into the line's x slot, dp carried across chunks.  Element 0's ring entry
+// holds a = +0 and dp starts at +0, so every element takes the same step:
+// d - (+0)(+0) is d itself for every d, signed zeros included.  The ring is
+// read TDC_G elements at a time, two groups ahead of the recurrence, so LDS
+// latency stays off it; only the chunk's last group tests its bounds.  A
+// chunk whose numerators leave the band of td_div_band is redone with `/`.
+__host__ __device__ inline void chain_forward(const TdPass& p, const TdLds& L, const TdTask& t, int b, int j,
+                                              double& dp) {
+    if (j >= t.m) return;
+    const int v = t.v_lo + j, u = t.w + 2 - v, s = chain_slot(p, u, v);
+    const double2* dr = L.dr + b * L.ring + (long)j * L.CS;
+    const double2* am = L.am + b * L.ring + (long)j * L.CS;
+    double* xs = L.xb + (long)s * L.LS + t.e0;
+    const int ce = t.ce, ng = (ce + TDC_G - 1) / TDC_G;
+    if (t.e0 == 0) dp = 0.0;
+    const double dp_in = dp;
+    unsigned worst = 0;
+    double2 d0[TDC_G], a0[TDC_G], d1[TDC_G], a1[TDC_G], d2[TDC_G], a2[TDC_G];
+    // group g holds elements g*TDC_G.. (reads past the chunk land in the
+    // row's tail, the next row or the padding, and go unused)
+    auto load = [&](int g, double2 (&D)[TDC_G], double2 (&A)[TDC_G]) {
+#pragma unroll
+        for (int r = 0; r < TDC_G; ++r) {
+            D[r] = dr[g * TDC_G + r];
+            A[r] = am[g * TDC_G + r];
+        }
+    };
+    auto step = [&](int tt, const double2& D, const double2& A) {
+        const double num = D.x - A.x * dp;
+        dp = td_div_band(num, A.y, D.y);
+        const unsigned dev = td_num_dev(num);
+        worst = dev > worst ? dev : worst;
+        xs[tt] = dp;
+    };
+    auto full = [&](int g, const double2 (&D)[TDC_G], const double2 (&A)[TDC_G]) {
+#pragma unroll
+        for (int r = 0; r < TDC_G; ++r) step(g * TDC_G + r, D[r], A[r]);
+    };
+    auto part = [&](int g, const double2 (&D)[TDC_G], const double2 (&A)[TDC_G]) {
+#pragma unroll
+        for (int r = 0; r < TDC_G; ++r)
+            if (g * TDC_G + r < ce) step(g * TDC_G + r, D[r], A[r]);
+    };
+    load(0, d0, a0);
+    load(1, d1, a1);
+    int g = 0;
+    for (; g + 3 < ng; g += 3) {
+        load(g + 2, d2, a2);
+        full(g, d0, a0);
+        load(g + 3, d0, a0);
+        full(g + 1, d1, a1);
+        load(g + 4, d1, a1);
+        full(g + 2, d2, a2);
+    }
+    if (ng - g == 1) {
+        part(g, d0, a0);
+    } else if (ng - g == 2) {
+        full(g, d0, a0);
+        part(g + 1, d1, a1);
+    } else {
+        load(g + 2, d2, a2);
+        full(g, d0, a0);
+        full(g + 1, d1, a1);
+        part(g + 2, d2, a2);
+    }
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (worst > TD_N_SPAN) {
+        dp = dp_in;
+        for (int tt = 0; tt < ce; ++tt) {
+            const double2 dv = dr[tt], av = am[tt];
+            dp = (dv.x - av.x * dp) / av.y;
+            xs[tt] = dp;
+        }
+    }
+#else
+    (void)dp_in;
+    (void)worst;
+#endif
+}
+
+// Back substitution of line j of diagonal t from e = n-1 down to 0: x into
+// the line's x slot.  c'(n-1) is +0 and x starts at +0, so the first
+// element's x = d' - (+0)(+0) is d' itself and every element takes the same
+// step; d'(e) and c'(e) are read TDC_G at a time, two groups ahead (reads
+// below element 0 land in the row before or the padding, and go unused).
+__host__ __device__ inline void chain_backward(const TdPass& p, const TdLds& L, const TdTask& t, int j) {
+    if (j >= t.m) return;
+    const int v = t.v_lo + j, u = t.w + 2 - v, s = chain_slot(p, u, v);
+    const int n = p.n, ng = (n + TDC_G - 1) / TDC_G;
+    double* xs = L.xb + (long)s * L.LS + n - 1;          // element n-1-i at xs[-i]
+    const double* cp = L.cp + (long)j * L.LS + n - 1;
+    double x = 0.0;
+    double p0[TDC_G], c0[TDC_G], p1[TDC_G], c1[TDC_G], p2[TDC_G], c2[TDC_G];
+    auto load = [&](int g, double (&X)[TDC_G], double (&Cc)[TDC_G]) {
+#pragma unroll
+        for (int r = 0; r < TDC_G; ++r) {
+            X[r] = xs[-(g * TDC_G + r)];
+            Cc[r] = cp[-(g * TDC_G + r)];
+        }
+    };
+    auto step = [&](int i, double X, double Cc) {
+        x = X - Cc * x;
+        xs[-i] = x;
+    };
+    auto full = [&](int g, const double (&X)[TDC_G], const double (&Cc)[TDC_G]) {
+#pragma unroll
+        for (int r = 0; r < TDC_G; ++r) step(g * TDC_G + r, X[r], Cc[r]);
+    };
+    auto part = [&](int g, const double (&X)[TDC_G], const double (&Cc)[TDC_G]) {
+#pragma unroll
+        for (int r = 0; r < TDC_G; ++r)
+            if (g * TDC_G + r < n) step(g * TDC_G + r, X[r], Cc[r]);
+    };
+    load(0, p0, c0);
+    load(1, p1, c1);
+    int g = 0;
+    for (; g + 3 < ng; g += 3) {
+        load(g + 2, p2, c2);
+        full(g, p0, c0);
+        load(g + 3, p0, c0);
+        full(g + 1, p1, c1);
+        load(g + 4, p1, c1);
+        full(g + 2, p2, c2);
+    }
+    if (ng - g == 1) {
+        part(g, p0, c0);
+    } else if (ng - g == 2) {
+        full(g, p0, c0);
+        part(g + 1, p1, c1);
+    } else {
+        load(g + 2, p2, c2);
+        full(g, p0, c0);
+        full(g + 1, p1, c1);
+        part(g + 2, p2, c2);
+    }
+}
+
+// The per-pass products of k_tdma_prep for element e of line l: RC_P2 and
+// RC_P4 (the neighbours that keep their pass-start values: (u+1,v), (u,v+1),
+// walls, and the wrap's partner (1,v) / (u,1) read before the pass changes
+// it), RC_O1 / RC_O3 on lines whose updated neighbour is a wall or wrap line,
+// and for e = 0 and e = n-1 the line's end-term product RC_Q.
+template <int P>
+__host__ __device__ inline void prep_elem(const TdPass& p, int l, int e) {
+    int u, v;
+    line_at(p, l, u, v);
+    int nb[4];
+    line_neighbours(p, P, u, v, nb);
+    if (P == 0 && u == p.nu) nb[1] = line_index(p, 1, v);
+    if (P == 2 && v == p.nv) nb[1] = line_index(p, u, 1);
+    const long NL = p.NL, plane = (long)p.n * NL, x = (long)e * NL + l;
+    cgdptr row = p.phi + (long)e * p.NLX;
+    gdptr r = p.rec + x * RC_N;
+    r[RC_P2] = p.pk[PK_O2 * plane + x] * row[nb[1]];
+    r[RC_P4] = p.pk[PK_O4 * plane + x] * row[nb[3]];
+    if (P == 2 ? v == 1 : u == 1) r[RC_O1] = p.pk[PK_O1 * plane + x] * row[nb[0]];
+    if (P == 2 ? u == 1 : v == 1) r[RC_O3] = p.pk[PK_O3 * plane + x] * row[nb[2]];
+    if (e == 0) {
+        const double lo = P == 1 ? p.phi[(long)(p.n - 1) * p.NLX + l] : p.ln[LN_LO * NL + l];
+        r[RC_Q] = p.ln[LN_AMF * NL + l] * lo;
+    } else if (e == p.n - 1) {
+        const double hi = P == 1 ? p.phi[l] : p.ln[LN_HI * NL + l];
+        r[RC_Q] = p.ln[LN_APL * NL + l] * hi;
+    }
 }
 
 }  // namespace tdma

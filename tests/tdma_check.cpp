@@ -34,7 +34,7 @@ extern "C" int oracle_tdma_3d(int nr, int nth, int nz, const double* aW, const d
 
 using namespace eafp::tdma;
 
-static long g_cases = 0;
+static long g_cases = 0, g_chain_passes = 0, g_off_band = 0;
 
 #define CHECK(cond, ...)                                                   \
     do {                                                                   \
@@ -121,7 +121,125 @@ static void replay_pass(const TdSys& s, int order) {
     std::free(sd); std::free(sa); std::free(sm); std::free(sc);
 }
 
-static void run_case(int nr, int nth, int nz, int n_sweep, unsigned seed, bool zero_pivots, int order) {
+// k_tdma_prep then k_tdma_chain for pass P, phase by phase: the helpers'
+// work of a phase (fill the ring slot and the c' entries of the next forward
+// chunk) and the chain lanes' work (a forward chunk or the whole back
+// substitution) run in either order (order 0: helpers first), which they
+// may since they touch different ring slots and different elements of the
+// x slots.  Every ring slot is filled with NaN before the helpers fill it,
+// every loaded record starts as NaN, and the x slots and c' entries start as
+// NaN, so a read of a value not written for it reaches phi.  Checks that
+// the helpers fill exactly the ring entries of their chunk, that they
+// prepare every forward chunk once, in the phase before it, and that the
+// chain solves every line once.
+template <int P>
+static void replay_chain(const TdSys& s, int order) {
+    const TdPass& p = s.ps[P];
+    for (long t = (long)p.n * p.NL - 1; t >= 0; --t) prep_elem<P>(p, (int)(t % p.NL), (int)(t / p.NL));
+    const int NC = 64 * p.NW, HL = TDC_BLOCK - NC;
+    CHECK(p.ML <= NC && p.NW <= 4, "pass %d: %d slots for %d chain lanes", P, p.ML, NC);
+    CHECK(chain_lds_doubles(p) <= TDC_LDS, "pass %d: LDS %ld doubles", P, chain_lds_doubles(p));
+    CHECK((long)p.C * p.ML <= (long)TDC_IPL * HL, "pass %d: chunks of %d x %d for %d helper lanes", P, p.C, p.ML, HL);
+    CHECK((long)(p.K - 1) * p.C < p.n && (long)p.K * p.C >= p.n, "pass %d: chunks %d x %d for %d", P, p.K, p.C, p.n);
+    const double qnan = std::nan("");
+    // the kernel's LDS layout (chain_lds) in one exact-size block, all NaN
+    const long nlds = chain_lds_doubles(p);
+    double* lds = alloc<double>(nlds);
+    for (long o = 0; o < nlds; ++o) lds[o] = qnan;
+    const TdLds L = chain_lds(p, lds);
+    const long ring = L.ring;
+    CHECK(L.CS % 2 == 1 && L.LS % 2 == 1 && L.CS >= p.C && L.LS >= p.n && ring == (long)p.ML * L.CS &&
+              L.xb + (long)p.ML * L.LS + TDC_PAD == lds + nlds && (double*)L.dr == lds + TDC_PAD,
+          "pass %d: LDS layout", P);
+    double2* dr[2] = {L.dr, L.dr + ring};
+    double2* am[2] = {L.am, L.am + ring};
+    std::vector<double> carry((size_t)NC, 0.0);
+    const long nph = chain_phases(p), nf = (long)(p.nu + p.nv - 1) * p.K;
+    TdTask tc = chain_first(p), f = chain_first_fwd(p);
+    long prepared = 0, fwd_lines = 0, bwd_lines = 0;
+    for (long X = 0; X < nph; ++X) {
+        auto helpers = [&]() {
+            if (prepared >= nf || chain_prep_phase(p, f) != X) return;
+            CHECK(f.kind == 1, "pass %d phase %ld: helpers' task is not a forward chunk", P, X);
+            const int bn = (int)((X + 1) & 1);
+            for (long o = 0; o < ring; ++o) {
+                dr[bn][o] = make_double2(qnan, qnan);
+                am[bn][o] = make_double2(qnan, qnan);
+            }
+            for (int h = HL - 1; h >= 0; --h)
+                for (int i = 0; i < TDC_IPL; ++i) {
+                    TdPre x;
+                    for (int q = 0; q < RC_N; ++q) x.r[q] = qnan;
+                    chain_load<P>(p, f, h + i * HL, x);
+                    chain_prepare<P>(p, L, f, bn, h + i * HL, x);
+                }
+            // the copy of the diagonal before's chunk (the sink on the first diagonal)
+            {
+                const bool real = f.w > 0;
+                int v_lo = 1, hi = 1;
+                if (real) diagonal_range(f.w - 1, p.nu, p.nv, v_lo, hi);
+                const int mp = hi - v_lo + 1, l0p = f.l0 - mp;
+                CHECK(!real || (long)f.ce * mp <= (long)TDC_IPL * HL, "pass %d: copy of %d x %d", P, f.ce, mp);
+                for (int h = 0; h < HL; ++h)
+                    for (int i = 0; i < TDC_IPL; ++i)
+                        chain_copy_out(p, L, real, f.w - 1, v_lo, mp, td_magic((unsigned)mp), l0p, f.e0, f.ce,
+                                       h + i * HL);
+            }
+            for (long tt = 0; tt < L.CS; ++tt)
+                for (long j = 0; j < p.ML; ++j) {
+                    const long o = j * L.CS + tt;
+                    const bool in = tt < f.ce && j < f.m;
+                    const bool set = !std::isnan(dr[bn][o].x) && !std::isnan(am[bn][o].y);
+                    CHECK(in == set, "pass %d phase %ld: ring entry (%ld,%ld) %s", P, X, tt, j,
+                          in ? "not filled" : "filled outside the chunk");
+                }
+            ++prepared;
+            chain_advance_fwd(p, f);
+        };
+        auto chain = [&]() {
+            const int b = (int)(X & 1);
+            for (int r = 0; r < NC; ++r) {
+                const int j = order == 0 ? NC - 1 - r : (int)((r + X) % NC);
+                if (tc.kind == 1) chain_forward(p, L, tc, b, j, carry[(size_t)j]);
+                else if (tc.kind == 2) chain_backward(p, L, tc, j);
+            }
+            if (tc.kind == 1 && tc.k == 0) fwd_lines += tc.m;
+            if (tc.kind == 2) bwd_lines += tc.m;
+        };
+        if (order == 0) {
+            helpers();
+            chain();
+        } else {
+            chain();
+            helpers();
+        }
+        chain_advance(p, tc);
+    }
+    CHECK(prepared == nf, "pass %d: %ld of %ld forward chunks prepared", P, prepared, nf);
+    {   // the last diagonal's x, after the last barrier
+        TdTask t = chain_first(p);
+        for (int w = 0; w < p.nu + p.nv - 2; ++w) chain_next_diagonal(p, t);
+        for (int it = p.n * t.m - 1; it >= 0; --it)
+            chain_copy_out(p, L, true, t.w, t.v_lo, t.m, t.mM, t.l0, 0, p.n, it);
+    }
+    CHECK(fwd_lines == p.NL && bwd_lines == p.NL, "pass %d: %ld / %ld of %d lines solved", P, fwd_lines, bwd_lines, p.NL);
+    std::free(lds);
+}
+
+template <int P>
+static void run_pass(const TdSys& s, int order) {
+    if (s.ps[P].rec && !s.off_band[P]) {
+        replay_chain<P>(s, order);
+    } else {
+        if (s.ps[P].rec) ++g_off_band;
+        replay_pass<P>(s, order);
+    }
+}
+
+// cmax: 0 replays k_tdma_pass for every pass; otherwise the chain kernel
+// wherever chain_plan admits the pass, with chunks of at most cmax elements.
+static void run_case(int nr, int nth, int nz, int n_sweep, unsigned seed, bool zero_pivots, int order, int cmax,
+                     bool huge = false) {
     const long sy = nr + 4, sz = (long)(nr + 4) * (nth + 4), n = sz * (nz + 4);
     const long ncells = (long)nr * nth * nz;
     std::mt19937_64 rng(seed);
@@ -142,6 +260,8 @@ static void run_case(int nr, int nth, int nz, int n_sweep, unsigned seed, bool z
             c[CP_][o] = (o % 2) ? 0.0 : -0.0;
             c[CW][o] = c[CE][o] = c[CS][o] = c[CN][o] = c[CB][o] = c[CT][o] = 0.0;
         }
+    if (huge)   // pivots near 1e308: outside the chain kernel's band, so k_tdma_pass takes those passes
+        for (long o = 0; o < n; o += 37) c[CP_][o] = 4.0e307 * (1.0 + U(rng));
     double* phi0 = alloc<double>(n);
     for (long o = 0; o < n; ++o) phi0[o] = N(rng);
     double* want = alloc<double>(n);
@@ -153,6 +273,8 @@ static void run_case(int nr, int nth, int nz, int n_sweep, unsigned seed, bool z
     s.sy = sy; s.sz = sz; s.n = n; s.ncells = ncells;
     for (int q = 0; q < TD_NCOEF; ++q) s.c[q] = (cgdptr)c[q];
     s.home = (gdptr)alloc<double>(n);
+    int off_band[3] = {0, 0, 0};
+    s.off_band = (decltype(s.off_band))off_band;
     std::memcpy((void*)s.home, phi0, sizeof(double) * n);
     std::vector<void*> owned;
     long lines = 0;
@@ -178,6 +300,17 @@ static void run_case(int nr, int nth, int nz, int n_sweep, unsigned seed, bool z
         CHECK(acc == p.NL, "diagonals cover the lines");
         p.pk = (gdptr)pk; p.ln = (gdptr)ln; p.phi = (gdptr)ph; p.dstart = (cgiptr)ds;
         owned.insert(owned.end(), {pk, ln, ph, ds});
+        p.rec = p.sink = nullptr;
+        if (cmax > 0 && chain_plan(p, cmax)) {
+            double* rec = alloc<double>(RC_N * ncells);
+            for (long o = 0; o < RC_N * ncells; ++o) rec[o] = std::nan("");
+            double* sink = alloc<double>(TDC_BLOCK);
+            p.rec = (gdptr)rec;
+            p.sink = (gdptr)sink;
+            owned.push_back(rec);
+            owned.push_back(sink);
+            ++g_chain_passes;
+        }
         lines += p.NL;
     }
     // k_tdma_setup: thread t = line index within the passes, past the end included
@@ -199,11 +332,11 @@ static void run_case(int nr, int nth, int nz, int n_sweep, unsigned seed, bool z
     }
     for (long t = 0; t < lines; ++t) CHECK(setup_seen[(size_t)t] == 1, "setup line %ld seen %d times", t, setup_seen[(size_t)t]);
     for (int sweep = 0; sweep < n_sweep; ++sweep) {
-        replay_pass<0>(s, order);
+        run_pass<0>(s, order);
         for (long cc = ncells - 1; cc >= 0; --cc) move_cell(s, 0, 1, cc);
-        replay_pass<1>(s, order);
+        run_pass<1>(s, order);
         for (long cc = 0; cc < ncells; ++cc) move_cell(s, 1, 2, cc);
-        replay_pass<2>(s, order);
+        run_pass<2>(s, order);
         for (long cc = ncells - 1; cc >= 0; --cc) move_cell(s, 2, sweep + 1 < n_sweep ? 0 : 3, cc);
     }
     if (n_sweep > 0)
@@ -211,8 +344,8 @@ static void run_case(int nr, int nth, int nz, int n_sweep, unsigned seed, bool z
     long bad = 0;
     for (long o = 0; o < n; ++o)
         if (std::memcmp((const void*)&s.home[o], &want[o], sizeof(double)) != 0) ++bad;
-    CHECK(bad == 0, "%dx%dx%d sweeps %d seed %u: %ld of %ld padded values differ from the oracle", nr, nth, nz,
-          n_sweep, seed, bad, n);
+    CHECK(bad == 0, "%dx%dx%d sweeps %d seed %u cmax %d: %ld of %ld padded values differ from the oracle", nr, nth, nz,
+          n_sweep, seed, cmax, bad, n);
     for (void* p : owned) std::free(p);
     for (int q = 0; q < TD_NCOEF; ++q) std::free(c[q]);
     std::free(phi0);
@@ -221,21 +354,45 @@ static void run_case(int nr, int nth, int nz, int n_sweep, unsigned seed, bool z
     ++g_cases;
 }
 
+// td_divmod against `/` and `%`: every divisor up to 1024, every dividend
+// below 2^20 / m (the kernels' dividends stay below 2^18, divisors below
+// 2^10).
+static void check_divmod() {
+    for (int m = 1; m <= 1024; ++m) {
+        const unsigned M = td_magic((unsigned)m);
+        for (int n = 0; n < (1 << 20) / m; ++n) {
+            int q, r;
+            td_divmod(n, m, M, q, r);
+            CHECK(q == n / m && r == n % m, "td_divmod(%d, %d): %d %d", n, m, q, r);
+        }
+    }
+}
+
 int main() {
+    check_divmod();
     // the last shapes give diagonals longer than the staging arrays (groups) and lines of one element
     const int shapes[][3] = {{5, 7, 6},   {1, 5, 4},    {6, 1, 3},   {4, 6, 1},    {3, 2, 5},
                              {1, 1, 1},   {2, 1, 1},    {1, 1, 7},   {12, 24, 16}, {9, 2, 7},
                              {7, 12, 9},  {20, 36, 28}, {60, 120, 84}, {300, 6, 280}, {2, 3000, 2},
                              {1, 90, 90}};
     unsigned seed = 1;
+    // each shape with k_tdma_pass (cmax 0) and the chain kernel, with chunks
+    // of at most 3 elements (several chunks, a short last one) and as planned
+    const int cmaxes[] = {0, 3, 1 << 20};
     for (const auto& sh : shapes) {
         const bool big = (long)sh[0] * sh[1] * sh[2] > 100000;
-        for (int n_sweep = 1; n_sweep <= (big ? 1 : 3); ++n_sweep)
-            for (int order = 0; order < (big ? 1 : 2); ++order) {
-                run_case(sh[0], sh[1], sh[2], n_sweep, seed++, false, order);
-                if (!big) run_case(sh[0], sh[1], sh[2], n_sweep, seed++, true, order);
-            }
+        for (int cmax : cmaxes)
+            for (int n_sweep = 1; n_sweep <= (big ? 1 : 3); ++n_sweep)
+                for (int order = 0; order < (big ? 1 : 2); ++order) {
+                    if (big && cmax == 3) continue;
+                    run_case(sh[0], sh[1], sh[2], n_sweep, seed++, false, order, cmax);
+                    if (!big) run_case(sh[0], sh[1], sh[2], n_sweep, seed++, true, order, cmax);
+                }
     }
-    std::printf("ok %ld\n", g_cases);
+    const int huge_shapes[][3] = {{12, 24, 16}, {7, 12, 9}};
+    for (const auto& sh : huge_shapes) run_case(sh[0], sh[1], sh[2], 2, seed++, false, 0, 1 << 20, true);
+    CHECK(g_off_band > 0, "no pass left the band");
+    CHECK(g_chain_passes > 100, "only %ld passes ran the chain kernel", g_chain_passes);
+    std::printf("ok %ld %ld\n", g_cases, g_chain_passes);
     return 0;
 }

@@ -6,7 +6,9 @@ Bar: phi bit-identical to the oracle on every padded cell, halos included
 (the GPU keeps the reference's line order exactly and evaluates the same
 expressions without contraction), after 1 to 5 sweeps, on random diagonally
 dominant systems, rows with zero pivots, the production p' system, ragged
-batches, degenerate axes and device-resident arrays.
+batches, degenerate axes and device-resident arrays; through both kernels
+(k_tdma_chain where it plans a pass, k_tdma_pass for lines of one element,
+pivots outside the chain kernel's band, and EAFP_TDMA_KERNEL=pass).
 """
 import ctypes
 
@@ -126,3 +128,18 @@ def test_tdma_device_resident_and_repeat():
         assert_bitwise(hip.get(phi_d, pad, np.float64), want, "device-resident")
     finally:
         hip.free()
+
+
+@pytest.mark.parametrize("shape", [(12, 24, 16), (60, 120, 84)])
+def test_tdma_both_kernels(shape, monkeypatch):
+    """The same call through k_tdma_pass (EAFP_TDMA_KERNEL=pass) and through
+    the chain kernel (the default wherever it plans the pass): both
+    bit-identical to the oracle."""
+    coef, phi0 = random_system(shape, 300 + shape[0])
+    want = oracle.tdma_3d(coef, phi0, shape, 2)
+    for kernel in ("pass", ""):
+        print(f"case tdma kernel={kernel or 'chain'} shape={shape}", flush=True)
+        monkeypatch.setenv("EAFP_TDMA_KERNEL", kernel)
+        phi = phi0.copy(order="F")
+        eafp.tdma_3d_mpi(*[coef[n] for n in NAMES], phi, eafp.Mesh(*shape), 2)
+        assert_bitwise(phi, want, f"{kernel or 'chain'} {shape}")
