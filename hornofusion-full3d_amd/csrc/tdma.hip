@@ -142,13 +142,10 @@ __device__ __forceinline__ void helper_chunk(const TdPass& p, const TdLds& L, lo
         for (int i = 0; i < TDC_IPL; ++i) chain_prepare<P>(p, L, f, (int)((ph + 1) & 1), h + i * HL, use[i]);
     }
     const bool real = live && f.w > 0;
-    int v_lo = 1, hi = 1;
-    if (real) diagonal_range(f.w - 1, p.nu, p.nv, v_lo, hi);
-    const int mp = hi - v_lo + 1, l0p = f.l0 - mp;
-    const unsigned mM = td_magic((unsigned)mp);
+    const int mp = real ? f.pm : 1;
 #pragma unroll
     for (int i = 0; i < TDC_IPL; ++i)
-        chain_copy_out(p, L, real, f.w - 1, v_lo, mp, mM, l0p, f.e0, f.ce, h + i * HL);
+        chain_copy_out(p, L, real, f.w - 1, f.pv_lo, mp, real ? f.pmM : 0u, f.pl0, f.e0, f.ce, h + i * HL);
 #pragma unroll
     for (int i = 0; i < TDC_IPL; ++i) chain_load<P>(p, fl, h + i * HL, next[i]);
     if (live) {
@@ -258,6 +255,9 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
         for (int P = 0; P < 3; ++P) {
             TdPass t{};
             pass_shape(descs[q], P, t.nu, t.nv, t.n);
+            t.NL = t.nu * t.nv;
+            t.NH = halo_lines(P, t.nu, t.nv);
+            t.NLX = t.NL + t.NH;
             if (!chain_plan(t, 1 << 20)) chain[P] = false;
         }
     for (int q = 0; q < nsys; ++q) {

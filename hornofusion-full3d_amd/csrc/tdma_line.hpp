@@ -193,6 +193,15 @@ __host__ __device__ inline double td_div_band(double num, double m, double r) {
 // and the quotient n; so does m = 0, the empty diagonal past a pass's last,
 // which divides nothing).  tests/tdma_check.cpp checks it against `/` for every
 // m up to 1024 and n up to 2^20 / m.
+// a * b for 0 <= a, b and a * b < 2^24 (full-rate 24-bit multiply on the GPU).
+__host__ __device__ inline int td_mul24(int a, int b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __mul24(a, b);
+#else
+    return a * b;
+#endif
+}
+
 __host__ __device__ inline unsigned td_magic(unsigned m) { return m <= 1 ? 0u : 0xffffffffu / m + 1u; }
 
 __host__ __device__ inline void td_divmod(int n, int m, unsigned M, int& q, int& r) {
@@ -202,7 +211,7 @@ __host__ __device__ inline void td_divmod(int n, int m, unsigned M, int& q, int&
     const unsigned hi = (unsigned)(((unsigned long long)(unsigned)n * M) >> 32);
 #endif
     q = M == 0u ? n : (int)hi;
-    r = n - q * m;
+    r = n - td_mul24(q, m);
 }
 
 __host__ __device__ inline void diagonal_range(int w, int nu, int nv, int& v_lo, int& v_hi) {
@@ -643,7 +652,7 @@ __host__ __device__ inline int odd_up(int x) { return x | 1; }
 constexpr int TDC_PAD = 4 * TDC_G;  // doubles before and after the arrays: reads ahead stay in the block
 
 __host__ __device__ inline bool chain_plan(TdPass& p, int cmax) {
-    if (p.n < 2) return false;
+    if (p.n < 2 || (long)p.n * p.NLX >= (1L << 24)) return false;   // element offsets in 24-bit products
     const int ML = p.nu < p.nv ? p.nu : p.nv;
     const int NW = (ML + 63) / 64;
     if (NW > 4) return false;
@@ -705,6 +714,8 @@ struct TdTask {
     int k, e0, ce;      // forward chunk, its first element, its length
     int v_lo, m, l0;    // the diagonal's first v, lines, first line index
     unsigned mM;        // td_magic(m)
+    int pv_lo, pm, pl0; // the same of the diagonal before (pm = 0 on the first)
+    unsigned pmM;
 };
 
 __host__ __device__ inline void chain_set_kind(const TdPass& p, TdTask& t) {
@@ -725,11 +736,19 @@ __host__ __device__ inline TdTask chain_first(const TdPass& p) {
     t.m = 1;
     t.mM = td_magic(1);
     t.l0 = 0;
+    t.pv_lo = 1;
+    t.pm = 0;
+    t.pl0 = 0;
+    t.pmM = 0;
     chain_set_kind(p, t);
     return t;
 }
 
 __host__ __device__ inline void chain_next_diagonal(const TdPass& p, TdTask& t) {
+    t.pv_lo = t.v_lo;
+    t.pm = t.m;
+    t.pl0 = t.l0;
+    t.pmM = t.mM;
     t.l0 += t.m;
     ++t.w;
     int hi;
@@ -787,9 +806,9 @@ __host__ __device__ inline void chain_load(const TdPass& p, const TdTask& t, int
     const int cnt = t.ce * t.m, itc = it < cnt ? it : cnt - 1;
     int tt, j;
     td_divmod(itc, t.m, t.mM, tt, j);
-    const long x0 = (long)(t.e0 + tt) * p.NL + t.l0 + j;
+    const unsigned x0 = (unsigned)(td_mul24(t.e0 + tt, p.NL) + t.l0 + j);
     typedef const TD_GLOBAL double2* cgd2ptr;
-    const cgd2ptr rp = reinterpret_cast<cgd2ptr>(p.rec + x0 * RC_N);
+    const cgd2ptr rp = reinterpret_cast<cgd2ptr>(p.rec) + x0 * (unsigned)(RC_N / 2);
 #pragma unroll
     for (int q = 0; q < RC_N / 2; ++q) {
         const double2 v2 = rp[q];
@@ -820,8 +839,7 @@ __host__ __device__ inline void chain_prepare(const TdPass& p, const TdLds& L, c
     const int sa = P == 2 ? chain_slot(p, u, v - 1) : chain_slot(p, u - 1, v);
     const int sb = P == 2 ? chain_slot(p, u - 1, v) : chain_slot(p, u, v - 1);
     const double* col = L.xb + e;
-    const long LS = L.LS;
-    const double na = col[LS * (sa > 0 ? sa : 0)], nb = col[LS * (sb > 0 ? sb : 0)];
+    const double na = col[td_mul24(sa > 0 ? sa : 0, L.LS)], nb = col[td_mul24(sb > 0 ? sb : 0, L.LS)];
     const double p1 = a_edge ? x.r[RC_O1] : x.r[RC_O1] * na;
     const double p3 = b_edge ? x.r[RC_O3] : x.r[RC_O3] * nb;
     double d = x.r[RC_SU] + p1;
@@ -830,10 +848,10 @@ __host__ __device__ inline void chain_prepare(const TdPass& p, const TdLds& L, c
     d = d + x.r[RC_P4];
     const double dq = d + x.r[RC_Q];
     d = e == 0 || e == p.n - 1 ? dq : d;
-    const long at = b * L.ring + (long)j * L.CS + tt;
+    const int at = td_mul24(b, (int)L.ring) + td_mul24(j, L.CS) + tt;
     L.dr[at] = make_double2(d, x.r[RC_R]);
     L.am[at] = make_double2(x.r[RC_A], x.r[RC_M]);
-    L.cp[(long)j * L.LS + e] = x.r[RC_C];
+    L.cp[td_mul24(j, L.LS) + e] = x.r[RC_C];
 }
 
 // Entry `it` of chunk (e0, ce) of diagonal w's x (m lines from v_lo, first
@@ -847,8 +865,8 @@ __host__ __device__ inline void chain_copy_out(const TdPass& p, const TdLds& L, 
     td_divmod(itc, m, mM, tt, j);
     const int e = e0 + tt;
     const int v = v_lo + j, u = w + 2 - v, sl = chain_slot(p, u, v);
-    const double x = L.xb[(long)(sl > 0 ? sl : 0) * L.LS + e];
-    gdptr dst = real ? p.phi + (long)e * p.NLX + l0 + j : p.sink + (it & (TDC_BLOCK - 1));
+    const double x = L.xb[td_mul24(sl > 0 ? sl : 0, L.LS) + e];
+    gdptr dst = real ? p.phi + (unsigned)(td_mul24(e, p.NLX) + l0 + j) : p.sink + (it & (TDC_BLOCK - 1));
     *dst = x;
 }
 

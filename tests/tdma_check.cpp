@@ -176,13 +176,16 @@ static void replay_chain(const TdSys& s, int order) {
             // the copy of the diagonal before's chunk (the sink on the first diagonal)
             {
                 const bool real = f.w > 0;
-                int v_lo = 1, hi = 1;
+                int v_lo = 1, hi = 0;
                 if (real) diagonal_range(f.w - 1, p.nu, p.nv, v_lo, hi);
-                const int mp = hi - v_lo + 1, l0p = f.l0 - mp;
+                CHECK(!real || (f.pv_lo == v_lo && f.pm == hi - v_lo + 1 && f.pl0 == f.l0 - f.pm &&
+                                f.pmM == td_magic((unsigned)f.pm)),
+                      "pass %d: the iterator's diagonal before %d", P, f.w);
+                const int mp = real ? f.pm : 1;
                 CHECK(!real || (long)f.ce * mp <= (long)TDC_IPL * HL, "pass %d: copy of %d x %d", P, f.ce, mp);
                 for (int h = 0; h < HL; ++h)
                     for (int i = 0; i < TDC_IPL; ++i)
-                        chain_copy_out(p, L, real, f.w - 1, v_lo, mp, td_magic((unsigned)mp), l0p, f.e0, f.ce,
+                        chain_copy_out(p, L, real, f.w - 1, f.pv_lo, mp, real ? f.pmM : 0u, f.pl0, f.e0, f.ce,
                                        h + i * HL);
             }
             for (long tt = 0; tt < L.CS; ++tt)
