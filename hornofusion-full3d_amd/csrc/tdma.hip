@@ -19,11 +19,20 @@
 // diagonal-major layout (tdma_line.hpp): for every element index the lines of
 // one diagonal are contiguous, so a work-group loads a whole diagonal
 // coalesced whatever the pass direction.  Kernels:
-//   k_tdma_setup   once per call, one thread per line of each pass: packs the
-//                  right-hand-side coefficients, a(e), the pivots m(e) and
-//                  c'(e) of the forward elimination (they depend on the
-//                  coefficients only), the wall halo lines, and pass 0's phi
-//   k_tdma_pass<P> one 1024-thread work-group per system runs one pass: per
+//   k_tdma_pack    once per call, one thread per cell, reading the caller's
+//                  arrays in their own order: the right-hand-side
+//                  coefficients, a(e), aP and a_plus of every pass, and pass
+//                  0's phi, into the passes' layouts
+//   k_tdma_setup   once per call, one thread per line of each pass: the
+//                  pivots m(e) and c'(e) of the forward elimination (they
+//                  depend on the coefficients only), the chain kernel's
+//                  records, the end terms and the wall halo lines
+//   k_tdma_prep<P> and k_tdma_chain<P>: the pass on one work-group per system,
+//                  the lines of a diagonal on the lanes of the chain waves
+//                  (the schedule is described in tdma_line.hpp); it runs a
+//                  pass wherever chain_plan admits it and the pass's pivots
+//                  are in its division's band
+//   k_tdma_pass<P> the other passes: one 1024-thread work-group per system: per
 //                  diagonal, all threads stage d(e) (the right-hand side from
 //                  the neighbour lines, whose indices follow from the
 //                  diagonal starts), a(e), m(e) and c'(e) of the diagonal's
@@ -39,7 +48,7 @@
 //                  wall columns
 // A single system is latency-bound by design: one CU walks the
 // (nth+nz-1) + (nr+nz-1) + (nr+nth-1) diagonals of a sweep, each a chain of
-// dependent FP64 divisions; independent systems (the equations of one time
+// dependent FP64 steps; independent systems (the equations of one time
 // step) run on other CUs side by side.
 //
 // Built without floating-point contraction, with the reference's operand
@@ -62,6 +71,12 @@ constexpr int TD_BLOCK = 256;          // threads per work-group of the all-CU k
 constexpr int TD_PASS_BLOCK = 1024;    // threads of a pass work-group (one per system)
 
 using namespace tdma;
+
+__global__ __launch_bounds__(TD_BLOCK) void k_tdma_pack(const TdSys* __restrict__ sys) {
+    const TdSys& s = sys[blockIdx.y];
+    const long c = (long)blockIdx.x * TD_BLOCK + threadIdx.x;
+    if (c < s.ncells) pack_cell(s, c);
+}
 
 __global__ __launch_bounds__(TD_BLOCK) void k_tdma_setup(const TdSys* __restrict__ sys) {
     const TdSys& s = sys[blockIdx.y];
@@ -274,8 +289,8 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
             int nu, nv, n;
             pass_shape(d, P, nu, nv, n);
             const long NL = (long)nu * nv, NLX = NL + halo_lines(P, nu, nv);
-            dwork += PK_N * ncells + LN_N * NL + (long)n * NLX + (chain[P] ? RC_N * ncells + TDC_BLOCK : 0);
-            iwork += nu + nv - 1;
+            dwork += PK_N * ncells + LN_N * NL + (long)n * NLX + (chain[P] ? (RC_REC + 2) * ncells + TDC_BLOCK + 2 : 0);
+            iwork += nu + nv - 1 + (chain[P] ? NL : 0);
             lines += NL;
         }
         iwork += 3;     // off_band flags
@@ -339,14 +354,20 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
             dcur += LN_N * (long)p.NL;
             p.phi = (gdptr)dcur;
             dcur += (long)p.n * p.NLX;
-            p.rec = p.sink = nullptr;
+            p.rec = p.sink = p.pp = nullptr;
+            p.uv = nullptr;
             p.C = p.K = p.ML = p.by_u = p.NW = p.pad_ = 0;
             if (chain[P]) {
                 if (!chain_plan(p, 1 << 20)) return EAFP_ERR_UNSUPPORTED;   // admitted above: cannot happen
+                dcur += (dcur - w.work.p) & 1;      // 16-byte records and pairs
                 p.rec = (gdptr)dcur;
-                dcur += RC_N * t.ncells;
+                dcur += RC_REC * t.ncells;
+                p.pp = (gdptr)dcur;
+                dcur += 2 * t.ncells;
                 p.sink = (gdptr)dcur;
                 dcur += TDC_BLOCK;
+                p.uv = (decltype(p.uv))(icur + ioff);
+                ioff += p.NL;
             }
             p.dstart = (cgiptr)(icur + ioff);
             int acc = 0;
@@ -366,6 +387,7 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
     const dim3 g_cells((unsigned)((max_cells + TD_BLOCK - 1) / TD_BLOCK), ny);
     const dim3 g_cols((unsigned)((max_cols + TD_BLOCK - 1) / TD_BLOCK), ny);
     const TdSys* tp = w.table.p;
+    k_tdma_pack<<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
     k_tdma_setup<<<g_lines, TD_BLOCK, 0, w.stream>>>(tp);
     for (int sweep = 0; sweep < n_sweep; ++sweep) {
         // with the chain kernel planned, k_tdma_pass still runs the systems whose

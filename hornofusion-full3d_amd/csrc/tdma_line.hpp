@@ -54,8 +54,14 @@ enum { LN_AMF = 0, LN_APL, LN_LO, LN_HI, LN_N };
 // itself on lines whose first / second updated neighbour is a wall or a
 // wrap line; RC_Q the end term a_minus(1) phi(before) of element 0 or
 // a_plus(n) phi(after) of element n-1), the forward elimination's a(e),
-// m(e) and c'(e), and the division's reciprocal part 1/m(e).
-enum { RC_SU = 0, RC_O1, RC_O3, RC_A, RC_M, RC_Q, RC_P2, RC_P4, RC_C, RC_R, RC_N };
+// m(e) and c'(e), and the division's reciprocal part 1/m(e); RC_P2 and
+// RC_P4 live beside the record, in the pass's plane of 16-byte pairs that
+// k_tdma_prep streams out.  Fields come in pairs written together.
+enum { RC_SU = 0, RC_O1, RC_O3, RC_A, RC_M, RC_C, RC_R, RC_Q, RC_REC, RC_P2 = RC_REC, RC_P4, RC_N };
+static_assert(RC_O1 == RC_SU + 1 && RC_A == RC_O3 + 1 && RC_C == RC_M + 1 && RC_Q == RC_R + 1 &&
+                  RC_P4 == RC_P2 + 1 && RC_SU % 2 == 0 && RC_O3 % 2 == 0 && RC_M % 2 == 0 && RC_R % 2 == 0 &&
+                  RC_REC % 2 == 0 && RC_N % 2 == 0,
+              "records are pairs of 16-byte-aligned doubles");
 
 constexpr int TD_LDS_ELEMS = 5120;   // elements staged per group: d, a, m, c', 160 KiB
 constexpr int TD_RING = 8;           // staged values read this many elements ahead of the recurrences
@@ -67,8 +73,10 @@ struct TdPass {
     gdptr pk;                  // PK_N * n * NL
     gdptr ln;                  // LN_N * NL
     gdptr phi;                 // n * NLX
-    gdptr rec;                 // RC_N * n * NL records, or null: the pass runs k_tdma_pass
+    gdptr rec;                 // RC_REC * n * NL records (16-byte aligned), or null: the pass runs k_tdma_pass
+    gdptr pp;                  // 2 * n * NL: RC_P2, RC_P4 of element-line x at 2x (16-byte aligned)
     gdptr sink;                // TDC_BLOCK doubles the helpers' idle stores go to
+    TD_GLOBAL int* uv;         // line l's (u, v) as u | v << 16 (chain kernel), written by k_tdma_setup
     int C, K;                  // chain kernel: elements per chunk, chunks per line
     int ML, by_u;              // chain kernel: x slots (min(nu, nv)), slot = u-1 (by_u) or v-1
     int NW, pad_;              // chain kernel: chain waves
@@ -363,52 +371,71 @@ __host__ __device__ inline void fill_wall_lines(const TdSys& s, int P, int u, in
     }
 }
 
-// Everything per call for line (u,v) of pass P: the packed right-hand-side
-// coefficients, a(e) = -a_minus(e), the pivots m(e) and c'(e) of the
-// reference's forward elimination (they do not depend on phi), the end
-// coefficients and wall values, the wall halo lines, and for pass 0 the
-// line's initial phi.
+// The caller's coefficients of interior cell c (r fastest), packed into
+// every pass's element-major planes (k_tdma_pack: coalesced reads in the
+// caller's order): Su, the four off-line coefficients, a(e) = -a_minus(e),
+// and, until setup_line replaces them with the pivots, aP in the m plane
+// and a_plus in the c' plane; for pass 0 also the cell's initial phi.
+__host__ __device__ inline void pack_cell(const TdSys& s, long c) {
+    const int i = (int)(c % s.nr) + 1;
+    const long rest = c / s.nr;
+    const int j = (int)(rest % s.nth) + 1, k = (int)(rest / s.nth) + 1;
+    const long cc = tat(s, i, j, k);
+    for (int P = 0; P < 3; ++P) {
+        const TdPass& p = s.ps[P];
+        int u, v, e;
+        line_of_cell(P, i, j, k, u, v, e);
+        int am, ap, o[4];
+        pass_slots(P, am, ap, o);
+        const long plane = (long)p.n * p.NL;
+        gdptr x = p.pk + (long)e * p.NL + line_index(p, u, v);
+        x[PK_SU * plane] = s.c[CSU][cc];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[(PK_O1 + q) * plane] = s.c[o[q]][cc];
+        x[PK_A * plane] = -s.c[am][cc];
+        x[PK_M * plane] = s.c[CP_][cc];
+        x[PK_C * plane] = s.c[ap][cc];
+        if (P == 0) p.phi[(long)e * p.NLX + line_index(p, u, v)] = s.home[cc];
+    }
+}
+
+// Everything else per call for line (u,v) of pass P, after pack_cell: the
+// pivots m(e) and c'(e) of the reference's forward elimination (they do not
+// depend on phi) in place of aP and a_plus, the chain kernel's records, the
+// end coefficients and wall values, and the wall halo lines.  The packed
+// planes keep a line's element e at e*NL + l, so the threads of the lines of
+// a diagonal read and write them side by side.
 __host__ __device__ inline void setup_line(const TdSys& s, int P, int u, int v) {
     const TdPass& p = s.ps[P];
     const int l = line_index(p, u, v);
     const long NL = p.NL, n = p.n, plane = n * NL;
     int am, ap, o[4];
     pass_slots(P, am, ap, o);
-    double cp = 0.0;
+    double cp = 0.0, a_first = 0.0, ap_last = 0.0;
     for (int e = 0; e < p.n; ++e) {
-        int i, j, k;
-        cell_of(P, u, v, e, i, j, k);
-        const long c = tat(s, i, j, k);
         gdptr x = p.pk + (long)e * NL + l;
-        x[PK_SU * plane] = s.c[CSU][c];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) x[(PK_O1 + q) * plane] = s.c[o[q]][c];
-        const double a = -s.c[am][c];
+        const double a = x[PK_A * plane], aP = x[PK_M * plane], aplus = x[PK_C * plane];
+        if (e == 0) a_first = a;
+        if (e == p.n - 1) ap_last = aplus;
         double m;
-        if (e == 0) m = clamp_pivot(s.c[CP_][c]);
-        else m = clamp_pivot(s.c[CP_][c] - a * cp);
-        cp = -s.c[ap][c] / m;
-        x[PK_A * plane] = a;
+        if (e == 0) m = clamp_pivot(aP);
+        else m = clamp_pivot(aP - a * cp);
+        cp = -aplus / m;
         x[PK_M * plane] = m;
         x[PK_C * plane] = e == p.n - 1 ? 0.0 : cp;    // c'(n-1) takes no part in the recurrences
         if (p.rec) {
-            gdptr r = p.rec + ((long)e * NL + l) * RC_N;
-            r[RC_SU] = s.c[CSU][c];
-            r[RC_O1] = s.c[o[0]][c];
-            r[RC_O3] = s.c[o[2]][c];
-            r[RC_A] = e == 0 ? 0.0 : a;                    // a(0) takes no part in the recurrences
-            r[RC_M] = m;
-            r[RC_C] = e == p.n - 1 ? 0.0 : cp;
-            r[RC_R] = td_recip_banded(m);
+            typedef TD_GLOBAL double2* gd2ptr;
+            const gd2ptr r = reinterpret_cast<gd2ptr>(p.rec + ((long)e * NL + l) * RC_REC);
+            r[RC_SU / 2] = make_double2(x[PK_SU * plane], x[PK_O1 * plane]);
+            r[RC_O3 / 2] = make_double2(x[PK_O3 * plane], e == 0 ? 0.0 : a);   // a(0) takes no part in the recurrences
+            r[RC_M / 2] = make_double2(m, e == p.n - 1 ? 0.0 : cp);            // nor does c'(n-1)
+            r[RC_R / 2] = make_double2(td_recip_banded(m), 0.0);
             if (td_bexp(m) - TD_M_LO > TD_M_SPAN) s.off_band[P] = 1;
         }
-        if (P == 0) p.phi[(long)e * p.NLX + l] = s.home[c];
     }
-    int i0, j0, k0, i1, j1, k1;
-    cell_of(P, u, v, 0, i0, j0, k0);
-    cell_of(P, u, v, p.n - 1, i1, j1, k1);
-    p.ln[LN_AMF * NL + l] = s.c[am][tat(s, i0, j0, k0)];
-    p.ln[LN_APL * NL + l] = s.c[ap][tat(s, i1, j1, k1)];
+    if (p.uv) p.uv[l] = u | v << 16;
+    p.ln[LN_AMF * NL + l] = -a_first;        // a_minus(0): the negation is exact
+    p.ln[LN_APL * NL + l] = ap_last;
     // wall values beyond the two ends (pass 1 reads its own line instead)
     if (P == 0) {
         p.ln[LN_LO * NL + l] = s.home[tat(s, 0, u, v)];
@@ -808,13 +835,16 @@ __host__ __device__ inline void chain_load(const TdPass& p, const TdTask& t, int
     td_divmod(itc, t.m, t.mM, tt, j);
     const unsigned x0 = (unsigned)(td_mul24(t.e0 + tt, p.NL) + t.l0 + j);
     typedef const TD_GLOBAL double2* cgd2ptr;
-    const cgd2ptr rp = reinterpret_cast<cgd2ptr>(p.rec) + x0 * (unsigned)(RC_N / 2);
+    const cgd2ptr rp = reinterpret_cast<cgd2ptr>(p.rec) + x0 * (unsigned)(RC_REC / 2);
 #pragma unroll
-    for (int q = 0; q < RC_N / 2; ++q) {
+    for (int q = 0; q < RC_REC / 2; ++q) {
         const double2 v2 = rp[q];
         x.r[2 * q] = v2.x;
         x.r[2 * q + 1] = v2.y;
     }
+    const double2 pq = reinterpret_cast<cgd2ptr>(p.pp)[x0];
+    x.r[RC_P2] = pq.x;
+    x.r[RC_P4] = pq.y;
 }
 
 // d(e) of ring entry `it` of forward chunk t, in the reference's operand
@@ -1015,17 +1045,17 @@ __host__ __device__ inline void chain_backward(const TdPass& p, const TdLds& L, 
 // and for e = 0 and e = n-1 the line's end-term product RC_Q.
 template <int P>
 __host__ __device__ inline void prep_elem(const TdPass& p, int l, int e) {
-    int u, v;
-    line_at(p, l, u, v);
+    const int uv = p.uv[l], u = uv & 0xffff, v = uv >> 16;
     int nb[4];
     line_neighbours(p, P, u, v, nb);
     if (P == 0 && u == p.nu) nb[1] = line_index(p, 1, v);
     if (P == 2 && v == p.nv) nb[1] = line_index(p, u, 1);
     const long NL = p.NL, plane = (long)p.n * NL, x = (long)e * NL + l;
     cgdptr row = p.phi + (long)e * p.NLX;
-    gdptr r = p.rec + x * RC_N;
-    r[RC_P2] = p.pk[PK_O2 * plane + x] * row[nb[1]];
-    r[RC_P4] = p.pk[PK_O4 * plane + x] * row[nb[3]];
+    gdptr r = p.rec + x * RC_REC;
+    typedef TD_GLOBAL double2* gd2ptr;
+    reinterpret_cast<gd2ptr>(p.pp)[x] =
+        make_double2(p.pk[PK_O2 * plane + x] * row[nb[1]], p.pk[PK_O4 * plane + x] * row[nb[3]]);
     if (P == 2 ? v == 1 : u == 1) r[RC_O1] = p.pk[PK_O1 * plane + x] * row[nb[0]];
     if (P == 2 ? u == 1 : v == 1) r[RC_O3] = p.pk[PK_O3 * plane + x] * row[nb[2]];
     if (e == 0) {

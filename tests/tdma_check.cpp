@@ -3,8 +3,12 @@
 // Runs the library's own per-line, per-element and per-cell code
 // (csrc/tdma_line.hpp, the functions the kernels call) on the CPU in the
 // order the kernels impose:
-//   - k_tdma_setup: every line of every pass (thread = line index);
-//   - per sweep and pass, k_tdma_pass: the snapshot of the wrap lines, then
+//   - k_tdma_pack: every interior cell; k_tdma_setup: every line of every
+//     pass (thread = line index);
+//   - per sweep and pass, k_tdma_prep and k_tdma_chain phase by phase
+//     (replay_chain, below) where chain_plan admits the pass (with chunks
+//     capped at 3 elements, or as planned) and its pivots are in band, else
+//     k_tdma_pass: the snapshot of the wrap lines, then
 //     each anti-diagonal in groups of lines: all elements staged (threads in
 //     reverse order), then the lines' recurrences (in reverse or rotated
 //     order: the GPU gives them none);
@@ -303,19 +307,30 @@ static void run_case(int nr, int nth, int nz, int n_sweep, unsigned seed, bool z
         CHECK(acc == p.NL, "diagonals cover the lines");
         p.pk = (gdptr)pk; p.ln = (gdptr)ln; p.phi = (gdptr)ph; p.dstart = (cgiptr)ds;
         owned.insert(owned.end(), {pk, ln, ph, ds});
-        p.rec = p.sink = nullptr;
+        p.rec = p.sink = p.pp = nullptr;
+        p.uv = nullptr;
         if (cmax > 0 && chain_plan(p, cmax)) {
-            double* rec = alloc<double>(RC_N * ncells);
-            for (long o = 0; o < RC_N * ncells; ++o) rec[o] = std::nan("");
+            double* rec = alloc<double>(RC_REC * ncells);
+            for (long o = 0; o < RC_REC * ncells; ++o) rec[o] = std::nan("");
+            double* pp = alloc<double>(2 * ncells);
+            for (long o = 0; o < 2 * ncells; ++o) pp[o] = std::nan("");
+            p.pp = (gdptr)pp;
+            owned.push_back(pp);
             double* sink = alloc<double>(TDC_BLOCK);
+            int* uv = alloc<int>(p.NL);
+            for (int l = 0; l < p.NL; ++l) uv[l] = -1;
             p.rec = (gdptr)rec;
             p.sink = (gdptr)sink;
+            p.uv = (decltype(p.uv))uv;
             owned.push_back(rec);
             owned.push_back(sink);
+            owned.push_back(uv);
             ++g_chain_passes;
         }
         lines += p.NL;
     }
+    // k_tdma_pack: every interior cell (threads in reverse)
+    for (long cc = ncells - 1; cc >= 0; --cc) pack_cell(s, cc);
     // k_tdma_setup: thread t = line index within the passes, past the end included
     std::vector<int> setup_seen((size_t)lines, 0);
     for (long t0 = lines + 40; t0 >= 0; --t0) {
