@@ -145,24 +145,30 @@ __global__ __launch_bounds__(TD_BLOCK) void k_tdma_prep(const TdSys* __restrict_
 // chunk f+2 into `next`, and close the phase.  Past the pass's last chunk
 // (`live` false, the same for every lane) and on the first diagonal the
 // copies go to the sink, and past the last chunk nothing is filled and no
-// barrier passed: every call issues the same stores and loads, so the
-// compiler's waits for `use` count exactly the accesses issued after it.
-template <int P>
+// barrier passed.  Every call issues the same stores and loads, so the
+// compiler's waits for `use` count exactly the accesses issued after it; a
+// wave whose entries all lie past the chunk (h0, its first lane's h, is
+// uniform) skips the fill and the copy's LDS read.
+template <int P, int IPL>
 __device__ __forceinline__ void helper_chunk(const TdPass& p, const TdLds& L, long& X, TdTask& f, TdTask& fl, int h,
-                                             int HL, bool live, const TdPre (&use)[TDC_IPL], TdPre (&next)[TDC_IPL]) {
+                                             int h0, int HL, bool live, const TdPre (&use)[IPL], TdPre (&next)[IPL]) {
     const long ph = chain_prep_phase(p, f);
     if (live) {
         for (; X < ph; ++X) __syncthreads();
+        const int cnt = f.ce * f.m;
 #pragma unroll
-        for (int i = 0; i < TDC_IPL; ++i) chain_prepare<P>(p, L, f, (int)((ph + 1) & 1), h + i * HL, use[i]);
+        for (int i = 0; i < IPL; ++i)
+            if (h0 + i * HL < cnt) chain_prepare<P>(p, L, f, (int)((ph + 1) & 1), h + i * HL, use[i]);
     }
     const bool real = live && f.w > 0;
-    const int mp = real ? f.pm : 1;
+    const int pcnt = real ? f.ce * f.pm : 0;
 #pragma unroll
-    for (int i = 0; i < TDC_IPL; ++i)
-        chain_copy_out(p, L, real, f.w - 1, f.pv_lo, mp, real ? f.pmM : 0u, f.pl0, f.e0, f.ce, h + i * HL);
+    for (int i = 0; i < IPL; ++i) {
+        if (h0 + i * HL < pcnt) chain_copy_out(p, L, true, f.w - 1, f.pv_lo, f.pm, f.pmM, f.pl0, f.e0, f.ce, h + i * HL);
+        else p.sink[(h + i * HL) & (TDC_BLOCK - 1)] = 0.0;
+    }
 #pragma unroll
-    for (int i = 0; i < TDC_IPL; ++i) chain_load<P>(p, fl, h + i * HL, next[i]);
+    for (int i = 0; i < IPL; ++i) chain_load<P>(p, fl, h + i * HL, next[i]);
     if (live) {
         __syncthreads();
         ++X;
@@ -173,10 +179,11 @@ __device__ __forceinline__ void helper_chunk(const TdPass& p, const TdLds& L, lo
 
 // One pass of one system per work-group (the schedule is described in
 // tdma_line.hpp): the first NW waves run the recurrences at raised wave
-// priority, the other waves feed them.  Every wave passes the same number
-// of barriers, one per phase; the branch between the two roles is uniform
-// per wave.
-template <int P>
+// priority, the other waves feed them, IPL ring entries per helper lane and
+// chunk (at least the pass's p.IPL).  Every wave passes the same number of
+// barriers, one per phase and one after the x slots' initial ones; the
+// branch between the two roles is uniform per wave.
+template <int P, int IPL>
 __global__ __launch_bounds__(TDC_BLOCK) void k_tdma_chain(const TdSys* __restrict__ sys) {
     __shared__ __align__(16) double lds[TDC_LDS];
     if (sys[blockIdx.x].off_band[P]) return;    // a pivot outside the band: k_tdma_pass runs this pass
@@ -184,6 +191,8 @@ __global__ __launch_bounds__(TDC_BLOCK) void k_tdma_chain(const TdSys* __restric
     const TdLds L = chain_lds(p, lds);
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const long nph = chain_phases(p);
+    chain_init_slots(p, L, (int)threadIdx.x, TDC_BLOCK);
+    __syncthreads();
     if (wave < p.NW) {
         __builtin_amdgcn_s_setprio(3);
         const int j = (int)threadIdx.x;
@@ -197,21 +206,21 @@ __global__ __launch_bounds__(TDC_BLOCK) void k_tdma_chain(const TdSys* __restric
         }
         __builtin_amdgcn_s_setprio(0);
     } else {
-        const int HL = TDC_BLOCK - 64 * p.NW, h = (int)threadIdx.x - 64 * p.NW;
+        const int HL = TDC_BLOCK - 64 * p.NW, h = (int)threadIdx.x - 64 * p.NW, h0 = 64 * (wave - p.NW);
         const long nf = (long)(p.nu + p.nv - 1) * p.K;     // forward chunks of the pass
-        TdPre r0[TDC_IPL], r1[TDC_IPL], r2[TDC_IPL];
+        TdPre r0[IPL], r1[IPL], r2[IPL];
         TdTask f = chain_first_fwd(p), fl = f;
 #pragma unroll
-        for (int i = 0; i < TDC_IPL; ++i) chain_load<P>(p, fl, h + i * HL, r0[i]);
+        for (int i = 0; i < IPL; ++i) chain_load<P>(p, fl, h + i * HL, r0[i]);
         chain_advance_fwd(p, fl);
 #pragma unroll
-        for (int i = 0; i < TDC_IPL; ++i) chain_load<P>(p, fl, h + i * HL, r1[i]);
+        for (int i = 0; i < IPL; ++i) chain_load<P>(p, fl, h + i * HL, r1[i]);
         chain_advance_fwd(p, fl);
         long X = 0;
         for (long q = 0; q < nf; q += 3) {
-            helper_chunk<P>(p, L, X, f, fl, h, HL, true, r0, r2);
-            helper_chunk<P>(p, L, X, f, fl, h, HL, q + 1 < nf, r1, r0);
-            helper_chunk<P>(p, L, X, f, fl, h, HL, q + 2 < nf, r2, r1);
+            helper_chunk<P, IPL>(p, L, X, f, fl, h, h0, HL, true, r0, r2);
+            helper_chunk<P, IPL>(p, L, X, f, fl, h, h0, HL, q + 1 < nf, r1, r0);
+            helper_chunk<P, IPL>(p, L, X, f, fl, h, h0, HL, q + 2 < nf, r2, r1);
         }
         for (; X < nph; ++X) __syncthreads();
     }
@@ -221,6 +230,13 @@ __global__ __launch_bounds__(TDC_BLOCK) void k_tdma_chain(const TdSys* __restric
     const int cnt = p.n * t.m;
     for (int it = (int)threadIdx.x; it < cnt; it += TDC_BLOCK)
         chain_copy_out(p, L, true, t.w, t.v_lo, t.m, t.mM, t.l0, 0, p.n, it);
+}
+
+template <int P>
+void launch_chain(int ipl, int nsys, hipStream_t st, const TdSys* tp) {
+    if (ipl <= 1) k_tdma_chain<P, 1><<<nsys, TDC_BLOCK, 0, st>>>(tp);
+    else if (ipl == 2) k_tdma_chain<P, 2><<<nsys, TDC_BLOCK, 0, st>>>(tp);
+    else k_tdma_chain<P, 3><<<nsys, TDC_BLOCK, 0, st>>>(tp);
 }
 
 template <int P, int Q>
@@ -265,6 +281,7 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
     // EAFP_TDMA_KERNEL=pass keeps k_tdma_pass for every pass.
     const char* force = std::getenv("EAFP_TDMA_KERNEL");
     bool chain[3] = {true, true, true};
+    int ipl[3] = {1, 1, 1};     // the chain kernel's ring entries per helper lane: the most any system needs
     if (force && std::strcmp(force, "pass") == 0) chain[0] = chain[1] = chain[2] = false;
     for (int q = 0; q < nsys; ++q)
         for (int P = 0; P < 3; ++P) {
@@ -274,6 +291,7 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
             t.NH = halo_lines(P, t.nu, t.nv);
             t.NLX = t.NL + t.NH;
             if (!chain_plan(t, 1 << 20)) chain[P] = false;
+            else ipl[P] = std::max(ipl[P], t.IPL);
         }
     for (int q = 0; q < nsys; ++q) {
         EAFP_TRY_RC(validate_desc(&descs[q]));
@@ -356,7 +374,7 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
             dcur += (long)p.n * p.NLX;
             p.rec = p.sink = p.pp = nullptr;
             p.uv = nullptr;
-            p.C = p.K = p.ML = p.by_u = p.NW = p.pad_ = 0;
+            p.C = p.K = p.ML = p.by_u = p.NW = p.IPL = 0;
             if (chain[P]) {
                 if (!chain_plan(p, 1 << 20)) return EAFP_ERR_UNSUPPORTED;   // admitted above: cannot happen
                 dcur += (dcur - w.work.p) & 1;      // 16-byte records and pairs
@@ -394,19 +412,19 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
         // pivots leave its band (each kernel returns at once for the others)
         if (chain[0]) {
             k_tdma_prep<0><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
-            k_tdma_chain<0><<<nsys, TDC_BLOCK, 0, w.stream>>>(tp);
+            launch_chain<0>(ipl[0], nsys, w.stream, tp);
         }
         k_tdma_pass<0><<<nsys, TD_PASS_BLOCK, 0, w.stream>>>(tp);
         k_tdma_move<0, 1><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
         if (chain[1]) {
             k_tdma_prep<1><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
-            k_tdma_chain<1><<<nsys, TDC_BLOCK, 0, w.stream>>>(tp);
+            launch_chain<1>(ipl[1], nsys, w.stream, tp);
         }
         k_tdma_pass<1><<<nsys, TD_PASS_BLOCK, 0, w.stream>>>(tp);
         k_tdma_move<1, 2><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
         if (chain[2]) {
             k_tdma_prep<2><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
-            k_tdma_chain<2><<<nsys, TDC_BLOCK, 0, w.stream>>>(tp);
+            launch_chain<2>(ipl[2], nsys, w.stream, tp);
         }
         k_tdma_pass<2><<<nsys, TD_PASS_BLOCK, 0, w.stream>>>(tp);
         if (sweep + 1 < n_sweep) k_tdma_move<2, 0><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);

@@ -48,19 +48,21 @@ enum { CW = 0, CE, CS, CN, CB, CT, CP_, CSU, TD_NCOEF };
 enum { PK_SU = 0, PK_O1, PK_O2, PK_O3, PK_O4, PK_A, PK_M, PK_C, PK_N };
 // Packed per-line arrays of a pass.
 enum { LN_AMF = 0, LN_APL, LN_LO, LN_HI, LN_N };
-// Per element-line record of a pass for the chain kernel, 80 B: the
-// right-hand side's coefficients and, per pass, its products with the
-// values that do not change during the pass (RC_O1 / RC_O3 hold the product
-// itself on lines whose first / second updated neighbour is a wall or a
-// wrap line; RC_Q the end term a_minus(1) phi(before) of element 0 or
-// a_plus(n) phi(after) of element n-1), the forward elimination's a(e),
-// m(e) and c'(e), and the division's reciprocal part 1/m(e); RC_P2 and
-// RC_P4 live beside the record, in the pass's plane of 16-byte pairs that
-// k_tdma_prep streams out.  Fields come in pairs written together.
-enum { RC_SU = 0, RC_O1, RC_O3, RC_A, RC_M, RC_C, RC_R, RC_Q, RC_REC, RC_P2 = RC_REC, RC_P4, RC_N };
-static_assert(RC_O1 == RC_SU + 1 && RC_A == RC_O3 + 1 && RC_C == RC_M + 1 && RC_Q == RC_R + 1 &&
-                  RC_P4 == RC_P2 + 1 && RC_SU % 2 == 0 && RC_O3 % 2 == 0 && RC_M % 2 == 0 && RC_R % 2 == 0 &&
-                  RC_REC % 2 == 0 && RC_N % 2 == 0,
+// Per element-line record of a pass for the chain kernel, 48 B in three
+// planes of 16-byte pairs (pair q of element-line x at (q * n * NL + x) * 2,
+// so the lanes of a wave, consecutive lines of one element, read whole
+// cache lines): the right-hand side's coefficients Su, o1, o3, and the
+// forward elimination's a(e), m(e), c'(e).  Per pass, k_tdma_prep writes
+// into them what changes with the pass's phi: RC_O1 / RC_O3 hold the
+// product itself on lines whose first / second updated neighbour is a wall
+// or a wrap line, and the slots of a(0) and c'(n-1), which take no part in
+// the recurrences, hold the end terms a_minus(1) phi(before) and a_plus(n)
+// phi(after).  RC_P2 and RC_P4, the products with the two neighbours that
+// do not change during the pass, live beside the record in the pass's plane
+// of 16-byte pairs that k_tdma_prep streams out.
+enum { RC_SU = 0, RC_O1, RC_O3, RC_A, RC_M, RC_C, RC_REC, RC_P2 = RC_REC, RC_P4, RC_N };
+static_assert(RC_O1 == RC_SU + 1 && RC_A == RC_O3 + 1 && RC_C == RC_M + 1 && RC_P4 == RC_P2 + 1 &&
+                  RC_SU % 2 == 0 && RC_O3 % 2 == 0 && RC_M % 2 == 0 && RC_REC % 2 == 0 && RC_N % 2 == 0,
               "records are pairs of 16-byte-aligned doubles");
 
 constexpr int TD_LDS_ELEMS = 5120;   // elements staged per group: d, a, m, c', 160 KiB
@@ -73,14 +75,19 @@ struct TdPass {
     gdptr pk;                  // PK_N * n * NL
     gdptr ln;                  // LN_N * NL
     gdptr phi;                 // n * NLX
-    gdptr rec;                 // RC_REC * n * NL records (16-byte aligned), or null: the pass runs k_tdma_pass
+    gdptr rec;                 // RC_REC * n * NL: the records' planes (16-byte aligned), or null: k_tdma_pass
     gdptr pp;                  // 2 * n * NL: RC_P2, RC_P4 of element-line x at 2x (16-byte aligned)
     gdptr sink;                // TDC_BLOCK doubles the helpers' idle stores go to
     TD_GLOBAL int* uv;         // line l's (u, v) as u | v << 16 (chain kernel), written by k_tdma_setup
     int C, K;                  // chain kernel: elements per chunk, chunks per line
     int ML, by_u;              // chain kernel: x slots (min(nu, nv)), slot = u-1 (by_u) or v-1
-    int NW, pad_;              // chain kernel: chain waves
+    int NW, IPL;               // chain kernel: chain waves, ring entries per helper lane and chunk
 };
+
+// Field f of element-line x of pass p's records.
+__host__ __device__ inline gdptr rec_at(const TdPass& p, int f, long x) {
+    return p.rec + ((long)(f >> 1) * p.n * p.NL + x) * 2 + (f & 1);
+}
 
 struct TdSys {
     int nr, nth, nz, pad_;
@@ -147,17 +154,12 @@ __host__ __device__ inline unsigned td_bexp(double x) {
 }
 
 // The chain kernel's division (td_div_band) holds for a divisor m with
-// 2^-300 <= |m| < 2^301 (biased exponent 723..1323) and a numerator num
-// that is +0 or has 2^-400 <= |num| < 2^401 (biased exponent 623..1423);
-// td_recip_banded is td_recip for such m and NaN otherwise, and
-// td_num_dev(num) - the caller's check - is at most TD_N_SPAN exactly for
-// such num.
+// 2^-300 <= |m| < 2^301 (biased exponent 723..1323; td_bexp(m) - TD_M_LO
+// at most TD_M_SPAN) and a numerator num that is +0 or has 2^-400 <= |num|
+// < 2^401 (biased exponent 623..1423); td_num_dev(num) - the caller's check
+// - is at most TD_N_SPAN exactly for such num.
 constexpr unsigned TD_M_LO = 723, TD_M_SPAN = 600;
 constexpr unsigned TD_N_LO = 623, TD_N_SPAN = 800;
-
-__host__ __device__ inline double td_recip_banded(double m) {
-    return td_bexp(m) - TD_M_LO <= TD_M_SPAN ? td_recip(m) : __builtin_nan("");
-}
 
 // 0 for num = +0, else the distance of num's biased exponent above
 // TD_N_LO (wrapping to a large value below it).
@@ -171,7 +173,7 @@ __host__ __device__ inline unsigned td_num_dev(double num) {
     return b == 0 ? 0u : ((unsigned)(b >> 52) & 0x7ffu) - TD_N_LO;
 }
 
-// num / m from r = td_recip_banded(m): the hardware-assisted division
+// num / m from r = td_recip(m): the hardware-assisted division
 // without its scaling and fix-up steps (q = num r, the residual, the fused
 // correction).  For m and num in the bands above the division's scale steps
 // change neither operand (their exponents are less than 768 apart, neither
@@ -425,11 +427,11 @@ __host__ __device__ inline void setup_line(const TdSys& s, int P, int u, int v) 
         x[PK_C * plane] = e == p.n - 1 ? 0.0 : cp;    // c'(n-1) takes no part in the recurrences
         if (p.rec) {
             typedef TD_GLOBAL double2* gd2ptr;
-            const gd2ptr r = reinterpret_cast<gd2ptr>(p.rec + ((long)e * NL + l) * RC_REC);
-            r[RC_SU / 2] = make_double2(x[PK_SU * plane], x[PK_O1 * plane]);
-            r[RC_O3 / 2] = make_double2(x[PK_O3 * plane], e == 0 ? 0.0 : a);   // a(0) takes no part in the recurrences
-            r[RC_M / 2] = make_double2(m, e == p.n - 1 ? 0.0 : cp);            // nor does c'(n-1)
-            r[RC_R / 2] = make_double2(td_recip_banded(m), 0.0);
+            const long xr = (long)e * NL + l;
+            // a(0) and c'(n-1) take no part in the recurrences: their slots carry the end terms
+            *reinterpret_cast<gd2ptr>(rec_at(p, RC_SU, xr)) = make_double2(x[PK_SU * plane], x[PK_O1 * plane]);
+            *reinterpret_cast<gd2ptr>(rec_at(p, RC_O3, xr)) = make_double2(x[PK_O3 * plane], e == 0 ? 0.0 : a);
+            *reinterpret_cast<gd2ptr>(rec_at(p, RC_M, xr)) = make_double2(m, e == p.n - 1 ? 0.0 : cp);
             if (td_bexp(m) - TD_M_LO > TD_M_SPAN) s.off_band[P] = 1;
         }
     }
@@ -635,23 +637,30 @@ __host__ __device__ inline void wall_halo_column(const TdSys& s, long t) {
 // diagonal, n forward and n backward steps.  The kernel keeps that chain
 // short and feeds it from LDS:
 //   - chain waves (the first NW waves, one lane per line of the diagonal)
-//     run only the recurrences: forward d'(e) = (d(e) - a(e) d'(e-1)) / m(e)
-//     with the division's reciprocal precomputed (td_div_band), then back
-//     substitution x(e) = d'(e) - c'(e) x(e+1), storing each x to the pass's
-//     phi row as it is formed (stores nobody waits for);
-//   - helper waves (the rest) stream the static per-element data from HBM
-//     into LDS ahead of the chain and form d(e) from the previous diagonal's
-//     x, kept in LDS.  They issue loads and nothing else to global memory,
-//     the same loads for every task, so their waits are exact.
+//     run only the recurrences, from LDS: forward d'(e) = (d(e) - a(e)
+//     d'(e-1)) / m(e) with the division's reciprocal formed ahead
+//     (td_div_band), then back substitution x(e) = d'(e) - c'(e) x(e+1);
+//   - helper waves (the rest) stream the records from HBM into LDS ahead of
+//     the chain, form d(e) from the previous diagonal's x, kept in LDS, and
+//     copy each finished x to the pass's phi row.  Every helper issues the
+//     same loads and stores for every chunk (a lane past the chunk repeats
+//     its last entry; a store with nothing to copy goes to the sink), so the
+//     compiler's waits for a chunk's loads count exactly the accesses issued
+//     after them; a wave with no entry of a chunk skips the LDS work.
 // x lives in LDS slots: the line (u,v) uses slot u-1 (by_u) or v-1, which
 // it shares with one of its two updated neighbours, (u-1,v) or (u,v-1); the
-// other sits in the slot before.  Every LDS array keeps a line's (or a
-// slot's) elements contiguous, rows of odd length so that the lanes of a
-// wave, one row each, fall in different banks; the chain then reads and
-// writes its elements at fixed offsets from one address per row.  Each slot holds, element by element, the
-// previous diagonal's x until the helpers have read it, then d'(e), then
-// the new x.  The forward sweep takes a line's elements in K chunks of C;
-// each diagonal is K+2 phases separated by work-group barriers:
+// other sits in the slot before.  A neighbour outside the line grid (a wall
+// or a wrap line) reads 1.0: from the row of ones before slot 0, or from
+// its line's own slot, which holds the 1.0 it starts with until the first
+// line of that slot is solved; the record then holds the neighbour's
+// product in place of its coefficient, and product * 1.0 is the product.
+// Every LDS array keeps a line's (or a slot's) elements contiguous, rows of
+// odd length so that the lanes of a wave, one row each, fall in different
+// banks; the chain then reads and writes its elements at fixed offsets from
+// one address per row.  Each slot holds, element by element, the previous
+// diagonal's x until the helpers have read it, then d'(e), then the new x.
+// The forward sweep takes a line's elements in K chunks of C; each diagonal
+// is K+2 phases separated by work-group barriers:
 //   G             the helpers form d of chunk 0 (it needs x(0) of the
 //                 diagonal before, just finished); the chain waits
 //   F_0..F_{K-1}  the chain's forward chunks, while the helpers fill the
@@ -665,13 +674,14 @@ __host__ __device__ inline void wall_halo_column(const TdSys& s, long t) {
 // formed after the previous diagonal's B and before F_k overwrites the
 // slots with d'.
 constexpr int TDC_BLOCK = 512;      // threads of a chain work-group
-constexpr int TDC_IPL = 3;          // ring entries a helper lane fills per chunk
+constexpr int TDC_IPL = 3;          // ring entries a helper lane fills per chunk, at most
 constexpr int TDC_LDS = 20480;      // doubles of LDS (160 KiB)
 constexpr int TDC_G = 4;            // ring entries the chain reads per group
 static_assert((TDC_BLOCK & (TDC_BLOCK - 1)) == 0, "the sink index masks with TDC_BLOCK - 1");
 
 // Plans the chain kernel for pass p: chunk length C (at most cmax), chunks
-// K, slots ML, slot direction, chain waves NW.  False when the pass cannot
+// K, slots ML, slot direction, chain waves NW, ring entries per helper lane
+// IPL (the fewest that cover a chunk).  False when the pass cannot
 // run it (lines of one element, diagonals over 256 lines, or LDS too small):
 // then it runs k_tdma_pass.
 __host__ __device__ inline int odd_up(int x) { return x | 1; }
@@ -685,8 +695,9 @@ __host__ __device__ inline bool chain_plan(TdPass& p, int cmax) {
     if (NW > 4) return false;
     const long HL = TDC_BLOCK - 64 * NW;
     // ring (two slots of ML rows of odd_up(C) entries, d/r and a/m), c' and
-    // x slots (ML rows of odd_up(n)) and the padding fit the LDS
-    long C = ((long)TDC_LDS - 2 * TDC_PAD - 2L * odd_up(p.n) * ML) / (8L * ML) - 1;
+    // x slots (ML rows of odd_up(n)), the row of ones and the padding fit
+    // the LDS
+    long C = ((long)TDC_LDS - 2 * TDC_PAD - (2L * ML + 1) * odd_up(p.n)) / (8L * ML) - 1;
     if ((long)TDC_IPL * HL / ML < C) C = (long)TDC_IPL * HL / ML;
     if (C > p.n) C = p.n;
     if (C > cmax) C = cmax;
@@ -702,11 +713,12 @@ __host__ __device__ inline bool chain_plan(TdPass& p, int cmax) {
     p.ML = ML;
     p.by_u = p.nu <= p.nv;
     p.NW = NW;
+    p.IPL = (int)(((long)p.C * ML + HL - 1) / HL);
     return true;
 }
 
 __host__ __device__ inline long chain_lds_doubles(const TdPass& p) {
-    return 2L * TDC_PAD + 8L * odd_up(p.C) * p.ML + 2L * odd_up(p.n) * p.ML;
+    return 2L * TDC_PAD + 8L * odd_up(p.C) * p.ML + (2L * p.ML + 1) * odd_up(p.n);
 }
 
 // The two ring slots b = 0, 1 of each array follow each other: slot b of dr
@@ -716,7 +728,7 @@ struct TdLds {
     double2* dr;        // ring: {d(e), 1/m(e)} of chunk element t of line j at j*CS + t
     double2* am;        //       {a(e), m(e)}
     double* cp;         // c'(e) of line j at j*LS + e
-    double* xb;         // x slots: element e of slot s at s*LS + e
+    double* xb;         // x slots: element e of slot s at s*LS + e; slot -1 is the row of ones
     long ring;          // entries per slot: ML * CS
     int CS, LS;         // odd row lengths: odd_up(C), odd_up(n)
 };
@@ -729,8 +741,14 @@ __host__ __device__ inline TdLds chain_lds(const TdPass& p, double* base) {
     L.dr = reinterpret_cast<double2*>(base + TDC_PAD);
     L.am = L.dr + 2 * L.ring;
     L.cp = base + TDC_PAD + 8 * L.ring;
-    L.xb = L.cp + (long)p.ML * L.LS;
+    L.xb = L.cp + (long)(p.ML + 1) * L.LS;
     return L;
+}
+
+// Thread `tid` of `nt`: every x slot and the row of ones start as 1.0
+// (before the first barrier of the pass).
+__host__ __device__ inline void chain_init_slots(const TdPass& p, const TdLds& L, int tid, int nt) {
+    for (int q = tid; q < (p.ML + 1) * L.LS; q += nt) L.xb[q - L.LS] = 1.0;
 }
 
 // The phase (w, idx) of a pass: idx 0 is G, 1..K are F_0..F_{K-1}, K+1 is
@@ -834,11 +852,12 @@ __host__ __device__ inline void chain_load(const TdPass& p, const TdTask& t, int
     int tt, j;
     td_divmod(itc, t.m, t.mM, tt, j);
     const unsigned x0 = (unsigned)(td_mul24(t.e0 + tt, p.NL) + t.l0 + j);
+    const unsigned plane = (unsigned)(p.n * p.NL);
     typedef const TD_GLOBAL double2* cgd2ptr;
-    const cgd2ptr rp = reinterpret_cast<cgd2ptr>(p.rec) + x0 * (unsigned)(RC_REC / 2);
+    const cgd2ptr rp = reinterpret_cast<cgd2ptr>(p.rec) + x0;
 #pragma unroll
     for (int q = 0; q < RC_REC / 2; ++q) {
-        const double2 v2 = rp[q];
+        const double2 v2 = rp[q * plane];
         x.r[2 * q] = v2.x;
         x.r[2 * q + 1] = v2.y;
     }
@@ -849,39 +868,36 @@ __host__ __device__ inline void chain_load(const TdPass& p, const TdTask& t, int
 
 // d(e) of ring entry `it` of forward chunk t, in the reference's operand
 // order (rhs_elem): Su + o1 phi1 + o2 phi2 + o3 phi3 + o4 phi4, then the end
-// term.  phi1 and phi3 are the updated neighbours, read from the x slots
-// unless they are walls or wrap lines (then RC_O1 / RC_O3 already hold the
-// product); phi2 and phi4 have not changed since the pass began (RC_P2,
-// RC_P4).  With it, a(e), m(e) and 1/m(e) into ring slot b, and c'(e) into
-// the diagonal's c' array.  Branch-free: entries past the chunk redo its
-// last one (the same values to the same places), both slots are read and
-// the products chosen.
+// term (element 0 and n-1; -0.0 elsewhere, which leaves every d as it is).
+// phi1 and phi3, the updated neighbours, are read from the x slots s-1 and
+// s of the line's slot s (1.0 for a wall or wrap line, whose RC_O1 / RC_O3
+// hold the product); phi2 and phi4 have not changed since the pass began
+// (RC_P2, RC_P4).  With d, 1/m(e), a(e) and m(e) into ring slot b, and
+// c'(e) into the diagonal's c' array.  Entries past the chunk do nothing.
 template <int P>
 __host__ __device__ inline void chain_prepare(const TdPass& p, const TdLds& L, const TdTask& t, int b, int it,
                                               const TdPre& x) {
-    const int cnt = t.ce * t.m, itc = it < cnt ? it : cnt - 1;
+    if (it >= t.ce * t.m) return;
     int tt, j;
-    td_divmod(itc, t.m, t.mM, tt, j);
+    td_divmod(it, t.m, t.mM, tt, j);
     const int e = t.e0 + tt;
     const int v = t.v_lo + j, u = t.w + 2 - v;
-    const bool a_edge = P == 2 ? v == 1 : u == 1;     // phi1: (u-1,v), or (u,v-1) for the z lines
-    const bool b_edge = P == 2 ? u == 1 : v == 1;     // phi3: (u,v-1), or (u-1,v) for the z lines
-    const int sa = P == 2 ? chain_slot(p, u, v - 1) : chain_slot(p, u - 1, v);
-    const int sb = P == 2 ? chain_slot(p, u - 1, v) : chain_slot(p, u, v - 1);
-    const double* col = L.xb + e;
-    const double na = col[td_mul24(sa > 0 ? sa : 0, L.LS)], nb = col[td_mul24(sb > 0 ? sb : 0, L.LS)];
-    const double p1 = a_edge ? x.r[RC_O1] : x.r[RC_O1] * na;
-    const double p3 = b_edge ? x.r[RC_O3] : x.r[RC_O3] * nb;
-    double d = x.r[RC_SU] + p1;
+    const bool lo1 = p.by_u != (P == 2);                  // phi1 in slot s-1, phi3 in slot s (else the reverse)
+    const double* xr = L.xb + td_mul24(chain_slot(p, u, v), L.LS) - L.LS + e;
+    const double n1 = xr[lo1 ? 0 : L.LS], n3 = xr[lo1 ? L.LS : 0];
+    const bool first = e == 0, last = e == p.n - 1;
+    double d = x.r[RC_SU] + x.r[RC_O1] * n1;
     d = d + x.r[RC_P2];
-    d = d + p3;
+    d = d + x.r[RC_O3] * n3;
     d = d + x.r[RC_P4];
-    const double dq = d + x.r[RC_Q];
-    d = e == 0 || e == p.n - 1 ? dq : d;
+    d = d + (first ? x.r[RC_A] : last ? x.r[RC_C] : -0.0);
+    const double m = x.r[RC_M];
     const int at = td_mul24(b, (int)L.ring) + td_mul24(j, L.CS) + tt;
-    L.dr[at] = make_double2(d, x.r[RC_R]);
-    L.am[at] = make_double2(x.r[RC_A], x.r[RC_M]);
-    L.cp[td_mul24(j, L.LS) + e] = x.r[RC_C];
+    double* drp = reinterpret_cast<double*>(L.dr + at);
+    drp[0] = d;
+    drp[1] = td_recip(m);
+    L.am[at] = make_double2(first ? 0.0 : x.r[RC_A], m);
+    L.cp[td_mul24(j, L.LS) + e] = last ? 0.0 : x.r[RC_C];
 }
 
 // Entry `it` of chunk (e0, ce) of diagonal w's x (m lines from v_lo, first
@@ -1042,7 +1058,8 @@ __host__ __device__ inline void chain_backward(const TdPass& p, const TdLds& L, 
 // RC_P4 (the neighbours that keep their pass-start values: (u+1,v), (u,v+1),
 // walls, and the wrap's partner (1,v) / (u,1) read before the pass changes
 // it), RC_O1 / RC_O3 on lines whose updated neighbour is a wall or wrap line,
-// and for e = 0 and e = n-1 the line's end-term product RC_Q.
+// and for e = 0 and e = n-1 the line's end-term product, in the slot of a(0)
+// or c'(n-1).
 template <int P>
 __host__ __device__ inline void prep_elem(const TdPass& p, int l, int e) {
     const int uv = p.uv[l], u = uv & 0xffff, v = uv >> 16;
@@ -1052,18 +1069,17 @@ __host__ __device__ inline void prep_elem(const TdPass& p, int l, int e) {
     if (P == 2 && v == p.nv) nb[1] = line_index(p, u, 1);
     const long NL = p.NL, plane = (long)p.n * NL, x = (long)e * NL + l;
     cgdptr row = p.phi + (long)e * p.NLX;
-    gdptr r = p.rec + x * RC_REC;
     typedef TD_GLOBAL double2* gd2ptr;
     reinterpret_cast<gd2ptr>(p.pp)[x] =
         make_double2(p.pk[PK_O2 * plane + x] * row[nb[1]], p.pk[PK_O4 * plane + x] * row[nb[3]]);
-    if (P == 2 ? v == 1 : u == 1) r[RC_O1] = p.pk[PK_O1 * plane + x] * row[nb[0]];
-    if (P == 2 ? u == 1 : v == 1) r[RC_O3] = p.pk[PK_O3 * plane + x] * row[nb[2]];
+    if (P == 2 ? v == 1 : u == 1) *rec_at(p, RC_O1, x) = p.pk[PK_O1 * plane + x] * row[nb[0]];
+    if (P == 2 ? u == 1 : v == 1) *rec_at(p, RC_O3, x) = p.pk[PK_O3 * plane + x] * row[nb[2]];
     if (e == 0) {
         const double lo = P == 1 ? p.phi[(long)(p.n - 1) * p.NLX + l] : p.ln[LN_LO * NL + l];
-        r[RC_Q] = p.ln[LN_AMF * NL + l] * lo;
+        *rec_at(p, RC_A, x) = p.ln[LN_AMF * NL + l] * lo;           // the end term, in the slot of a(0)
     } else if (e == p.n - 1) {
         const double hi = P == 1 ? p.phi[l] : p.ln[LN_HI * NL + l];
-        r[RC_Q] = p.ln[LN_APL * NL + l] * hi;
+        *rec_at(p, RC_C, x) = p.ln[LN_APL * NL + l] * hi;           // the end term, in the slot of c'(n-1)
     }
 }
 

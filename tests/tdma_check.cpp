@@ -131,8 +131,10 @@ static void replay_pass(const TdSys& s, int order) {
 // substitution) run in either order (order 0: helpers first), which they
 // may since they touch different ring slots and different elements of the
 // x slots.  Every ring slot is filled with NaN before the helpers fill it,
-// every loaded record starts as NaN, and the x slots and c' entries start as
-// NaN, so a read of a value not written for it reaches phi.  Checks that
+// every loaded record starts as NaN and the c' entries start as NaN, so a
+// read of a value not written for it reaches phi; the x slots start as the
+// kernel starts them, 1.0, which only a wall or wrap neighbour may read
+// (any other read of it changes phi).  Checks that
 // the helpers fill exactly the ring entries of their chunk, that they
 // prepare every forward chunk once, in the phase before it, and that the
 // chain solves every line once.
@@ -143,7 +145,9 @@ static void replay_chain(const TdSys& s, int order) {
     const int NC = 64 * p.NW, HL = TDC_BLOCK - NC;
     CHECK(p.ML <= NC && p.NW <= 4, "pass %d: %d slots for %d chain lanes", P, p.ML, NC);
     CHECK(chain_lds_doubles(p) <= TDC_LDS, "pass %d: LDS %ld doubles", P, chain_lds_doubles(p));
-    CHECK((long)p.C * p.ML <= (long)TDC_IPL * HL, "pass %d: chunks of %d x %d for %d helper lanes", P, p.C, p.ML, HL);
+    CHECK(p.IPL >= 1 && p.IPL <= TDC_IPL && (long)p.C * p.ML <= (long)p.IPL * HL &&
+              (long)p.C * p.ML > (long)(p.IPL - 1) * HL,
+          "pass %d: chunks of %d x %d for %d helper lanes, %d entries each", P, p.C, p.ML, HL, p.IPL);
     CHECK((long)(p.K - 1) * p.C < p.n && (long)p.K * p.C >= p.n, "pass %d: chunks %d x %d for %d", P, p.K, p.C, p.n);
     const double qnan = std::nan("");
     // the kernel's LDS layout (chain_lds) in one exact-size block, all NaN
@@ -153,8 +157,13 @@ static void replay_chain(const TdSys& s, int order) {
     const TdLds L = chain_lds(p, lds);
     const long ring = L.ring;
     CHECK(L.CS % 2 == 1 && L.LS % 2 == 1 && L.CS >= p.C && L.LS >= p.n && ring == (long)p.ML * L.CS &&
-              L.xb + (long)p.ML * L.LS + TDC_PAD == lds + nlds && (double*)L.dr == lds + TDC_PAD,
+              L.xb + (long)p.ML * L.LS + TDC_PAD == lds + nlds && (double*)L.dr == lds + TDC_PAD &&
+              L.cp + (long)(p.ML + 1) * L.LS == L.xb,
           "pass %d: LDS layout", P);
+    // the x slots and the row of ones start as 1.0 (the kernel's first step),
+    // by threads in reverse order
+    for (int tid = TDC_BLOCK - 1; tid >= 0; --tid) chain_init_slots(p, L, tid, TDC_BLOCK);
+    for (long o = -L.LS; o < (long)p.ML * L.LS; ++o) CHECK(L.xb[o] == 1.0, "pass %d: x slot %ld not 1.0", P, o);
     double2* dr[2] = {L.dr, L.dr + ring};
     double2* am[2] = {L.am, L.am + ring};
     std::vector<double> carry((size_t)NC, 0.0);
@@ -171,7 +180,7 @@ static void replay_chain(const TdSys& s, int order) {
                 am[bn][o] = make_double2(qnan, qnan);
             }
             for (int h = HL - 1; h >= 0; --h)
-                for (int i = 0; i < TDC_IPL; ++i) {
+                for (int i = 0; i < p.IPL; ++i) {
                     TdPre x;
                     for (int q = 0; q < RC_N; ++q) x.r[q] = qnan;
                     chain_load<P>(p, f, h + i * HL, x);
@@ -186,9 +195,9 @@ static void replay_chain(const TdSys& s, int order) {
                                 f.pmM == td_magic((unsigned)f.pm)),
                       "pass %d: the iterator's diagonal before %d", P, f.w);
                 const int mp = real ? f.pm : 1;
-                CHECK(!real || (long)f.ce * mp <= (long)TDC_IPL * HL, "pass %d: copy of %d x %d", P, f.ce, mp);
+                CHECK(!real || (long)f.ce * mp <= (long)p.IPL * HL, "pass %d: copy of %d x %d", P, f.ce, mp);
                 for (int h = 0; h < HL; ++h)
-                    for (int i = 0; i < TDC_IPL; ++i)
+                    for (int i = 0; i < p.IPL; ++i)
                         chain_copy_out(p, L, real, f.w - 1, f.pv_lo, mp, real ? f.pmM : 0u, f.pl0, f.e0, f.ce,
                                        h + i * HL);
             }
