@@ -19,10 +19,11 @@
 // diagonal-major layout (tdma_line.hpp): for every element index the lines of
 // one diagonal are contiguous, so a work-group loads a whole diagonal
 // coalesced whatever the pass direction.  Kernels:
-//   k_tdma_pack    once per call, one thread per cell, reading the caller's
-//                  arrays in their own order: the right-hand-side
-//                  coefficients, a(e), aP and a_plus of every pass, and pass
-//                  0's phi, into the passes' layouts
+//   k_tdma_pack    once per call, one work-group per tile of 16^3 cells:
+//                  the right-hand-side coefficients, a(e), aP and a_plus of
+//                  every pass, and pass 0's phi, from the caller's arrays
+//                  (read in their own order) into the passes' layouts
+//                  (written in theirs), through LDS
 //   k_tdma_setup   once per call, one thread per line of each pass: the
 //                  pivots m(e) and c'(e) of the forward elimination (they
 //                  depend on the coefficients only), the chain kernel's
@@ -72,10 +73,25 @@ constexpr int TD_PASS_BLOCK = 1024;    // threads of a pass work-group (one per 
 
 using namespace tdma;
 
-__global__ __launch_bounds__(TD_BLOCK) void k_tdma_pack(const TdSys* __restrict__ sys) {
+// One tile of one system per work-group (pack_tile_*): each source array is
+// staged in one of two LDS buffers while the other one's values are stored,
+// one barrier per source.
+constexpr int TD_PACK_BLOCK = 512;
+__global__ __launch_bounds__(TD_PACK_BLOCK) void k_tdma_pack(const TdSys* __restrict__ sys) {
+    __shared__ double buf[2][TD_PT3];
+    __shared__ int order[3][TD_PT * TD_PT];
     const TdSys& s = sys[blockIdx.y];
-    const long c = (long)blockIdx.x * TD_BLOCK + threadIdx.x;
-    if (c < s.ncells) pack_cell(s, c);
+    PackTile t;
+    if (!pack_tile_of(s, blockIdx.x, t)) return;     // the whole work-group
+    const int tid = (int)threadIdx.x;
+    for (int P = 0; P < 3; ++P) pack_tile_order(P, t, order[P], tid, TD_PACK_BLOCK);
+    pack_tile_load(s, t, 0, buf[0], tid, TD_PACK_BLOCK);
+    __syncthreads();
+    for (int src = 0; src < TD_PSRC; ++src) {
+        if (src + 1 < TD_PSRC) pack_tile_load(s, t, src + 1, buf[(src + 1) & 1], tid, TD_PACK_BLOCK);
+        for (int P = 0; P < 3; ++P) pack_tile_store(s, P, t, src, buf[src & 1], order[P], tid, TD_PACK_BLOCK);
+        __syncthreads();
+    }
 }
 
 __global__ __launch_bounds__(TD_BLOCK) void k_tdma_setup(const TdSys* __restrict__ sys) {
@@ -276,7 +292,7 @@ int halo_lines(int P, int nu, int nv) { return P == 0 ? 2 * nu + nv : P == 1 ? 2
 
 int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* sys, int n_sweep, int on_device) {
     if (nsys < 1 || !descs || !sys || n_sweep < 0) return EAFP_ERR_ARG;
-    long dwork = 0, iwork = 0, stage = 0, max_lines = 0, max_cells = 0, max_cols = 0;
+    long dwork = 0, iwork = 0, stage = 0, max_lines = 0, max_cells = 0, max_cols = 0, max_tiles = 0;
     // A pass runs the chain kernel when every system admits it (chain_plan);
     // EAFP_TDMA_KERNEL=pass keeps k_tdma_pass for every pass.
     const char* force = std::getenv("EAFP_TDMA_KERNEL");
@@ -316,6 +332,8 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
         max_lines = std::max(max_lines, lines);
         max_cells = std::max(max_cells, ncells);
         max_cols = std::max(max_cols, (long)(d.nr + 4) * (d.nz + 4));
+        max_tiles = std::max(max_tiles, (long)((d.nr + TD_PT - 1) / TD_PT) * ((d.ntheta + TD_PT - 1) / TD_PT) *
+                                            ((d.nz + TD_PT - 1) / TD_PT));
     }
     if (n_sweep == 0) return EAFP_OK;
     std::lock_guard<std::mutex> lock(g_td_mutex);
@@ -405,7 +423,7 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
     const dim3 g_cells((unsigned)((max_cells + TD_BLOCK - 1) / TD_BLOCK), ny);
     const dim3 g_cols((unsigned)((max_cols + TD_BLOCK - 1) / TD_BLOCK), ny);
     const TdSys* tp = w.table.p;
-    k_tdma_pack<<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
+    k_tdma_pack<<<dim3((unsigned)max_tiles, ny), TD_PACK_BLOCK, 0, w.stream>>>(tp);
     k_tdma_setup<<<g_lines, TD_BLOCK, 0, w.stream>>>(tp);
     for (int sweep = 0; sweep < n_sweep; ++sweep) {
         // with the chain kernel planned, k_tdma_pass still runs the systems whose

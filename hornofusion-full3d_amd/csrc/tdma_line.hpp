@@ -373,35 +373,112 @@ __host__ __device__ inline void fill_wall_lines(const TdSys& s, int P, int u, in
     }
 }
 
-// The caller's coefficients of interior cell c (r fastest), packed into
-// every pass's element-major planes (k_tdma_pack: coalesced reads in the
-// caller's order): Su, the four off-line coefficients, a(e) = -a_minus(e),
-// and, until setup_line replaces them with the pivots, aP in the m plane
-// and a_plus in the c' plane; for pass 0 also the cell's initial phi.
-__host__ __device__ inline void pack_cell(const TdSys& s, long c) {
-    const int i = (int)(c % s.nr) + 1;
-    const long rest = c / s.nr;
-    const int j = (int)(rest % s.nth) + 1, k = (int)(rest / s.nth) + 1;
-    const long cc = tat(s, i, j, k);
-    for (int P = 0; P < 3; ++P) {
-        const TdPass& p = s.ps[P];
-        int u, v, e;
-        line_of_cell(P, i, j, k, u, v, e);
-        int am, ap, o[4];
-        pass_slots(P, am, ap, o);
-        const long plane = (long)p.n * p.NL;
-        gdptr x = p.pk + (long)e * p.NL + line_index(p, u, v);
-        x[PK_SU * plane] = s.c[CSU][cc];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) x[(PK_O1 + q) * plane] = s.c[o[q]][cc];
-        x[PK_A * plane] = -s.c[am][cc];
-        x[PK_M * plane] = s.c[CP_][cc];
-        x[PK_C * plane] = s.c[ap][cc];
-        if (P == 0) p.phi[(long)e * p.NLX + line_index(p, u, v)] = s.home[cc];
+// k_tdma_pack packs the caller's coefficients into every pass's
+// element-major planes: Su, the four off-line coefficients, a(e) =
+// -a_minus(e), and, until setup_line replaces them with the pivots, aP in
+// the m plane and a_plus in the c' plane; for pass 0 also the initial phi.
+// It works in tiles of TD_PT^3 interior cells, one work-group each.  A
+// source array's values over the tile are staged in LDS with rows of TD_PT
+// consecutive r-cells read coalesced, then stored to every pass's plane in
+// that pass's order: for each element, the tile's lines diagonal by
+// diagonal, so consecutive threads write consecutive line indices (a tile
+// holds up to TD_PT lines of each diagonal, contiguous in the plane).
+constexpr int TD_PT = 16;
+constexpr int TD_PT3 = TD_PT * TD_PT * TD_PT;
+constexpr int TD_PSRC = TD_NCOEF + 1;      // the coefficients, then phi (pass 0 only)
+
+struct PackTile {
+    int i0, j0, k0;      // first cell (0-based)
+    int ni, nj, nk;      // extents, at most TD_PT
+};
+
+// Tile b of system s; false past the system's last tile.
+__host__ __device__ inline bool pack_tile_of(const TdSys& s, long b, PackTile& t) {
+    const int ti = (s.nr + TD_PT - 1) / TD_PT, tj = (s.nth + TD_PT - 1) / TD_PT, tk = (s.nz + TD_PT - 1) / TD_PT;
+    if (b >= (long)ti * tj * tk) return false;
+    const int bi = (int)(b % ti), bj = (int)((b / ti) % tj), bk = (int)(b / ((long)ti * tj));
+    t.i0 = bi * TD_PT;
+    t.j0 = bj * TD_PT;
+    t.k0 = bk * TD_PT;
+    t.ni = s.nr - t.i0 < TD_PT ? s.nr - t.i0 : TD_PT;
+    t.nj = s.nth - t.j0 < TD_PT ? s.nth - t.j0 : TD_PT;
+    t.nk = s.nz - t.k0 < TD_PT ? s.nz - t.k0 : TD_PT;
+    return true;
+}
+
+// The tile's extents in pass P's terms: lines (u, v) and elements.
+__host__ __device__ inline void pack_tile_extents(int P, const PackTile& t, int& nu, int& nv, int& ne) {
+    if (P == 0) { nu = t.nj; nv = t.nk; ne = t.ni; }
+    else if (P == 1) { nu = t.ni; nv = t.nk; ne = t.nj; }
+    else { nu = t.ni; nv = t.nj; ne = t.nk; }
+}
+
+// Thread tid of nt: pass P's table of the tile's lines in diagonal order,
+// entry r = (lu | lv << 8) for the r-th line (lu, lv tile-local).
+__host__ __device__ inline void pack_tile_order(int P, const PackTile& t, int* order, int tid, int nt) {
+    int nu, nv, ne;
+    pack_tile_extents(P, t, nu, nv, ne);
+    for (int q = tid; q < nu * nv; q += nt) {
+        const int lu = q % nu, lv = q / nu, d = lu + lv;
+        int r = 0;
+        for (int dd = 0; dd < d; ++dd) {
+            const int a = dd - (nu - 1) > 0 ? dd - (nu - 1) : 0, b = dd < nv - 1 ? dd : nv - 1;
+            r += b - a + 1;
+        }
+        r += lv - (d - (nu - 1) > 0 ? d - (nu - 1) : 0);
+        order[r] = lu | lv << 8;
     }
 }
 
-// Everything else per call for line (u,v) of pass P, after pack_cell: the
+// Thread tid of nt: source array `src` (a coefficient, or phi) over the tile
+// into buf, cell (ci, cj, ck) at (ck * TD_PT + cj) * TD_PT + ci.
+__host__ __device__ inline void pack_tile_load(const TdSys& s, const PackTile& t, int src, double* buf, int tid,
+                                               int nt) {
+    cgdptr a = src < TD_NCOEF ? s.c[src] : s.home;
+    for (int q = tid; q < TD_PT3; q += nt) {
+        const int ci = q % TD_PT, cj = (q / TD_PT) % TD_PT, ck = q / (TD_PT * TD_PT);
+        if (ci < t.ni && cj < t.nj && ck < t.nk) buf[q] = a[tat(s, t.i0 + ci + 1, t.j0 + cj + 1, t.k0 + ck + 1)];
+    }
+}
+
+// Where source `src` goes in pass P: the plane (PK_*), negated for a(e) =
+// -a_minus(e); -1 for phi outside pass 0.  Pass 0's phi goes to its phi rows.
+__host__ __device__ inline int pack_plane(int P, int src, bool& neg) {
+    int am, ap, o[4];
+    pass_slots(P, am, ap, o);
+    neg = src == am;
+    if (src == CSU) return PK_SU;
+    if (src == CP_) return PK_M;
+    if (src == am) return PK_A;
+    if (src == ap) return PK_C;
+    for (int q = 0; q < 4; ++q)
+        if (src == o[q]) return PK_O1 + q;
+    return -1;
+}
+
+// Thread tid of nt: the staged source `src` into pass P's plane (or, for
+// phi, pass 0's phi rows), the tile's lines in diagonal order per element.
+__host__ __device__ inline void pack_tile_store(const TdSys& s, int P, const PackTile& t, int src, const double* buf,
+                                                const int* order, int tid, int nt) {
+    const TdPass& p = s.ps[P];
+    bool neg = false;
+    const int pl = src < TD_NCOEF ? pack_plane(P, src, neg) : -1;
+    if (src < TD_NCOEF ? pl < 0 : P != 0) return;
+    int nu, nv, ne;
+    pack_tile_extents(P, t, nu, nv, ne);
+    const int R = nu * nv;
+    const int u0 = P == 0 ? t.j0 : t.i0, v0 = P == 2 ? t.j0 : t.k0, e0 = P == 0 ? t.i0 : P == 1 ? t.j0 : t.k0;
+    for (int q = tid; q < ne * R; q += nt) {
+        const int le = q / R, o = order[q - le * R], lu = o & 255, lv = o >> 8;
+        const int ci = P == 0 ? le : lu, cj = P == 0 ? lu : P == 1 ? le : lv, ck = P == 2 ? le : lv;
+        const double x = buf[(ck * TD_PT + cj) * TD_PT + ci];
+        const long l = line_index(p, u0 + lu + 1, v0 + lv + 1), e = e0 + le;
+        if (src < TD_NCOEF) p.pk[(long)pl * p.n * p.NL + e * p.NL + l] = neg ? -x : x;
+        else p.phi[e * p.NLX + l] = x;
+    }
+}
+
+// Everything else per call for line (u,v) of pass P, after the pack: the
 // pivots m(e) and c'(e) of the reference's forward elimination (they do not
 // depend on phi) in place of aP and a_plus, the chain kernel's records, the
 // end coefficients and wall values, and the wall halo lines.  The packed

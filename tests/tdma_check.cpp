@@ -3,7 +3,7 @@
 // Runs the library's own per-line, per-element and per-cell code
 // (csrc/tdma_line.hpp, the functions the kernels call) on the CPU in the
 // order the kernels impose:
-//   - k_tdma_pack: every interior cell; k_tdma_setup: every line of every
+//   - k_tdma_pack: every tile; k_tdma_setup: every line of every
 //     pass (thread = line index);
 //   - per sweep and pass, k_tdma_prep and k_tdma_chain phase by phase
 //     (replay_chain, below) where chain_plan admits the pass (with chunks
@@ -23,6 +23,7 @@
 // and, built with AddressSanitizer on the host, that no access leaves its
 // exact-size array.  Prints "ok <cases>" and exits 0, or the first failure
 // and exits 1.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -338,8 +339,43 @@ static void run_case(int nr, int nth, int nz, int n_sweep, unsigned seed, bool z
         }
         lines += p.NL;
     }
-    // k_tdma_pack: every interior cell (threads in reverse)
-    for (long cc = ncells - 1; cc >= 0; --cc) pack_cell(s, cc);
+    // k_tdma_pack: every tile (work-groups and threads in reverse), the LDS
+    // buffer NaN-filled before each source is staged
+    {
+        std::vector<double> buf(TD_PT3);
+        std::vector<int> order(3 * TD_PT * TD_PT);
+        const int NT = 512;
+        PackTile t;
+        long tiles = 0;
+        while (pack_tile_of(s, tiles, t)) ++tiles;
+        CHECK(tiles == (long)((nr + TD_PT - 1) / TD_PT) * ((nth + TD_PT - 1) / TD_PT) * ((nz + TD_PT - 1) / TD_PT),
+              "pack: %ld tiles", tiles);
+        for (long b = tiles - 1; b >= 0; --b) {
+            CHECK(pack_tile_of(s, b, t), "pack: tile %ld", b);
+            std::fill(order.begin(), order.end(), -1);
+            for (int P = 0; P < 3; ++P) {
+                int nu, nv, ne;
+                pack_tile_extents(P, t, nu, nv, ne);
+                for (int tid = NT - 1; tid >= 0; --tid) pack_tile_order(P, t, order.data() + P * TD_PT * TD_PT, tid, NT);
+                std::vector<int> seen((size_t)nu * nv, 0);
+                for (int r = 0; r < nu * nv; ++r) {
+                    const int o = order[(size_t)P * TD_PT * TD_PT + r], lu = o & 255, lv = o >> 8;
+                    CHECK(o >= 0 && lu < nu && lv < nv && ++seen[(size_t)lv * nu + lu] == 1, "pack: order %d of pass %d", r, P);
+                    if (r > 0) {   // diagonal by diagonal, v ascending within a diagonal
+                        const int p = order[(size_t)P * TD_PT * TD_PT + r - 1], pu = p & 255, pv = p >> 8;
+                        CHECK(pu + pv < lu + lv || (pu + pv == lu + lv && pv < lv), "pack: order of pass %d at %d", P, r);
+                    }
+                }
+            }
+            for (int src = 0; src < TD_PSRC; ++src) {
+                std::fill(buf.begin(), buf.end(), std::nan(""));
+                for (int tid = NT - 1; tid >= 0; --tid) pack_tile_load(s, t, src, buf.data(), tid, NT);
+                for (int P = 0; P < 3; ++P)
+                    for (int tid = NT - 1; tid >= 0; --tid)
+                        pack_tile_store(s, P, t, src, buf.data(), order.data() + P * TD_PT * TD_PT, tid, NT);
+            }
+        }
+    }
     // k_tdma_setup: thread t = line index within the passes, past the end included
     std::vector<int> setup_seen((size_t)lines, 0);
     for (long t0 = lines + 40; t0 >= 0; --t0) {
