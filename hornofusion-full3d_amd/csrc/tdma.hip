@@ -79,17 +79,17 @@ using namespace tdma;
 constexpr int TD_PACK_BLOCK = 512;
 __global__ __launch_bounds__(TD_PACK_BLOCK) void k_tdma_pack(const TdSys* __restrict__ sys) {
     __shared__ double buf[2][TD_PT3];
-    __shared__ int order[3][TD_PT * TD_PT];
+    __shared__ int order[3][TD_PT * TD_PT], lidx[3][TD_PT * TD_PT];
     const TdSys& s = sys[blockIdx.y];
     PackTile t;
     if (!pack_tile_of(s, blockIdx.x, t)) return;     // the whole work-group
     const int tid = (int)threadIdx.x;
-    for (int P = 0; P < 3; ++P) pack_tile_order(P, t, order[P], tid, TD_PACK_BLOCK);
+    for (int P = 0; P < 3; ++P) pack_tile_order(s, P, t, order[P], lidx[P], tid, TD_PACK_BLOCK);
     pack_tile_load(s, t, 0, buf[0], tid, TD_PACK_BLOCK);
     __syncthreads();
     for (int src = 0; src < TD_PSRC; ++src) {
         if (src + 1 < TD_PSRC) pack_tile_load(s, t, src + 1, buf[(src + 1) & 1], tid, TD_PACK_BLOCK);
-        for (int P = 0; P < 3; ++P) pack_tile_store(s, P, t, src, buf[src & 1], order[P], tid, TD_PACK_BLOCK);
+        for (int P = 0; P < 3; ++P) pack_tile_store(s, P, t, src, buf[src & 1], order[P], lidx[P], tid, TD_PACK_BLOCK);
         __syncthreads();
     }
 }

@@ -413,11 +413,14 @@ __host__ __device__ inline void pack_tile_extents(int P, const PackTile& t, int&
     else { nu = t.ni; nv = t.nj; ne = t.nk; }
 }
 
-// Thread tid of nt: pass P's table of the tile's lines in diagonal order,
-// entry r = (lu | lv << 8) for the r-th line (lu, lv tile-local).
-__host__ __device__ inline void pack_tile_order(int P, const PackTile& t, int* order, int tid, int nt) {
+// Thread tid of nt: pass P's table of the tile's lines in diagonal order:
+// for the r-th line (lu, lv tile-local), order[r] = lu | lv << 8 and
+// lidx[r] its line index in the pass.
+__host__ __device__ inline void pack_tile_order(const TdSys& s, int P, const PackTile& t, int* order, int* lidx,
+                                                int tid, int nt) {
     int nu, nv, ne;
     pack_tile_extents(P, t, nu, nv, ne);
+    const int u0 = P == 0 ? t.j0 : t.i0, v0 = P == 2 ? t.j0 : t.k0;
     for (int q = tid; q < nu * nv; q += nt) {
         const int lu = q % nu, lv = q / nu, d = lu + lv;
         int r = 0;
@@ -427,6 +430,7 @@ __host__ __device__ inline void pack_tile_order(int P, const PackTile& t, int* o
         }
         r += lv - (d - (nu - 1) > 0 ? d - (nu - 1) : 0);
         order[r] = lu | lv << 8;
+        lidx[r] = line_index(s.ps[P], u0 + lu + 1, v0 + lv + 1);
     }
 }
 
@@ -459,7 +463,7 @@ __host__ __device__ inline int pack_plane(int P, int src, bool& neg) {
 // Thread tid of nt: the staged source `src` into pass P's plane (or, for
 // phi, pass 0's phi rows), the tile's lines in diagonal order per element.
 __host__ __device__ inline void pack_tile_store(const TdSys& s, int P, const PackTile& t, int src, const double* buf,
-                                                const int* order, int tid, int nt) {
+                                                const int* order, const int* lidx, int tid, int nt) {
     const TdPass& p = s.ps[P];
     bool neg = false;
     const int pl = src < TD_NCOEF ? pack_plane(P, src, neg) : -1;
@@ -467,16 +471,18 @@ __host__ __device__ inline void pack_tile_store(const TdSys& s, int P, const Pac
     int nu, nv, ne;
     pack_tile_extents(P, t, nu, nv, ne);
     const int R = nu * nv;
-    const int u0 = P == 0 ? t.j0 : t.i0, v0 = P == 2 ? t.j0 : t.k0, e0 = P == 0 ? t.i0 : P == 1 ? t.j0 : t.k0;
+    const int e0 = P == 0 ? t.i0 : P == 1 ? t.j0 : t.k0;
     for (int q = tid; q < ne * R; q += nt) {
-        const int le = q / R, o = order[q - le * R], lu = o & 255, lv = o >> 8;
+        const int le = q / R, r = q - le * R, o = order[r], lu = o & 255, lv = o >> 8;
         const int ci = P == 0 ? le : lu, cj = P == 0 ? lu : P == 1 ? le : lv, ck = P == 2 ? le : lv;
         const double x = buf[(ck * TD_PT + cj) * TD_PT + ci];
-        const long l = line_index(p, u0 + lu + 1, v0 + lv + 1), e = e0 + le;
+        const long l = lidx[r], e = e0 + le;
         if (src < TD_NCOEF) p.pk[(long)pl * p.n * p.NL + e * p.NL + l] = neg ? -x : x;
         else p.phi[e * p.NLX + l] = x;
     }
 }
+
+constexpr int TD_SB = 8;   // setup_line's elements per block of loads
 
 // Everything else per call for line (u,v) of pass P, after the pack: the
 // pivots m(e) and c'(e) of the reference's forward elimination (they do not
@@ -491,25 +497,48 @@ __host__ __device__ inline void setup_line(const TdSys& s, int P, int u, int v) 
     int am, ap, o[4];
     pass_slots(P, am, ap, o);
     double cp = 0.0, a_first = 0.0, ap_last = 0.0;
-    for (int e = 0; e < p.n; ++e) {
-        gdptr x = p.pk + (long)e * NL + l;
-        const double a = x[PK_A * plane], aP = x[PK_M * plane], aplus = x[PK_C * plane];
-        if (e == 0) a_first = a;
-        if (e == p.n - 1) ap_last = aplus;
-        double m;
-        if (e == 0) m = clamp_pivot(aP);
-        else m = clamp_pivot(aP - a * cp);
-        cp = -aplus / m;
-        x[PK_M * plane] = m;
-        x[PK_C * plane] = e == p.n - 1 ? 0.0 : cp;    // c'(n-1) takes no part in the recurrences
-        if (p.rec) {
-            typedef TD_GLOBAL double2* gd2ptr;
-            const long xr = (long)e * NL + l;
-            // a(0) and c'(n-1) take no part in the recurrences: their slots carry the end terms
-            *reinterpret_cast<gd2ptr>(rec_at(p, RC_SU, xr)) = make_double2(x[PK_SU * plane], x[PK_O1 * plane]);
-            *reinterpret_cast<gd2ptr>(rec_at(p, RC_O3, xr)) = make_double2(x[PK_O3 * plane], e == 0 ? 0.0 : a);
-            *reinterpret_cast<gd2ptr>(rec_at(p, RC_M, xr)) = make_double2(m, e == p.n - 1 ? 0.0 : cp);
-            if (td_bexp(m) - TD_M_LO > TD_M_SPAN) s.off_band[P] = 1;
+    // blocks of TD_SB elements: the block's inputs are loaded before its
+    // stores (each element's m and c' replace its own aP and a_plus), so one
+    // memory latency covers TD_SB steps of the recurrence; with them the
+    // chain kernel's records
+    for (int e0 = 0; e0 < p.n; e0 += TD_SB) {
+        double A[TD_SB], M[TD_SB], Cp[TD_SB], Su[TD_SB], O1[TD_SB], O3[TD_SB];
+#pragma unroll
+        for (int q = 0; q < TD_SB; ++q) {
+            cgdptr x = p.pk + (long)(e0 + q < p.n ? e0 + q : p.n - 1) * NL + l;
+            A[q] = x[PK_A * plane];
+            M[q] = x[PK_M * plane];
+            Cp[q] = x[PK_C * plane];
+            if (p.rec) {
+                Su[q] = x[PK_SU * plane];
+                O1[q] = x[PK_O1 * plane];
+                O3[q] = x[PK_O3 * plane];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TD_SB; ++q) {
+            const int e = e0 + q;
+            if (e >= p.n) break;
+            if (e == 0) a_first = A[q];
+            if (e == p.n - 1) ap_last = Cp[q];
+            double m;
+            if (e == 0) m = clamp_pivot(M[q]);
+            else m = clamp_pivot(M[q] - A[q] * cp);
+            cp = -Cp[q] / m;
+            const double c = e == p.n - 1 ? 0.0 : cp;      // c'(n-1) takes no part in the recurrences
+            const long xe = (long)e * NL + l;
+            gdptr x = p.pk + xe;
+            x[PK_M * plane] = m;
+            x[PK_C * plane] = c;
+            if (p.rec) {
+                // the chain kernel's record; a(0) and c'(n-1) take no part in
+                // the recurrences: their slots carry the end terms (k_tdma_prep)
+                typedef TD_GLOBAL double2* gd2ptr;
+                *reinterpret_cast<gd2ptr>(rec_at(p, RC_SU, xe)) = make_double2(Su[q], O1[q]);
+                *reinterpret_cast<gd2ptr>(rec_at(p, RC_O3, xe)) = make_double2(O3[q], e == 0 ? 0.0 : A[q]);
+                *reinterpret_cast<gd2ptr>(rec_at(p, RC_M, xe)) = make_double2(m, c);
+                if (td_bexp(m) - TD_M_LO > TD_M_SPAN) s.off_band[P] = 1;
+            }
         }
     }
     if (p.uv) p.uv[l] = u | v << 16;
@@ -528,6 +557,7 @@ __host__ __device__ inline void setup_line(const TdSys& s, int P, int u, int v) 
     }
     fill_wall_lines(s, P, u, v);
 }
+
 
 // d(e) of line l of pass P: Su + o1 phi1 + o2 phi2, then + o3 phi3 + o4 phi4
 // (the reference's operand order in all three passes), then the end terms

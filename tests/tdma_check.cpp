@@ -343,7 +343,7 @@ static void run_case(int nr, int nth, int nz, int n_sweep, unsigned seed, bool z
     // buffer NaN-filled before each source is staged
     {
         std::vector<double> buf(TD_PT3);
-        std::vector<int> order(3 * TD_PT * TD_PT);
+        std::vector<int> order(3 * TD_PT * TD_PT), lidx(3 * TD_PT * TD_PT);
         const int NT = 512;
         PackTile t;
         long tiles = 0;
@@ -353,14 +353,19 @@ static void run_case(int nr, int nth, int nz, int n_sweep, unsigned seed, bool z
         for (long b = tiles - 1; b >= 0; --b) {
             CHECK(pack_tile_of(s, b, t), "pack: tile %ld", b);
             std::fill(order.begin(), order.end(), -1);
+            std::fill(lidx.begin(), lidx.end(), -1);
             for (int P = 0; P < 3; ++P) {
                 int nu, nv, ne;
                 pack_tile_extents(P, t, nu, nv, ne);
-                for (int tid = NT - 1; tid >= 0; --tid) pack_tile_order(P, t, order.data() + P * TD_PT * TD_PT, tid, NT);
+                for (int tid = NT - 1; tid >= 0; --tid)
+                    pack_tile_order(s, P, t, order.data() + P * TD_PT * TD_PT, lidx.data() + P * TD_PT * TD_PT, tid, NT);
                 std::vector<int> seen((size_t)nu * nv, 0);
                 for (int r = 0; r < nu * nv; ++r) {
                     const int o = order[(size_t)P * TD_PT * TD_PT + r], lu = o & 255, lv = o >> 8;
                     CHECK(o >= 0 && lu < nu && lv < nv && ++seen[(size_t)lv * nu + lu] == 1, "pack: order %d of pass %d", r, P);
+                    const int u0 = P == 0 ? t.j0 : t.i0, v0 = P == 2 ? t.j0 : t.k0;
+                    CHECK(lidx[(size_t)P * TD_PT * TD_PT + r] == line_index(s.ps[P], u0 + lu + 1, v0 + lv + 1),
+                          "pack: line index %d of pass %d", r, P);
                     if (r > 0) {   // diagonal by diagonal, v ascending within a diagonal
                         const int p = order[(size_t)P * TD_PT * TD_PT + r - 1], pu = p & 255, pv = p >> 8;
                         CHECK(pu + pv < lu + lv || (pu + pv == lu + lv && pv < lv), "pack: order of pass %d at %d", P, r);
@@ -372,7 +377,8 @@ static void run_case(int nr, int nth, int nz, int n_sweep, unsigned seed, bool z
                 for (int tid = NT - 1; tid >= 0; --tid) pack_tile_load(s, t, src, buf.data(), tid, NT);
                 for (int P = 0; P < 3; ++P)
                     for (int tid = NT - 1; tid >= 0; --tid)
-                        pack_tile_store(s, P, t, src, buf.data(), order.data() + P * TD_PT * TD_PT, tid, NT);
+                        pack_tile_store(s, P, t, src, buf.data(), order.data() + P * TD_PT * TD_PT,
+                                        lidx.data() + P * TD_PT * TD_PT, tid, NT);
             }
         }
     }
