@@ -111,6 +111,25 @@ def test_tdma_batch_ragged():
         assert_bitwise(sys_[q]["phi"], wants[q], f"system {q} {shapes[q]}")
 
 
+def test_tdma_batch_ragged_chain_entries():
+    """Every pass of every system runs the chain kernel, whose helper lanes
+    take as many ring entries per chunk as the batch's largest plan needs
+    (1 to 3 here), and whose pack tiles are cut at every mesh edge."""
+    shapes = [(12, 24, 16), (60, 30, 20), (17, 40, 8), (33, 20, 25), (2, 70, 3)]
+    sys_, meshes, wants = [], [], []
+    for q, shape in enumerate(shapes):
+        coef, phi0 = random_system(shape, 300 + q)
+        wants.append(oracle.tdma_3d(coef, phi0, shape, 2))
+        d = dict(coef)
+        d["phi"] = phi0.copy(order="F")
+        sys_.append(d)
+        meshes.append(eafp.Mesh(*shape))
+    print(f"case tdma chain batch of {len(shapes)}", flush=True)
+    eafp.tdma_batch(sys_, meshes, 2)
+    for q in range(len(shapes)):
+        assert_bitwise(sys_[q]["phi"], wants[q], f"system {q} {shapes[q]}")
+
+
 def test_tdma_device_resident_and_repeat():
     """arrays_on_device = 1; two calls in a row continue from the first one's phi."""
     shape = (20, 36, 28)
