@@ -44,7 +44,8 @@
 //                  coalesced; barriers between the three steps.  A diagonal too long for the LDS is taken in
 //                  groups of lines.
 //   k_tdma_move    moves phi between the passes' layouts (and into the
-//                  caller's array at the end), all CUs
+//                  caller's array at the end), a tile of 16^3 cells per
+//                  work-group through LDS, as k_tdma_pack
 //   k_tdma_wall_halo  the theta halo planes of the caller's array on the r/z
 //                  wall columns
 // A single system is latency-bound by design: one CU walks the
@@ -255,11 +256,21 @@ void launch_chain(int ipl, int nsys, hipStream_t st, const TdSys* tp) {
     else k_tdma_chain<P, 3><<<nsys, TDC_BLOCK, 0, st>>>(tp);
 }
 
+// One tile of one system per work-group (move_tile_*).
 template <int P, int Q>
-__global__ __launch_bounds__(TD_BLOCK) void k_tdma_move(const TdSys* __restrict__ sys) {
+__global__ __launch_bounds__(TD_PACK_BLOCK) void k_tdma_move(const TdSys* __restrict__ sys) {
+    __shared__ double buf[TD_PT3];
+    __shared__ int order[2][TD_PT * TD_PT], lidx[2][TD_PT * TD_PT];
     const TdSys& s = sys[blockIdx.y];
-    const long c = (long)blockIdx.x * TD_BLOCK + threadIdx.x;
-    if (c < s.ncells) move_cell(s, P, Q, c);
+    PackTile t;
+    if (!pack_tile_of(s, blockIdx.x, t)) return;     // the whole work-group
+    const int tid = (int)threadIdx.x;
+    pack_tile_order(s, P, t, order[0], lidx[0], tid, TD_PACK_BLOCK);
+    if (Q < 3) pack_tile_order(s, Q, t, order[1], lidx[1], tid, TD_PACK_BLOCK);
+    __syncthreads();
+    move_tile_gather(s, P, t, buf, order[0], lidx[0], tid, TD_PACK_BLOCK);
+    __syncthreads();
+    move_tile_scatter(s, P, Q, t, buf, order[1], lidx[1], tid, TD_PACK_BLOCK);
 }
 
 __global__ __launch_bounds__(TD_BLOCK) void k_tdma_wall_halo(const TdSys* __restrict__ sys) {
@@ -423,7 +434,8 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
     const dim3 g_cells((unsigned)((max_cells + TD_BLOCK - 1) / TD_BLOCK), ny);
     const dim3 g_cols((unsigned)((max_cols + TD_BLOCK - 1) / TD_BLOCK), ny);
     const TdSys* tp = w.table.p;
-    k_tdma_pack<<<dim3((unsigned)max_tiles, ny), TD_PACK_BLOCK, 0, w.stream>>>(tp);
+    const dim3 g_tiles((unsigned)max_tiles, ny);
+    k_tdma_pack<<<g_tiles, TD_PACK_BLOCK, 0, w.stream>>>(tp);
     k_tdma_setup<<<g_lines, TD_BLOCK, 0, w.stream>>>(tp);
     for (int sweep = 0; sweep < n_sweep; ++sweep) {
         // with the chain kernel planned, k_tdma_pass still runs the systems whose
@@ -433,20 +445,20 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
             launch_chain<0>(ipl[0], nsys, w.stream, tp);
         }
         k_tdma_pass<0><<<nsys, TD_PASS_BLOCK, 0, w.stream>>>(tp);
-        k_tdma_move<0, 1><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
+        k_tdma_move<0, 1><<<g_tiles, TD_PACK_BLOCK, 0, w.stream>>>(tp);
         if (chain[1]) {
             k_tdma_prep<1><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
             launch_chain<1>(ipl[1], nsys, w.stream, tp);
         }
         k_tdma_pass<1><<<nsys, TD_PASS_BLOCK, 0, w.stream>>>(tp);
-        k_tdma_move<1, 2><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
+        k_tdma_move<1, 2><<<g_tiles, TD_PACK_BLOCK, 0, w.stream>>>(tp);
         if (chain[2]) {
             k_tdma_prep<2><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
             launch_chain<2>(ipl[2], nsys, w.stream, tp);
         }
         k_tdma_pass<2><<<nsys, TD_PASS_BLOCK, 0, w.stream>>>(tp);
-        if (sweep + 1 < n_sweep) k_tdma_move<2, 0><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
-        else k_tdma_move<2, 3><<<g_cells, TD_BLOCK, 0, w.stream>>>(tp);
+        if (sweep + 1 < n_sweep) k_tdma_move<2, 0><<<g_tiles, TD_PACK_BLOCK, 0, w.stream>>>(tp);
+        else k_tdma_move<2, 3><<<g_tiles, TD_PACK_BLOCK, 0, w.stream>>>(tp);
     }
     k_tdma_wall_halo<<<g_cols, TD_BLOCK, 0, w.stream>>>(tp);
     EAFP_TRY(hipGetLastError());

@@ -696,30 +696,55 @@ __host__ __device__ inline int theta_src(int j, int nth) {
     return q + 1;
 }
 
-// Cell c (interior, r fastest) moved from pass P's phi to pass Q's; Q = 3
-// is the caller's padded array.  Moving pass 1's values to pass 2 also
-// writes the caller's theta halo planes of the cell's column (the copy the
+// k_tdma_move moves phi between the passes' layouts (Q = 3: into the
+// caller's padded array) a tile of pack_tile_of at a time, through LDS:
+// gathered in pass P's diagonal order, stored in pass Q's (or, for the
+// caller's array, in its own order).  Moving pass 1's values to pass 2 also
+// writes the caller's theta halo planes of the tile's columns (the copy the
 // reference makes before the z lines, the last copy of a call).
-__host__ __device__ inline void move_cell(const TdSys& s, int P, int Q, long c) {
-    const int i = (int)(c % s.nr) + 1;
-    const long rest = c / s.nr;
-    const int j = (int)(rest % s.nth) + 1, k = (int)(rest / s.nth) + 1;
-    int u, v, e;
-    line_of_cell(P, i, j, k, u, v, e);
-    const TdPass& a = s.ps[P];
-    const double val = a.phi[(long)e * a.NLX + line_index(a, u, v)];
-    if (Q == 3) {
-        s.home[tat(s, i, j, k)] = val;
-        return;
+//
+// Thread tid of nt: pass P's phi over the tile into buf (cell order).
+__host__ __device__ inline void move_tile_gather(const TdSys& s, int P, const PackTile& t, double* buf,
+                                                 const int* order, const int* lidx, int tid, int nt) {
+    const TdPass& p = s.ps[P];
+    int nu, nv, ne;
+    pack_tile_extents(P, t, nu, nv, ne);
+    const int R = nu * nv, e0 = P == 0 ? t.i0 : P == 1 ? t.j0 : t.k0;
+    for (int q = tid; q < ne * R; q += nt) {
+        const int le = q / R, r = q - le * R, o = order[r], lu = o & 255, lv = o >> 8;
+        const int ci = P == 0 ? le : lu, cj = P == 0 ? lu : P == 1 ? le : lv, ck = P == 2 ? le : lv;
+        buf[(ck * TD_PT + cj) * TD_PT + ci] = p.phi[(long)(e0 + le) * p.NLX + lidx[r]];
     }
-    const TdPass& b = s.ps[Q];
-    line_of_cell(Q, i, j, k, u, v, e);
-    b.phi[(long)e * b.NLX + line_index(b, u, v)] = val;
-    if (P == 1 && Q == 2) {
-        const int halo[4] = {0, -1, s.nth + 1, s.nth + 2};
+}
+
+// Thread tid of nt: buf into pass Q's phi (Q's order and line table) or,
+// for Q = 3, the caller's array; after pass 1 also the theta halo planes.
+__host__ __device__ inline void move_tile_scatter(const TdSys& s, int P, int Q, const PackTile& t, const double* buf,
+                                                  const int* order, const int* lidx, int tid, int nt) {
+    if (Q < 3) {
+        const TdPass& p = s.ps[Q];
+        int nu, nv, ne;
+        pack_tile_extents(Q, t, nu, nv, ne);
+        const int R = nu * nv, e0 = Q == 0 ? t.i0 : Q == 1 ? t.j0 : t.k0;
+        for (int q = tid; q < ne * R; q += nt) {
+            const int le = q / R, r = q - le * R, o = order[r], lu = o & 255, lv = o >> 8;
+            const int ci = Q == 0 ? le : lu, cj = Q == 0 ? lu : Q == 1 ? le : lv, ck = Q == 2 ? le : lv;
+            p.phi[(long)(e0 + le) * p.NLX + lidx[r]] = buf[(ck * TD_PT + cj) * TD_PT + ci];
+        }
+    }
+    if (Q < 3 && !(P == 1 && Q == 2)) return;
+    const int halo[4] = {0, -1, s.nth + 1, s.nth + 2};
+    for (int q = tid; q < TD_PT3; q += nt) {
+        const int ci = q % TD_PT, cj = (q / TD_PT) % TD_PT, ck = q / (TD_PT * TD_PT);
+        if (ci >= t.ni || cj >= t.nj || ck >= t.nk) continue;
+        const int i = t.i0 + ci + 1, j = t.j0 + cj + 1, k = t.k0 + ck + 1;
+        if (Q == 3) {
+            s.home[tat(s, i, j, k)] = buf[q];
+            continue;
+        }
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (theta_src(halo[q], s.nth) == j) s.home[tat(s, i, halo[q], k)] = val;
+        for (int h = 0; h < 4; ++h)
+            if (theta_src(halo[h], s.nth) == j) s.home[tat(s, i, halo[h], k)] = buf[q];
     }
 }
 

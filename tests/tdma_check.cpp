@@ -243,6 +243,30 @@ static void replay_chain(const TdSys& s, int order) {
     std::free(lds);
 }
 
+// k_tdma_move<P, Q>: every tile (work-groups and threads in reverse), the
+// LDS buffer NaN-filled before the gather.
+static void replay_move(const TdSys& s, int P, int Q) {
+    std::vector<double> buf(TD_PT3);
+    std::vector<int> order(2 * TD_PT * TD_PT), lidx(2 * TD_PT * TD_PT);
+    const int NT = 512, TT = TD_PT * TD_PT;
+    PackTile t;
+    long tiles = 0;
+    while (pack_tile_of(s, tiles, t)) ++tiles;
+    for (long b = tiles - 1; b >= 0; --b) {
+        CHECK(pack_tile_of(s, b, t), "move: tile %ld", b);
+        std::fill(order.begin(), order.end(), -1);
+        std::fill(lidx.begin(), lidx.end(), -1);
+        for (int tid = NT - 1; tid >= 0; --tid) {
+            pack_tile_order(s, P, t, order.data(), lidx.data(), tid, NT);
+            if (Q < 3) pack_tile_order(s, Q, t, order.data() + TT, lidx.data() + TT, tid, NT);
+        }
+        std::fill(buf.begin(), buf.end(), std::nan(""));
+        for (int tid = NT - 1; tid >= 0; --tid) move_tile_gather(s, P, t, buf.data(), order.data(), lidx.data(), tid, NT);
+        for (int tid = NT - 1; tid >= 0; --tid)
+            move_tile_scatter(s, P, Q, t, buf.data(), order.data() + TT, lidx.data() + TT, tid, NT);
+    }
+}
+
 template <int P>
 static void run_pass(const TdSys& s, int order) {
     if (s.ps[P].rec && !s.off_band[P]) {
@@ -402,11 +426,11 @@ static void run_case(int nr, int nth, int nz, int n_sweep, unsigned seed, bool z
     for (long t = 0; t < lines; ++t) CHECK(setup_seen[(size_t)t] == 1, "setup line %ld seen %d times", t, setup_seen[(size_t)t]);
     for (int sweep = 0; sweep < n_sweep; ++sweep) {
         run_pass<0>(s, order);
-        for (long cc = ncells - 1; cc >= 0; --cc) move_cell(s, 0, 1, cc);
+        replay_move(s, 0, 1);
         run_pass<1>(s, order);
-        for (long cc = 0; cc < ncells; ++cc) move_cell(s, 1, 2, cc);
+        replay_move(s, 1, 2);
         run_pass<2>(s, order);
-        for (long cc = ncells - 1; cc >= 0; --cc) move_cell(s, 2, sweep + 1 < n_sweep ? 0 : 3, cc);
+        replay_move(s, 2, sweep + 1 < n_sweep ? 0 : 3);
     }
     if (n_sweep > 0)
         for (long t = (long)(nr + 4) * (nz + 4) - 1; t >= 0; --t) wall_halo_column(s, t);
