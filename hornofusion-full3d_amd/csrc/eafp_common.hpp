@@ -462,6 +462,42 @@ __device__ __forceinline__ void reduce_tiles(const double* partial, int t0, int 
     block_sum<NV>(out, sh);
 }
 
+// reduce_tiles in two steps, so that a kernel can issue the partials' loads
+// ahead of its own cell loads and add them after: each thread's first
+// RT_REG partials are loaded into registers, then added with the rest in the
+// order reduce_tiles adds them (the same bits).
+constexpr int RT_REG = 4;
+template <int NV>
+struct TilePre {
+    double v[RT_REG][NV];
+};
+template <int NV>
+__device__ __forceinline__ void reduce_tiles_load(const double* partial, int t0, int t1, TilePre<NV>& pre) {
+#pragma unroll
+    for (int k = 0; k < RT_REG; ++k) {
+        const int t = t0 + (int)threadIdx.x + k * BLOCK;
+#pragma unroll
+        for (int c = 0; c < NV; ++c) pre.v[k][c] = t < t1 ? partial[(long)t * NV + c] : 0.0;
+    }
+}
+template <int NV>
+__device__ __forceinline__ void reduce_tiles_sum(const double* partial, int t0, int t1, const TilePre<NV>& pre,
+                                                 double (&out)[NV], double* sh) {
+#pragma unroll
+    for (int c = 0; c < NV; ++c) out[c] = 0.0;
+#pragma unroll
+    for (int k = 0; k < RT_REG; ++k)
+        if (t0 + (int)threadIdx.x + k * BLOCK < t1) {
+#pragma unroll
+            for (int c = 0; c < NV; ++c) out[c] += pre.v[k][c];
+        }
+    for (int t = t0 + (int)threadIdx.x + RT_REG * BLOCK; t < t1; t += BLOCK) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) out[c] += partial[(long)t * NV + c];
+    }
+    block_sum<NV>(out, sh);
+}
+
 // The tiles of block b of a grid of G blocks (G a multiple of NXCD).  The
 // hardware deals blocks to the XCDs round robin, block b to XCD x = b % 8;
 // the table is cut into 8 contiguous chunks, chunk x for XCD x, and inside

@@ -987,11 +987,18 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
         rIn = wraw<F>(a.w[0], a.scale, base(dir > 0 ? i0 - 1 : i0) + qoff);
         issue(i0, A);
     };
+    // the partials' loads go out ahead of the cell loads, so they return first
+    const int pb0 = a.line ? C.lt_begin : C.tile, pb1 = a.line ? C.lt_end : C.tile_end;
+    TilePre<2> pre;
     MxDyn P;
     if (a.state_first) {
         P = a.dyn[1][s];
-        if (!(P.done || P.broken)) prologue();
+        if (!(P.done || P.broken)) {
+            reduce_tiles_load<2>(a.partB, pb0, pb1, pre);
+            prologue();
+        }
     } else {
+        reduce_tiles_load<2>(a.partB, pb0, pb1, pre);
         prologue();
         P = a.dyn[1][s];
     }
@@ -999,7 +1006,7 @@ __global__ __launch_bounds__(BLOCK) void k_mspmv_col(MxArgs a, int par) {
     bool skip = P.done || P.broken;
     if (!skip) {
         double tot[2];
-        reduce_tiles<2>(a.partB, a.line ? C.lt_begin : C.tile, a.line ? C.lt_end : C.tile_end, tot, sh);
+        reduce_tiles_sum<2>(a.partB, pb0, pb1, pre, tot, sh);
         if (!(tot[1] > 0.0)) {
             skip = true;
             N.broken = 1;
@@ -1557,11 +1564,16 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate_line(MxArgs a, int par) {
             C.stage((const float*)a.fac, G[4]);
         }
     };
+    // one system: the partials' loads go out ahead of the cell loads (a
+    // batch keeps the plain reduction after them: there the early loads
+    // measured 1.7 us slower per launch)
+    TilePre<1> pre;
     MxDyn P;
     if (a.state_first) {
         P = a.dyn[0][s];
         if (!(P.done || P.broken)) load_cells();
     } else {
+        reduce_tiles_load<1>(a.partA, L.tile_begin, L.tile_begin + L.pa_n, pre);
         load_cells();
         P = a.dyn[0][s];
     }
@@ -1569,7 +1581,8 @@ __global__ __launch_bounds__(BLOCK) void k_mupdate_line(MxArgs a, int par) {
     bool skip = P.done || P.broken;
     if (!skip) {
         double tot[1];
-        reduce_tiles<1>(a.partA, L.tile_begin, L.tile_begin + L.pa_n, tot, sh);   // syncs the block
+        if (a.state_first) reduce_tiles<1>(a.partA, L.tile_begin, L.tile_begin + L.pa_n, tot, sh);   // syncs the block
+        else reduce_tiles_sum<1>(a.partA, L.tile_begin, L.tile_begin + L.pa_n, pre, tot, sh);
         if (!(fabs(tot[0]) > SMALL)) {
             skip = true;
             N.broken = 1;
