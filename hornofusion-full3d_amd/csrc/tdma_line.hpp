@@ -445,8 +445,8 @@ __host__ __device__ inline void pack_tile_load(const TdSys& s, const PackTile& t
     }
 }
 
-// Where source `src` goes in pass P: the plane (PK_*), negated for a(e) =
-// -a_minus(e); -1 for phi outside pass 0.  Pass 0's phi goes to its phi rows.
+// Where coefficient `src` goes in pass P: its plane (PK_*), negated for
+// a(e) = -a_minus(e) (every coefficient has one plane in every pass).
 __host__ __device__ inline int pack_plane(int P, int src, bool& neg) {
     int am, ap, o[4];
     pass_slots(P, am, ap, o);
