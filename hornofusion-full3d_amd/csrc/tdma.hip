@@ -182,7 +182,7 @@ __device__ __forceinline__ void helper_chunk(const TdPass& p, const TdLds& L, lo
 #pragma unroll
     for (int i = 0; i < IPL; ++i) {
         if (h0 + i * HL < pcnt) chain_copy_out(p, L, true, f.w - 1, f.pv_lo, f.pm, f.pmM, f.pl0, f.e0, f.ce, h + i * HL);
-        else p.sink[(h + i * HL) & (TDC_BLOCK - 1)] = 0.0;
+        else p.sink[(h + i * HL) & (TDC_SINK - 1)] = 0.0;
     }
 #pragma unroll
     for (int i = 0; i < IPL; ++i) chain_load<P>(p, fl, h + i * HL, next[i]);
@@ -334,7 +334,7 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
             int nu, nv, n;
             pass_shape(d, P, nu, nv, n);
             const long NL = (long)nu * nv, NLX = NL + halo_lines(P, nu, nv);
-            dwork += PK_N * ncells + LN_N * NL + (long)n * NLX + (chain[P] ? (RC_REC + 2) * ncells + TDC_BLOCK + 2 : 0);
+            dwork += PK_N * ncells + LN_N * NL + (long)n * NLX + (chain[P] ? (RC_REC + 2) * ncells + TDC_SINK + 2 : 0);
             iwork += nu + nv - 1 + (chain[P] ? NL : 0);
             lines += NL;
         }
@@ -412,7 +412,7 @@ int tdma_solve_batch(int nsys, const eafp_desc* descs, const eafp_tdma_system* s
                 p.pp = (gdptr)dcur;
                 dcur += 2 * t.ncells;
                 p.sink = (gdptr)dcur;
-                dcur += TDC_BLOCK;
+                dcur += TDC_SINK;
                 p.uv = (decltype(p.uv))(icur + ioff);
                 ioff += p.NL;
             }

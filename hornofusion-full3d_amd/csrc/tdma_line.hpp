@@ -77,7 +77,7 @@ struct TdPass {
     gdptr phi;                 // n * NLX
     gdptr rec;                 // RC_REC * n * NL: the records' planes (16-byte aligned), or null: k_tdma_pass
     gdptr pp;                  // 2 * n * NL: RC_P2, RC_P4 of element-line x at 2x (16-byte aligned)
-    gdptr sink;                // TDC_BLOCK doubles the helpers' idle stores go to
+    gdptr sink;                // TDC_SINK doubles the helpers' idle stores go to
     TD_GLOBAL int* uv;         // line l's (u, v) as u | v << 16 (chain kernel), written by k_tdma_setup
     int C, K;                  // chain kernel: elements per chunk, chunks per line
     int ML, by_u;              // chain kernel: x slots (min(nu, nv)), slot = u-1 (by_u) or v-1
@@ -805,11 +805,12 @@ __host__ __device__ inline void wall_halo_column(const TdSys& s, long t) {
 // rotation), so loads have two chunks to arrive.  The d of chunk k is
 // formed after the previous diagonal's B and before F_k overwrites the
 // slots with d'.
-constexpr int TDC_BLOCK = 512;      // threads of a chain work-group
+constexpr int TDC_BLOCK = 768;      // threads of a chain work-group (three waves per SIMD)
+constexpr int TDC_SINK = 1024;      // doubles of a pass's sink, indexed modulo TDC_SINK
 constexpr int TDC_IPL = 3;          // ring entries a helper lane fills per chunk, at most
 constexpr int TDC_LDS = 20480;      // doubles of LDS (160 KiB)
 constexpr int TDC_G = 4;            // ring entries the chain reads per group
-static_assert((TDC_BLOCK & (TDC_BLOCK - 1)) == 0, "the sink index masks with TDC_BLOCK - 1");
+static_assert((TDC_SINK & (TDC_SINK - 1)) == 0, "the sink index masks with TDC_SINK - 1");
 
 // Plans the chain kernel for pass p: chunk length C (at most cmax), chunks
 // K, slots ML, slot direction, chain waves NW, ring entries per helper lane
@@ -1044,7 +1045,7 @@ __host__ __device__ inline void chain_copy_out(const TdPass& p, const TdLds& L, 
     const int e = e0 + tt;
     const int v = v_lo + j, u = w + 2 - v, sl = chain_slot(p, u, v);
     const double x = L.xb[td_mul24(sl > 0 ? sl : 0, L.LS) + e];
-    gdptr dst = real ? p.phi + (unsigned)(td_mul24(e, p.NLX) + l0 + j) : p.sink + (it & (TDC_BLOCK - 1));
+    gdptr dst = real ? p.phi + (unsigned)(td_mul24(e, p.NLX) + l0 + j) : p.sink + (it & (TDC_SINK - 1));
     *dst = x;
 }
 

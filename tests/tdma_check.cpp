@@ -350,7 +350,7 @@ static void run_case(int nr, int nth, int nz, int n_sweep, unsigned seed, bool z
             for (long o = 0; o < 2 * ncells; ++o) pp[o] = std::nan("");
             p.pp = (gdptr)pp;
             owned.push_back(pp);
-            double* sink = alloc<double>(TDC_BLOCK);
+            double* sink = alloc<double>(TDC_SINK);
             int* uv = alloc<int>(p.NL);
             for (int l = 0; l < p.NL; ++l) uv[l] = -1;
             p.rec = (gdptr)rec;
