@@ -835,6 +835,12 @@ __host__ __device__ inline bool chain_plan(TdPass& p, int cmax) {
     if (C > p.n) C = p.n;
     if (C > cmax) C = cmax;
     if (C < 1) return false;
+    // one ring entry per helper lane where that costs at most 3 more chunks
+    // per line (measured at 60x120x84: z lines in 8 chunks of 11 rather than
+    // 5 of up to 20 took 0.5 % off one system's sweep and 3 % off batches;
+    // r lines in 9 chunks of 7 rather than 5 of 12 were 1 % slower alone)
+    const long C1 = HL / ML;
+    if (C1 >= 1 && C1 < C && (p.n + C1 - 1) / C1 <= (p.n + C - 1) / C + 3) C = C1;
     // whole groups of the recurrences' reads where that costs no extra chunk
     const long Cg = C >= TDC_G ? C - C % TDC_G : C;
     if ((p.n + Cg - 1) / Cg == (p.n + C - 1) / C) C = Cg;
