@@ -470,14 +470,15 @@ int main() {
                              {1, 90, 90}};
     unsigned seed = 1;
     // each shape with k_tdma_pass (cmax 0) and the chain kernel, with chunks
-    // of at most 3 elements (several chunks, a short last one) and as planned
-    const int cmaxes[] = {0, 3, 1 << 20};
+    // of at most 3, 7 and 13 elements (several chunks, a short last one; one,
+    // two and four groups of the forward's reads) and as planned
+    const int cmaxes[] = {0, 3, 7, 13, 1 << 20};
     for (const auto& sh : shapes) {
         const bool big = (long)sh[0] * sh[1] * sh[2] > 100000;
         for (int cmax : cmaxes)
             for (int n_sweep = 1; n_sweep <= (big ? 1 : 3); ++n_sweep)
                 for (int order = 0; order < (big ? 1 : 2); ++order) {
-                    if (big && cmax == 3) continue;
+                    if (big && cmax > 0 && cmax < 1 << 20) continue;
                     run_case(sh[0], sh[1], sh[2], n_sweep, seed++, false, order, cmax);
                     if (!big) run_case(sh[0], sh[1], sh[2], n_sweep, seed++, true, order, cmax);
                 }
