@@ -155,11 +155,11 @@ __global__ __launch_bounds__(TD_BLOCK) void k_tdma_prep(const TdSys* __restrict_
     if (t < (long)p.n * p.NL) prep_elem<P>(p, (int)(t % p.NL), (int)(t / p.NL));
 }
 
-// One forward chunk f for a helper lane: wait at the barriers of the phases
-// up to the one before f's, fill f's ring slot from `use` (loaded two chunks
-// ago), copy the same chunk of the diagonal before out to phi (its last
-// reader was this fill; the chain overwrites it next), issue the loads of
-// chunk f+2 into `next`, and close the phase.  Past the pass's last chunk
+// One forward chunk f for a helper lane: issue the loads of chunk f+2 into
+// `next` (after the copy below with three entries per lane), wait at the barriers of the phases up to the one before f's, fill
+// f's ring slot from `use` (loaded two chunks ago), copy the same chunk of
+// the diagonal before out to phi (its last reader was this fill; the chain
+// overwrites it next), and close the phase.  Past the pass's last chunk
 // (`live` false, the same for every lane) and on the first diagonal the
 // copies go to the sink, and past the last chunk nothing is filled and no
 // barrier passed.  Every call issues the same stores and loads, so the
@@ -170,6 +170,13 @@ template <int P, int IPL>
 __device__ __forceinline__ void helper_chunk(const TdPass& p, const TdLds& L, long& X, TdTask& f, TdTask& fl, int h,
                                              int h0, int HL, bool live, const TdPre (&use)[IPL], TdPre (&next)[IPL]) {
     const long ph = chain_prep_phase(p, f);
+    // chunk fl's loads first: they depend on nothing this phase does, and
+    // they drain through the memory pipeline while the fill and the copy run
+    // (with three entries per lane the registers do not allow it)
+    if constexpr (IPL < 3) {
+#pragma unroll
+        for (int i = 0; i < IPL; ++i) chain_load<P>(p, fl, h + i * HL, next[i]);
+    }
     if (live) {
         for (; X < ph; ++X) __syncthreads();
         const int cnt = f.ce * f.m;
@@ -184,8 +191,10 @@ __device__ __forceinline__ void helper_chunk(const TdPass& p, const TdLds& L, lo
         if (h0 + i * HL < pcnt) chain_copy_out(p, L, true, f.w - 1, f.pv_lo, f.pm, f.pmM, f.pl0, f.e0, f.ce, h + i * HL);
         else p.sink[(h + i * HL) & (TDC_SINK - 1)] = 0.0;
     }
+    if constexpr (IPL >= 3) {
 #pragma unroll
-    for (int i = 0; i < IPL; ++i) chain_load<P>(p, fl, h + i * HL, next[i]);
+        for (int i = 0; i < IPL; ++i) chain_load<P>(p, fl, h + i * HL, next[i]);
+    }
     if (live) {
         __syncthreads();
         ++X;
