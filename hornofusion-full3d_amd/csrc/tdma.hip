@@ -156,10 +156,11 @@ __global__ __launch_bounds__(TD_BLOCK) void k_tdma_prep(const TdSys* __restrict_
 }
 
 // One forward chunk f for a helper lane: issue the loads of chunk f+2 into
-// `next` (after the copy below with three entries per lane), wait at the barriers of the phases up to the one before f's, fill
-// f's ring slot from `use` (loaded two chunks ago), copy the same chunk of
-// the diagonal before out to phi (its last reader was this fill; the chain
-// overwrites it next), and close the phase.  Past the pass's last chunk
+// `next` (at the end instead with three entries per lane), wait at the
+// barriers of the phases up to the one before f's, read the same chunk of
+// the diagonal before from the x slots (the chain overwrites it next), fill
+// f's ring slot from `use` (loaded two chunks ago), store the copy to phi,
+// and close the phase.  Past the pass's last chunk
 // (`live` false, the same for every lane) and on the first diagonal the
 // copies go to the sink, and past the last chunk nothing is filled and no
 // barrier passed.  Every call issues the same stores and loads, so the
@@ -177,20 +178,28 @@ __device__ __forceinline__ void helper_chunk(const TdPass& p, const TdLds& L, lo
 #pragma unroll
         for (int i = 0; i < IPL; ++i) chain_load<P>(p, fl, h + i * HL, next[i]);
     }
-    if (live) {
+    if (live)
         for (; X < ph; ++X) __syncthreads();
+    // the copy's LDS reads before the fill, its stores after it
+    const bool real = live && f.w > 0;
+    const int pcnt = real ? f.ce * f.pm : 0;
+    double cx[IPL];
+    gdptr cd[IPL];
+#pragma unroll
+    for (int i = 0; i < IPL; ++i) {
+        cx[i] = 0.0;
+        cd[i] = p.sink + ((h + i * HL) & (TDC_SINK - 1));
+        if (h0 + i * HL < pcnt)
+            chain_copy_read(p, L, true, f.w - 1, f.pv_lo, f.pm, f.pmM, f.pl0, f.e0, f.ce, h + i * HL, cx[i], cd[i]);
+    }
+    if (live) {
         const int cnt = f.ce * f.m;
 #pragma unroll
         for (int i = 0; i < IPL; ++i)
             if (h0 + i * HL < cnt) chain_prepare<P>(p, L, f, (int)((ph + 1) & 1), h + i * HL, use[i]);
     }
-    const bool real = live && f.w > 0;
-    const int pcnt = real ? f.ce * f.pm : 0;
 #pragma unroll
-    for (int i = 0; i < IPL; ++i) {
-        if (h0 + i * HL < pcnt) chain_copy_out(p, L, true, f.w - 1, f.pv_lo, f.pm, f.pmM, f.pl0, f.e0, f.ce, h + i * HL);
-        else p.sink[(h + i * HL) & (TDC_SINK - 1)] = 0.0;
-    }
+    for (int i = 0; i < IPL; ++i) *cd[i] = cx[i];
     if constexpr (IPL >= 3) {
 #pragma unroll
         for (int i = 0; i < IPL; ++i) chain_load<P>(p, fl, h + i * HL, next[i]);

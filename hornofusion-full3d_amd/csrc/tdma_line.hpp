@@ -1040,18 +1040,27 @@ __host__ __device__ inline void chain_prepare(const TdPass& p, const TdLds& L, c
 }
 
 // Entry `it` of chunk (e0, ce) of diagonal w's x (m lines from v_lo, first
-// line l0), from its x slot to the pass's phi row.  Entries past the chunk
-// store its last entry again (the same value to the same address), and with
-// `real` false the store goes to the sink, so every call issues one store.
-__host__ __device__ inline void chain_copy_out(const TdPass& p, const TdLds& L, bool real, int w, int v_lo, int m,
-                                               unsigned mM, int l0, int e0, int ce, int it) {
+// line l0), from its x slot to the pass's phi row (chain_copy_out; a helper
+// splits it into chain_copy_read, the value and its destination, and the
+// store).  Entries past the chunk store its last entry again (the same value
+// to the same address), and with `real` false the store goes to the sink,
+// so every copy issues one store.
+__host__ __device__ inline void chain_copy_read(const TdPass& p, const TdLds& L, bool real, int w, int v_lo, int m,
+                                                unsigned mM, int l0, int e0, int ce, int it, double& x, gdptr& dst) {
     const int cnt = ce * m, itc = it < cnt ? it : cnt - 1;
     int tt, j;
     td_divmod(itc, m, mM, tt, j);
     const int e = e0 + tt;
     const int v = v_lo + j, u = w + 2 - v, sl = chain_slot(p, u, v);
-    const double x = L.xb[td_mul24(sl > 0 ? sl : 0, L.LS) + e];
-    gdptr dst = real ? p.phi + (unsigned)(td_mul24(e, p.NLX) + l0 + j) : p.sink + (it & (TDC_SINK - 1));
+    x = L.xb[td_mul24(sl > 0 ? sl : 0, L.LS) + e];
+    dst = real ? p.phi + (unsigned)(td_mul24(e, p.NLX) + l0 + j) : p.sink + (it & (TDC_SINK - 1));
+}
+
+__host__ __device__ inline void chain_copy_out(const TdPass& p, const TdLds& L, bool real, int w, int v_lo, int m,
+                                               unsigned mM, int l0, int e0, int ce, int it) {
+    double x;
+    gdptr dst;
+    chain_copy_read(p, L, real, w, v_lo, m, mM, l0, e0, ce, it, x, dst);
     *dst = x;
 }
 
